@@ -1,0 +1,90 @@
+/* MI355X extensions of the rcabeam C ABI.
+ *
+ * These entry points have no counterpart in the reference's C header; each
+ * comment names the reference code whose job it takes over.  Conventions match
+ * include/rcabeam.h: device pointers, default (null) stream, asynchronous
+ * return after enqueue, launch/argument errors only, no allocation unless a
+ * comment says otherwise.
+ */
+#pragma once
+
+#include <stddef.h>
+
+#include "rcabeam.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Sample storage of packed channels or beamformed IQ. */
+typedef enum rcabeam_storage {
+    RCABEAM_STORAGE_C64 = 0,  /* complex64: two float32 */
+    RCABEAM_STORAGE_FP16 = 1, /* two IEEE binary16 (real, imaginary) */
+    RCABEAM_STORAGE_BF16 = 2, /* two bfloat16, real in the low half-word */
+} rcabeam_storage;
+
+/* Pack complex64 channels to complex BF16, same layout conversion and range
+ * flag semantics as rcabeam_pack_ensemble_fp16_device (non-finite components
+ * or components beyond the largest finite bfloat16 set *range_error = 1).
+ * Extends reference src/rcabeam/ensemble.cu:129-141 to bfloat16. */
+rcabeam_status rcabeam_pack_ensemble_bf16_device(const void* source, void* target, int* range_error, size_t ns,
+                                                 size_t nc, size_t na, size_t nt);
+
+/* OPW from packed BF16 RC/CR, float32 arithmetic; arguments as
+ * rcabeam_opw_fp16_device without the unused geometry workspace. */
+rcabeam_status rcabeam_opw_bf16_device(const void* rc, const void* cr, const float* scan, const float* xe,
+                                       const float* ye, const float* angles, const float* starts, float* pd,
+                                       void* signal, size_t ns, size_t nc, size_t na, size_t nt, size_t nv, float c,
+                                       float fs, float fd, float fn);
+
+/* ---- temporal clutter filter (replaces the CuPy/cuBLAS/cuSOLVER path of
+ * reference src/rcabeam/gpu_clutter.py:112-273) ----------------------------
+ * iq is a row-major Casorati matrix (rows, frames) of `storage` samples. */
+
+/* out[0] = bits of max(|Re|, |Im|) over n complex values, out[1] = 1 if any
+ * component is non-finite, out[2] = bits of 1 / out[0] (1 for all-zero input).
+ * accumulate != 0 folds another array into an existing out (max / or), as
+ * the reference's joint scale over an RC/CR tuple (gpu_clutter.py:141). */
+rcabeam_status rcabeam_iq_absmax_device(const void* iq, int storage, size_t n, unsigned* out, int accumulate);
+
+/* Workspace (bytes) and real-Gram size (doubles) for rcabeam_temporal_gram_device. */
+size_t rcabeam_gram_workspace_bytes(size_t rows, size_t frames);
+size_t rcabeam_gram_real_elements(size_t frames);
+
+/* Real Gram C = A^T A of A = (iq * inv_scale) viewed as (rows, 2*frames)
+ * reals, upper 32x32 blocks only, float64, row-major with leading dimension
+ * 32*ceil(2*frames/32); accumulate != 0 adds to real_gram.  complex64 input
+ * uses bfloat16 hi+lo split products on MFMA (inv_scale: device float, e.g.
+ * out + 2 of rcabeam_iq_absmax_device); FP16/BF16 input is exact in one
+ * product and ignores inv_scale.  frames <= 1024. */
+rcabeam_status rcabeam_temporal_gram_device(const void* iq, int storage, size_t rows, size_t frames,
+                                            const float* inv_scale, double* real_gram, float* workspace,
+                                            size_t workspace_bytes, int accumulate);
+
+/* Complex Hermitian Gram G = scale2 * X^H X (complex128, column-major
+ * frames x frames) from the real Gram above (reference gpu_clutter.py:160-161). */
+rcabeam_status rcabeam_gram_hermitian_device(const double* real_gram, size_t frames, double scale2, void* gram);
+
+/* Hermitian eigendecomposition in float64 on the GPU (rocSOLVER zheevd):
+ * eigenvalues ascending in values[frames], eigenvectors overwrite gram
+ * (columns).  workspace_e: frames doubles; info: one device int (0 = ok).
+ * The solver handle is created once per device on first use (the only
+ * allocation in this ABI). */
+rcabeam_status rcabeam_eigh_device(void* gram, double* values, double* workspace_e, int* info, size_t frames);
+
+/* Filtered power Doppler per row, float64, original frame count as
+ * denominator (reference gpu_clutter.py:244-273).  vectors: complex128
+ * column-major (frames, r).  subtract != 0: power of x - V V^H x (V = the
+ * discarded vectors); subtract == 0: sum_k |<x, v_k>|^2 / frames (V = the
+ * retained vectors).  Arithmetic after the load is float64. */
+rcabeam_status rcabeam_filtered_power_device(const void* iq, int storage, size_t rows, size_t frames,
+                                             const void* vectors, size_t r, int subtract, double* power);
+
+/* Filtered IQ (complex64) with the same projector convention
+ * (reference gpu_clutter.py:212-241). */
+rcabeam_status rcabeam_temporal_filter_device(const void* iq, int storage, size_t rows, size_t frames,
+                                              const void* vectors, size_t r, int subtract, void* out);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,28 @@
+"""MI355X-native OPW row-column beamforming (drop-in for the reference `rcabeam` hot path).
+
+Host API mirrors the reference package (src/rcabeam/__init__.py:3-49) for the
+beamforming / clutter-filter / power-Doppler path.  Every numerical entry
+point calls the HIP library through the C ABI (include/rcabeam.h,
+include/rcabeam_mi355x.h); there is no CPU fallback.
+"""
+
+from rcabeam_amd.ensemble import (
+    EnsembleTiming,
+    beamform_apertures,
+    beamform_ensemble,
+    ensemble_pd_from_channels,
+    opw_voxel_order,
+)
+from rcabeam_amd.sim import RCAGeometry, delay_rca_channel_data, delay_rca_channels, depth_axis
+
+__all__ = [
+    "EnsembleTiming",
+    "RCAGeometry",
+    "beamform_apertures",
+    "beamform_ensemble",
+    "delay_rca_channel_data",
+    "delay_rca_channels",
+    "depth_axis",
+    "ensemble_pd_from_channels",
+    "opw_voxel_order",
+]

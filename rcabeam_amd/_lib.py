@@ -1,0 +1,114 @@
+"""ctypes binding of the MI355X shared library (the C ABI in include/rcabeam.h and
+include/rcabeam_mi355x.h).
+
+The library is loaded lazily on first use and there is no fallback: a missing
+library or a missing GPU raises `RuntimeError` naming the cause.  Status codes
+map to exceptions the way the reference bindings do
+(src/rcabeam/bindings/python.cpp:15-25): argument errors become `ValueError`,
+runtime errors `RuntimeError`.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import threading
+from ctypes import POINTER, c_char_p, c_double, c_float, c_int, c_size_t, c_void_p
+from pathlib import Path
+
+LIBRARY_PATH = Path(__file__).resolve().parent / "lib" / "librcabeam_mi355x.so"
+
+SUCCESS, ERROR_DEVICE, ERROR_ARGUMENT = 0, 1, 2
+STORAGE = {"float32": 0, "complex64": 0, "float16": 1, "bfloat16": 2}
+
+_F = POINTER(c_float)
+_D = POINTER(c_double)
+_I = POINTER(c_int)
+
+# name -> argument types (all return rcabeam_status as int unless listed in _RESTYPE)
+_SIGNATURES: dict[str, list] = {
+    "rcabeam_status_string": [c_int],
+    "rcabeam_opw_device": [c_void_p, c_void_p, c_size_t, c_size_t, c_size_t],
+    "rcabeam_xdoppler_pd_device": [c_void_p, c_void_p] + [c_size_t] * 7,
+    "rcabeam_rc_fmas_pd_device": [c_void_p, c_void_p] + [c_size_t] * 7,
+    "rcabeam_delay_rca_channels_device": [c_void_p] * 6 + [c_size_t] * 4 + [c_int] + [c_float] * 4,
+    "rcabeam_delay_rca_channel_data_device": [c_void_p] * 8 + [c_size_t] * 4 + [c_int] + [c_float] * 4,
+    "rcabeam_opw_pd_from_channels_device": [c_void_p] * 8 + [c_size_t] * 4 + [c_float] * 4,
+    "rcabeam_xdoppler_pd_from_channels_device": [c_void_p] * 8 + [c_size_t] * 4 + [c_float] * 4,
+    "rcabeam_rc_fmas_pd_from_channels_device": [c_void_p] * 8 + [c_size_t] * 4 + [c_float] * 4,
+    "rcabeam_fast_pd_from_channels_device": [c_void_p] * 10 + [c_size_t] * 4 + [c_float] * 4,
+    "rcabeam_dmas_ccf_acf_from_channels_device": [c_void_p] * 8 + [c_size_t] * 4 + [c_float] * 4,
+    "rcabeam_pack_ensemble_device": [c_void_p, c_void_p] + [c_size_t] * 4,
+    "rcabeam_pack_ensemble_fp16_device": [c_void_p, c_void_p, c_void_p] + [c_size_t] * 4,
+    "rcabeam_pack_ensemble_bf16_device": [c_void_p, c_void_p, c_void_p] + [c_size_t] * 4,
+    "rcabeam_ensemble_device": [c_void_p] * 12 + [c_size_t] * 5 + [c_float] * 4 + [c_int, c_int],
+    "rcabeam_opw_fp16_device": [c_void_p] * 10 + [c_size_t] * 5 + [c_float] * 4,
+    "rcabeam_opw_bf16_device": [c_void_p] * 9 + [c_size_t] * 5 + [c_float] * 4,
+    "rcabeam_iq_absmax_device": [c_void_p, c_int, c_size_t, c_void_p, c_int],
+    "rcabeam_gram_workspace_bytes": [c_size_t, c_size_t],
+    "rcabeam_gram_real_elements": [c_size_t],
+    "rcabeam_temporal_gram_device": [c_void_p, c_int, c_size_t, c_size_t, c_void_p, c_void_p, c_void_p, c_size_t, c_int],
+    "rcabeam_gram_hermitian_device": [c_void_p, c_size_t, c_double, c_void_p],
+    "rcabeam_eigh_device": [c_void_p, c_void_p, c_void_p, c_void_p, c_size_t],
+    "rcabeam_filtered_power_device": [c_void_p, c_int, c_size_t, c_size_t, c_void_p, c_size_t, c_int, c_void_p],
+    "rcabeam_temporal_filter_device": [c_void_p, c_int, c_size_t, c_size_t, c_void_p, c_size_t, c_int, c_void_p],
+}
+_RESTYPE = {
+    "rcabeam_status_string": c_char_p,
+    "rcabeam_gram_workspace_bytes": c_size_t,
+    "rcabeam_gram_real_elements": c_size_t,
+}
+
+# Symbols declared in include/*.h that the library must export.
+EXPORTED = tuple(_SIGNATURES)
+
+_lock = threading.Lock()
+_lib: ctypes.CDLL | None = None
+
+
+def library_path() -> Path:
+    return LIBRARY_PATH
+
+
+def load(require_gpu: bool = True) -> ctypes.CDLL:
+    """Load the shared library (once).  With `require_gpu`, also require a HIP device."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not LIBRARY_PATH.exists():
+                raise RuntimeError(
+                    f"rcabeam MI355X library not built: {LIBRARY_PATH} is missing "
+                    "(run `python -m rcabeam_amd.build`)"
+                )
+            lib = ctypes.CDLL(str(LIBRARY_PATH))
+            for name, argtypes in _SIGNATURES.items():
+                fn = getattr(lib, name, None)
+                if fn is None:
+                    continue
+                fn.argtypes = argtypes
+                fn.restype = _RESTYPE.get(name, c_int)
+            _lib = lib
+    if require_gpu:
+        import torch
+
+        if not torch.cuda.is_available():
+            raise RuntimeError("rcabeam MI355X kernels need a HIP GPU; none is visible")
+    return _lib
+
+
+def status_string(status: int) -> str:
+    return load(require_gpu=False).rcabeam_status_string(status).decode()
+
+
+def check(status: int, what: str) -> None:
+    """Raise for a non-success status, as the reference bindings do."""
+    if status == SUCCESS:
+        return
+    message = f"{what}: {status_string(status)}"
+    if status == ERROR_ARGUMENT:
+        raise ValueError(message)
+    raise RuntimeError(message)
+
+
+def call(name: str, *args) -> None:
+    """Call an entry point by name and check its status."""
+    check(getattr(load(), name)(*args), name)

@@ -1,0 +1,513 @@
+// Temporal SVD clutter filter for MI355X: input scale, Casorati Gram on MFMA,
+// FP64 Hermitian eigendecomposition (rocSOLVER), and fused projector +
+// filtered power.  Replaces the CuPy/cuBLAS/cuSOLVER path of reference
+// src/rcabeam/gpu_clutter.py:112-273 (scaled Gram, eigh, smaller projector,
+// float64 residual power).
+//
+// Gram.  The complex Casorati matrix X (rows x T) is viewed as the real matrix
+// A (rows x 2T) with interleaved columns (Re x_t, Im x_t), which is exactly the
+// memory layout of complex data.  C = A^T A (2T x 2T, symmetric) gives
+//   Re G[t][u] = C[2t][2u] + C[2t+1][2u+1],  Im G[t][u] = C[2t][2u+1] - C[2t+1][2u].
+// C is tiled into 32x32 blocks; only the upper-triangular blocks are formed,
+// each on v_mfma_f32_32x32x16_{bf16,f16}.  complex64 input is scaled by
+// 1/max(|Re|,|Im|) and split into bfloat16 hi + lo (x = hi + lo to 2^-16
+// relative), accumulating hi*hi + hi*lo + lo*hi; FP16/BF16 input is exact in
+// one product.  Each workgroup sums a bounded run of rows in float32 (the
+// rounding of short runs stays ~1e-8 relative after the cross-workgroup sum)
+// and writes a float32 partial; partials are reduced in float64.
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <mutex>
+
+#include "common.hpp"
+#include "rcabeam_mi355x.h"
+
+namespace rcb {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+constexpr int kGramThreads = 512;             // 8 waves, 2 per SIMD
+constexpr int kGramWaves = kGramThreads / kWave;
+constexpr int kGramMaxBlocksPerWave = 12;
+constexpr int kGramBlocksPerPass = kGramWaves * kGramMaxBlocksPerWave;  // 96
+constexpr int kGramK = 16;                    // rows per MFMA step
+constexpr int kGramKPad = kGramK + 8;         // LDS column stride (bf16 elements, 48 bytes)
+constexpr int kGramRowsPerGroup = 2048;       // rows summed in float32 by one workgroup
+constexpr int kGramMaxColumnBlocks = 64;      // 2T <= 2048
+
+struct GramPlan {
+  int nb;                                     // column blocks of 32 (2T padded)
+  int wave_start[kGramWaves];
+  int wave_count[kGramWaves];
+  unsigned char bi[kGramBlocksPerPass];
+  unsigned char bj[kGramBlocksPerPass];
+  int pass_first;                             // index of this pass's first block in the full list
+  int pass_blocks;
+};
+
+enum GramMode { kSplitBf16 = 0, kExactF16 = 1, kExactBf16 = 2 };
+
+__device__ __forceinline__ unsigned short bf16_rn_bits(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return static_cast<unsigned short>(u >> 16);
+}
+__device__ __forceinline__ float bf16_bits_to_float(unsigned short b) { return __uint_as_float(static_cast<uint32_t>(b) << 16); }
+
+// MODE kSplitBf16: T = float2 (complex64), hi/lo bfloat16.
+// MODE kExactF16:  T = __half2, one fp16 product.
+// MODE kExactBf16: T = bf16x2, one bf16 product.
+template <int MODE, typename T>
+__global__ __launch_bounds__(kGramThreads) void k_gram(const T* __restrict__ x, long long rows, int frames,
+                                                      const float* __restrict__ inv_scale_ptr, GramPlan plan,
+                                                      float* __restrict__ partials) {
+  constexpr bool kSplit = MODE == kSplitBf16;
+  extern __shared__ __align__(16) unsigned short opnd[];  // [2 (hi, lo)][cols][kGramKPad]
+  const int cols = plan.nb * 32;
+  unsigned short* hi = opnd;
+  unsigned short* lo = opnd + cols * kGramKPad;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x / kWave;
+  const float inv_scale = kSplit ? *inv_scale_ptr : 1.0f;
+  const long long row0 = static_cast<long long>(blockIdx.x) * kGramRowsPerGroup;
+  const long long row_end = row0 + kGramRowsPerGroup < rows ? row0 + kGramRowsPerGroup : rows;
+  const int steps = static_cast<int>((row_end - row0 + kGramK - 1) / kGramK);
+
+  // Zero the padding columns once (they stay zero).
+  for (int idx = threadIdx.x; idx < (cols - 2 * frames) * kGramKPad; idx += blockDim.x) {
+    hi[2 * frames * kGramKPad + idx] = 0;
+    lo[2 * frames * kGramKPad + idx] = 0;
+  }
+
+  // Each thread stages up to kPer complex values of a 16-row step.
+  constexpr int kPer = 4;  // values staged per thread and pass
+  const int per_step = kGramK * frames;
+  floatx16 acc[kGramMaxBlocksPerWave];
+#pragma unroll
+  for (int b = 0; b < kGramMaxBlocksPerWave; ++b)
+#pragma unroll
+    for (int e = 0; e < 16; ++e) acc[b][e] = 0.f;
+
+  const int my_start = plan.wave_start[wave];
+  const int my_count = plan.wave_count[wave];
+
+  for (int s = 0; s < steps; ++s) {
+    const long long rbase = row0 + static_cast<long long>(s) * kGramK;
+    __syncthreads();
+    for (int base = 0; base < per_step; base += kPer * kGramThreads) {
+#pragma unroll
+      for (int u = 0; u < kPer; ++u) {
+        const int idx = base + u * kGramThreads + threadIdx.x;
+        if (idx < per_step) {
+          const int k = idx / frames, t = idx - k * frames;
+          const long long r = rbase + k;
+          float2 v = make_float2(0.f, 0.f);
+          if (r < row_end) v = load_sample(x + r * frames + t);
+          unsigned short hr, hi_i, lr = 0, li = 0;
+          if constexpr (MODE == kExactF16) {
+            hr = __half_as_ushort(__float2half_rn(v.x));
+            hi_i = __half_as_ushort(__float2half_rn(v.y));
+          } else {
+            v.x *= inv_scale;
+            v.y *= inv_scale;
+            hr = bf16_rn_bits(v.x);
+            hi_i = bf16_rn_bits(v.y);
+            if constexpr (kSplit) {
+              lr = bf16_rn_bits(v.x - bf16_bits_to_float(hr));
+              li = bf16_rn_bits(v.y - bf16_bits_to_float(hi_i));
+            }
+          }
+          hi[(2 * t) * kGramKPad + k] = hr;
+          hi[(2 * t + 1) * kGramKPad + k] = hi_i;
+          if constexpr (kSplit) {
+            lo[(2 * t) * kGramKPad + k] = lr;
+            lo[(2 * t + 1) * kGramKPad + k] = li;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    const int koff = 8 * (lane / 32);
+    int prev = -1;
+    bf16x8 a_hi, a_lo;
+#pragma unroll
+    for (int b = 0; b < kGramMaxBlocksPerWave; ++b) {
+      if (b < my_count) {
+        const int I = plan.bi[my_start + b];
+        const int J = plan.bj[my_start + b];
+        if (I != prev) {
+          a_hi = *reinterpret_cast<const bf16x8*>(hi + (32 * I + (lane & 31)) * kGramKPad + koff);
+          if constexpr (kSplit) a_lo = *reinterpret_cast<const bf16x8*>(lo + (32 * I + (lane & 31)) * kGramKPad + koff);
+          prev = I;
+        }
+        const bf16x8 b_hi = *reinterpret_cast<const bf16x8*>(hi + (32 * J + (lane & 31)) * kGramKPad + koff);
+        if constexpr (MODE == kExactF16) {
+          acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8, a_hi),
+                                                          __builtin_bit_cast(f16x8, b_hi), acc[b], 0, 0, 0);
+        } else {
+          acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_hi, b_hi, acc[b], 0, 0, 0);
+          if constexpr (kSplit) {
+            const bf16x8 b_lo = *reinterpret_cast<const bf16x8*>(lo + (32 * J + (lane & 31)) * kGramKPad + koff);
+            acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_hi, b_lo, acc[b], 0, 0, 0);
+            acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_lo, b_hi, acc[b], 0, 0, 0);
+          }
+        }
+      }
+    }
+  }
+  // Partials: [group][pass block][lane][16]
+  float* out = partials + (static_cast<long long>(blockIdx.x) * plan.pass_blocks) * 1024;
+#pragma unroll
+  for (int b = 0; b < kGramMaxBlocksPerWave; ++b) {
+    if (b < my_count) {
+      float* dst = out + static_cast<long long>(my_start + b) * 1024 + lane * 16;
+#pragma unroll
+      for (int e = 0; e < 16; ++e) dst[e] = acc[b][e];
+    }
+  }
+}
+
+// Sum the float32 partials of all row groups in float64 into the real Gram
+// C (ldc = 32 nb, row-major, upper blocks only), optionally accumulating.
+__global__ void k_gram_reduce(const float* __restrict__ partials, int groups, GramPlan plan,
+                              double* __restrict__ c, int accumulate) {
+  const long long idx = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (idx >= static_cast<long long>(plan.pass_blocks) * 1024) return;
+  const int b = static_cast<int>(idx / 1024);
+  const int e = static_cast<int>(idx % 1024);
+  double sum = 0.0;
+  for (int g = 0; g < groups; ++g) sum += static_cast<double>(partials[(static_cast<long long>(g) * plan.pass_blocks + b) * 1024 + e]);
+  const int l = e / 16, r = e % 16;
+  // 32x32 MFMA accumulator layout: lane l, register r -> (row, col).
+  const int i = 8 * (r / 4) + 4 * (l / 32) + (r % 4);
+  const int j = l % 32;
+  const int I = plan.bi[b], J = plan.bj[b];
+  const long long ldc = 32LL * plan.nb;
+  double* dst = c + (32LL * I + i) * ldc + 32LL * J + j;
+  *dst = accumulate ? *dst + sum : sum;
+}
+
+__device__ __forceinline__ double c_at(const double* c, long long ldc, int i, int j) {
+  return (i / 32 <= j / 32) ? c[i * ldc + j] : c[j * ldc + i];
+}
+
+// G (complex128, column-major T x T) = scale2 * complex Gram from C.
+__global__ void k_gram_form(const double* __restrict__ c, int nb, int frames, double scale2, double2* __restrict__ g) {
+  const long long idx = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (idx >= static_cast<long long>(frames) * frames) return;
+  const int t = static_cast<int>(idx % frames);  // row
+  const int u = static_cast<int>(idx / frames);  // column
+  const long long ldc = 32LL * nb;
+  const double re = c_at(c, ldc, 2 * t, 2 * u) + c_at(c, ldc, 2 * t + 1, 2 * u + 1);
+  const double im = c_at(c, ldc, 2 * t, 2 * u + 1) - c_at(c, ldc, 2 * t + 1, 2 * u);
+  g[idx] = make_double2(re * scale2, im * scale2);
+}
+
+// ---- max(|Re|, |Im|) and finiteness ---------------------------------------
+__global__ void k_zero2(unsigned* out) {
+  if (threadIdx.x < 3) out[threadIdx.x] = 0u;
+}
+
+// out[2] = bits of 1 / max (1 when the input is all zero).
+__global__ void k_inverse_scale(unsigned* out) {
+  if (threadIdx.x == 0) {
+    const float m = __uint_as_float(out[0]);
+    out[2] = __float_as_uint(m > 0.f ? 1.0f / m : 1.0f);
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_absmax(const T* __restrict__ x, long long n, unsigned* out) {
+  float m = 0.f;
+  bool bad = false;
+  for (long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<long long>(gridDim.x) * blockDim.x) {
+    const float2 v = load_sample(x + i);
+    bad |= !isfinite(v.x) || !isfinite(v.y);
+    m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, kWave));
+  const bool any_bad = __any(bad);
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    atomicMax(out, __float_as_uint(m));
+    if (any_bad) atomicOr(out + 1, 1u);
+  }
+}
+
+// ---- projector + filtered power ---------------------------------------------
+// One wave per Casorati row.  coefficients c_k = sum_t x_t conj(V[t,k]) and
+// the residual / retained energy are formed in float64 (the operation is
+// HBM-bound; float64 removes the cancellation of x - P x for strong clutter).
+template <typename T, bool FILTER, int kMaxPerLane>
+__global__ __launch_bounds__(256) void k_project(const T* __restrict__ x, long long rows, int frames,
+                                                 const double2* __restrict__ v, int r, int subtract,
+                                                 double* __restrict__ power, float2* __restrict__ filtered) {
+  const long long row = static_cast<long long>(blockIdx.x) * 4 + threadIdx.x / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
+  if (row >= rows) return;
+  const T* xr = x + row * frames;
+  double2 xv[kMaxPerLane];
+#pragma unroll
+  for (int j = 0; j < kMaxPerLane; ++j) {
+    const int t = lane + kWave * j;
+    const float2 s = t < frames ? load_sample(xr + t) : make_float2(0.f, 0.f);
+    xv[j] = make_double2(s.x, s.y);
+  }
+  double2 resid[kMaxPerLane];
+#pragma unroll
+  for (int j = 0; j < kMaxPerLane; ++j) resid[j] = subtract ? xv[j] : make_double2(0.0, 0.0);
+  double kept = 0.0;
+  for (int k = 0; k < r; ++k) {
+    const double2* vk = v + static_cast<long long>(k) * frames;
+    double cr = 0.0, ci = 0.0;
+#pragma unroll
+    for (int j = 0; j < kMaxPerLane; ++j) {
+      const int t = lane + kWave * j;
+      if (t < frames) {
+        const double2 b = vk[t];
+        cr += xv[j].x * b.x + xv[j].y * b.y;  // x * conj(b)
+        ci += xv[j].y * b.x - xv[j].x * b.y;
+      }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      cr += __shfl_xor(cr, off, kWave);
+      ci += __shfl_xor(ci, off, kWave);
+    }
+    if (!subtract && !FILTER) {
+      kept += cr * cr + ci * ci;
+      continue;
+    }
+    const double sign = subtract ? -1.0 : 1.0;
+#pragma unroll
+    for (int j = 0; j < kMaxPerLane; ++j) {
+      const int t = lane + kWave * j;
+      if (t < frames) {
+        const double2 b = vk[t];
+        resid[j].x += sign * (cr * b.x - ci * b.y);
+        resid[j].y += sign * (cr * b.y + ci * b.x);
+      }
+    }
+  }
+  if constexpr (FILTER) {
+#pragma unroll
+    for (int j = 0; j < kMaxPerLane; ++j) {
+      const int t = lane + kWave * j;
+      if (t < frames) filtered[row * frames + t] = make_float2(static_cast<float>(resid[j].x), static_cast<float>(resid[j].y));
+    }
+  } else {
+    double e = 0.0;
+    if (subtract || FILTER) {
+#pragma unroll
+      for (int j = 0; j < kMaxPerLane; ++j) e += resid[j].x * resid[j].x + resid[j].y * resid[j].y;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) e += __shfl_xor(e, off, kWave);
+    } else {
+      e = kept;
+    }
+    if (lane == 0) power[row] = e / frames;
+  }
+}
+
+// ---- host helpers -----------------------------------------------------------
+
+static int column_blocks(size_t frames) { return static_cast<int>((2 * frames + 31) / 32); }
+
+static long long triangle_blocks(int nb) { return static_cast<long long>(nb) * (nb + 1) / 2; }
+
+static void make_plan(int nb, int pass, GramPlan& plan) {
+  plan.nb = nb;
+  const long long total = triangle_blocks(nb);
+  const long long first = static_cast<long long>(pass) * kGramBlocksPerPass;
+  const int count = static_cast<int>(total - first < kGramBlocksPerPass ? total - first : kGramBlocksPerPass);
+  plan.pass_first = static_cast<int>(first);
+  plan.pass_blocks = count;
+  long long idx = 0;
+  int filled = 0;
+  for (int I = 0; I < nb; ++I)
+    for (int J = I; J < nb; ++J, ++idx)
+      if (idx >= first && idx < first + count) {
+        plan.bi[filled] = static_cast<unsigned char>(I);
+        plan.bj[filled] = static_cast<unsigned char>(J);
+        ++filled;
+      }
+  // Contiguous runs of the row-major triangle: consecutive blocks mostly share I.
+  const int base = count / kGramWaves, extra = count % kGramWaves;
+  int start = 0;
+  for (int w = 0; w < kGramWaves; ++w) {
+    const int n = base + (w < extra ? 1 : 0);
+    plan.wave_start[w] = start;
+    plan.wave_count[w] = n;
+    start += n;
+  }
+}
+
+static size_t lds_bytes(int nb) { return static_cast<size_t>(2) * nb * 32 * kGramKPad * sizeof(unsigned short); }
+
+template <int MODE, typename T>
+static hipError_t gram_launch(const void* x, long long rows, int frames, const float* inv_scale, double* c,
+                              float* workspace, int accumulate) {
+  const int nb = column_blocks(frames);
+  const long long groups = (rows + kGramRowsPerGroup - 1) / kGramRowsPerGroup;
+  const int passes = static_cast<int>((triangle_blocks(nb) + kGramBlocksPerPass - 1) / kGramBlocksPerPass);
+  const size_t lds = lds_bytes(nb);
+  if (lds > 160 * 1024) return hipErrorInvalidValue;
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_gram<MODE, T>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+  if (e != hipSuccess) return e;
+  for (int pass = 0; pass < passes; ++pass) {
+    GramPlan plan;
+    make_plan(nb, pass, plan);
+    k_gram<MODE, T><<<dim3(static_cast<unsigned>(groups)), kGramThreads, lds>>>(
+        static_cast<const T*>(x), rows, frames, inv_scale, plan, workspace);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    const long long n = static_cast<long long>(plan.pass_blocks) * 1024;
+    k_gram_reduce<<<dim3(static_cast<unsigned>((n + 255) / 256)), 256>>>(workspace, static_cast<int>(groups), plan, c,
+                                                                          accumulate);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+struct SolverState {
+  std::mutex lock;
+  rocblas_handle handle[64] = {};
+};
+static SolverState g_solver;
+
+}  // namespace rcb
+
+using namespace rcb;
+
+extern "C" size_t rcabeam_gram_workspace_bytes(size_t rows, size_t frames) {
+  if (!rows || !frames || 2 * frames > 32 * kGramMaxColumnBlocks) return 0;
+  const int nb = column_blocks(frames);
+  const long long groups = (static_cast<long long>(rows) + kGramRowsPerGroup - 1) / kGramRowsPerGroup;
+  const long long per_pass = triangle_blocks(nb) < kGramBlocksPerPass ? triangle_blocks(nb) : kGramBlocksPerPass;
+  return static_cast<size_t>(groups * per_pass * 1024 * sizeof(float));
+}
+
+extern "C" size_t rcabeam_gram_real_elements(size_t frames) {
+  const size_t ld = 32 * static_cast<size_t>(column_blocks(frames));
+  return ld * ld;
+}
+
+extern "C" rcabeam_status rcabeam_iq_absmax_device(const void* iq, int storage, size_t n, unsigned* out,
+                                                   int accumulate) {
+  if (!iq || !out || !n || storage < 0 || storage > 2) return RCABEAM_ERROR_ARGUMENT;
+  if (!accumulate) k_zero2<<<1, 64>>>(out);
+  const long long blocks = static_cast<long long>((n + 255) / 256) < 4096 ? static_cast<long long>((n + 255) / 256) : 4096;
+  const unsigned grid = static_cast<unsigned>(blocks);
+  switch (storage) {
+    case RCABEAM_STORAGE_C64:
+      k_absmax<float2><<<grid, 256>>>(static_cast<const float2*>(iq), static_cast<long long>(n), out);
+      break;
+    case RCABEAM_STORAGE_FP16:
+      k_absmax<__half2><<<grid, 256>>>(static_cast<const __half2*>(iq), static_cast<long long>(n), out);
+      break;
+    default:
+      k_absmax<bf16x2><<<grid, 256>>>(static_cast<const bf16x2*>(iq), static_cast<long long>(n), out);
+  }
+  k_inverse_scale<<<1, 64>>>(out);
+  return launch_status();
+}
+
+extern "C" rcabeam_status rcabeam_temporal_gram_device(const void* iq, int storage, size_t rows, size_t frames,
+                                                       const float* inv_scale, double* real_gram, float* workspace,
+                                                       size_t workspace_bytes, int accumulate) {
+  if (!iq || !real_gram || !workspace || !rows || !frames || storage < 0 || storage > 2) return RCABEAM_ERROR_ARGUMENT;
+  if (storage == RCABEAM_STORAGE_C64 && !inv_scale) return RCABEAM_ERROR_ARGUMENT;
+  const size_t need = rcabeam_gram_workspace_bytes(rows, frames);
+  if (!need || workspace_bytes < need) return RCABEAM_ERROR_ARGUMENT;
+  hipError_t e;
+  const long long n = static_cast<long long>(rows);
+  const int t = static_cast<int>(frames);
+  switch (storage) {
+    case RCABEAM_STORAGE_C64:
+      e = gram_launch<kSplitBf16, float2>(iq, n, t, inv_scale, real_gram, workspace, accumulate);
+      break;
+    case RCABEAM_STORAGE_FP16:
+      e = gram_launch<kExactF16, __half2>(iq, n, t, inv_scale, real_gram, workspace, accumulate);
+      break;
+    default:
+      e = gram_launch<kExactBf16, bf16x2>(iq, n, t, inv_scale, real_gram, workspace, accumulate);
+  }
+  if (e == hipErrorInvalidValue) return RCABEAM_ERROR_ARGUMENT;
+  return e == hipSuccess ? RCABEAM_SUCCESS : RCABEAM_ERROR_CUDA;
+}
+
+extern "C" rcabeam_status rcabeam_gram_hermitian_device(const double* real_gram, size_t frames, double scale2,
+                                                        void* gram) {
+  if (!real_gram || !gram || !frames) return RCABEAM_ERROR_ARGUMENT;
+  const long long n = static_cast<long long>(frames) * static_cast<long long>(frames);
+  k_gram_form<<<dim3(static_cast<unsigned>((n + 255) / 256)), 256>>>(real_gram, column_blocks(frames),
+                                                                     static_cast<int>(frames), scale2,
+                                                                     static_cast<double2*>(gram));
+  return launch_status();
+}
+
+extern "C" rcabeam_status rcabeam_eigh_device(void* gram, double* values, double* workspace_e, int* info,
+                                              size_t frames) {
+  if (!gram || !values || !workspace_e || !info || !frames || frames > 0x7fffffff) return RCABEAM_ERROR_ARGUMENT;
+  int device = 0;
+  if (hipGetDevice(&device) != hipSuccess || device < 0 || device >= 64) return RCABEAM_ERROR_CUDA;
+  rocblas_handle handle;
+  {
+    std::lock_guard<std::mutex> guard(g_solver.lock);
+    if (!g_solver.handle[device]) {
+      if (rocblas_create_handle(&g_solver.handle[device]) != rocblas_status_success) return RCABEAM_ERROR_CUDA;
+      rocblas_set_stream(g_solver.handle[device], nullptr);
+    }
+    handle = g_solver.handle[device];
+  }
+  const rocblas_int n = static_cast<rocblas_int>(frames);
+  const rocblas_status s =
+      rocsolver_zheevd(handle, rocblas_evect_original, rocblas_fill_upper, n,
+                       static_cast<rocblas_double_complex*>(gram), n, values, workspace_e, info);
+  if (s == rocblas_status_invalid_size || s == rocblas_status_invalid_pointer) return RCABEAM_ERROR_ARGUMENT;
+  return s == rocblas_status_success ? RCABEAM_SUCCESS : RCABEAM_ERROR_CUDA;
+}
+
+template <bool FILTER>
+static rcabeam_status project(const void* iq, int storage, size_t rows, size_t frames, const void* vectors, size_t r,
+                              int subtract, double* power, void* filtered) {
+  if (!iq || !rows || !frames || frames > 1024 || (r && !vectors) || storage < 0 || storage > 2)
+    return RCABEAM_ERROR_ARGUMENT;
+  const unsigned grid = static_cast<unsigned>((rows + 3) / 4);
+  const long long n = static_cast<long long>(rows);
+  const int t = static_cast<int>(frames);
+  const double2* v = static_cast<const double2*>(vectors);
+  float2* out = static_cast<float2*>(filtered);
+  const int ri = static_cast<int>(r);
+#define RCB_PROJECT(TYPE, MPL) \
+  k_project<TYPE, FILTER, MPL><<<grid, 256>>>(static_cast<const TYPE*>(iq), n, t, v, ri, subtract, power, out)
+  const bool small = frames <= 4 * kWave;
+  switch (storage) {
+    case RCABEAM_STORAGE_C64:
+      if (small) RCB_PROJECT(float2, 4); else RCB_PROJECT(float2, 16);
+      break;
+    case RCABEAM_STORAGE_FP16:
+      if (small) RCB_PROJECT(__half2, 4); else RCB_PROJECT(__half2, 16);
+      break;
+    default:
+      if (small) RCB_PROJECT(bf16x2, 4); else RCB_PROJECT(bf16x2, 16);
+  }
+#undef RCB_PROJECT
+  return launch_status();
+}
+
+extern "C" rcabeam_status rcabeam_filtered_power_device(const void* iq, int storage, size_t rows, size_t frames,
+                                                        const void* vectors, size_t r, int subtract, double* power) {
+  if (!power) return RCABEAM_ERROR_ARGUMENT;
+  return project<false>(iq, storage, rows, frames, vectors, r, subtract, power, nullptr);
+}
+
+extern "C" rcabeam_status rcabeam_temporal_filter_device(const void* iq, int storage, size_t rows, size_t frames,
+                                                         const void* vectors, size_t r, int subtract, void* out) {
+  if (!out) return RCABEAM_ERROR_ARGUMENT;
+  return project<true>(iq, storage, rows, frames, vectors, r, subtract, nullptr, out);
+}

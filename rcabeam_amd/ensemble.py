@@ -1,0 +1,233 @@
+"""Frame-batched RCA reconstruction on MI355X.
+
+Public functions mirror reference src/rcabeam/ensemble.py: `opw_voxel_order`
+(:24-49), `ensemble_pd_from_channels` (:133-218), `beamform_ensemble`
+(:221-270), `beamform_apertures` (:273-287), with the same validation messages
+(:66-84), broadcasting of per-angle start times (:82) and St-SW normalisation
+(:127-129).  The staging that the reference binding does in C++
+(src/rcabeam/bindings/python.cpp:495-571: upload, pack, voxel tiles,
+download) happens here over the C ABI with torch-owned device buffers.
+
+Inputs may be NumPy arrays (uploaded, results returned as NumPy, as in the
+reference) or torch tensors already on the GPU (results stay on the GPU).
+"""
+
+from __future__ import annotations
+
+from time import perf_counter
+from typing import Literal, TypedDict
+
+import numpy as np
+from numpy.typing import NDArray
+
+from rcabeam_amd import _device as dev
+from rcabeam_amd import _lib
+from rcabeam_amd.sim import RCAGeometry, _scan_coords
+
+Method = Literal["opw", "xdoppler", "rc_fmas", "dmas", "st_sw"]
+IQStorage = Literal["float32", "float16", "bfloat16"]
+_METHODS = {"opw": 0, "xdoppler": 1, "rc_fmas": 2, "dmas": 3, "st_sw": 4, "apertures": 5}
+_RANGE_MESSAGE = {
+    "float16": "FP16 IQ requires finite components within +/-65504; rescale inputs explicitly",
+    "bfloat16": "BF16 IQ requires finite components within the bfloat16 range; rescale inputs explicitly",
+}
+# Bound on the St-SW correlation workspace per launch (voxel tiles beyond it).
+_STSW_WORKSPACE_BYTES = 1 << 28
+
+
+class EnsembleTiming(TypedDict):
+    """Host wall time for completed raw RC/CR uploads, excluding allocation."""
+
+    raw_upload_seconds: float
+
+
+def opw_voxel_order(shape: tuple[int, int, int]) -> NDArray[np.intp]:
+    """Voxel ids in 2 (x) by 4 (y) lateral patches, tails kept (ensemble.py:24-49).
+
+    Scatter values computed in this order into these ids to restore `(x, z, y)`.
+    """
+    if min(shape) < 1:
+        raise ValueError("Grid dimensions must be positive")
+    nx, nz, ny = shape
+    ids = np.arange(nx * nz * ny, dtype=np.intp).reshape(shape)
+    ex, ey = nx - nx % 2, ny - ny % 4
+    patches = ids[:ex, :, :ey].reshape(ex // 2, 2, nz, ey // 4, 4).transpose(0, 2, 3, 1, 4).ravel()
+    return np.concatenate((patches, ids[:ex, :, ey:].ravel(), ids[ex:, :, :].ravel()))
+
+
+def _validate(rc, cr, angles, grid, geom: RCAGeometry, method: str, k: int, return_signal: bool, iq_storage: str):
+    if method not in _METHODS or (method == "apertures" and not return_signal):
+        raise ValueError(f"Unknown method: {method}")
+    if iq_storage not in ("float32", "float16", "bfloat16") or (iq_storage != "float32" and method != "opw"):
+        raise ValueError("IQ storage must be float32, or float16/bfloat16 for OPW only")
+    shape = tuple(rc.shape)
+    if len(shape) != 4 or tuple(cr.shape) != shape or any(n == 0 for n in shape) or shape[0] < 2:
+        raise ValueError("RC/CR must have matching nonempty (samples, channels, angles, frames) shapes")
+    if len(angles) != shape[2] or len(geom.x_el) != shape[1] or len(geom.y_el) != shape[1]:
+        raise ValueError("Geometry must match the channel/angle dimensions")
+    if any(np.ndim(axis) != 1 or not len(axis) or not np.isfinite(axis).all() for axis in grid):
+        raise ValueError("Grid axes must be nonempty finite coordinate vectors")
+    if not np.isfinite([geom.c, geom.fs, geom.f_demod]).all() or geom.c <= 0 or geom.fs <= 0:
+        raise ValueError("Sound speed and sampling frequency must be finite and positive")
+    if geom.fnumber is not None and (not np.isfinite(geom.fnumber) or geom.fnumber <= 0):
+        raise ValueError("F-number must be positive or None")
+    if method == "st_sw" and (k < 2 or k > 4 or len(angles) < k):
+        raise ValueError("St-SW requires 2 <= k <= 4 and at least k angles")
+
+
+def pack_channels(raw, storage: str, packed=None, range_error=None):
+    """Pack one complex64 device ensemble `(ns, nc, na, nt)` into the frame-inner layout.
+
+    Returns the packed device buffer (complex64, or int32 words holding two
+    16-bit components).  FP16/BF16 range errors raise `ValueError`.
+    """
+    t = dev.torch()
+    ns, nc, na, nt = raw.shape
+    if packed is None:
+        packed = dev.empty((nc, na, ns, nt), t.complex64 if storage == "float32" else t.int32)
+    if storage == "float32":
+        _lib.call("rcabeam_pack_ensemble_device", dev.ptr(raw), dev.ptr(packed), ns, nc, na, nt)
+        return packed
+    if range_error is None:
+        range_error = dev.empty((1,), t.int32)
+    range_error.zero_()
+    name = "rcabeam_pack_ensemble_fp16_device" if storage == "float16" else "rcabeam_pack_ensemble_bf16_device"
+    _lib.call(name, dev.ptr(raw), dev.ptr(packed), dev.ptr(range_error), ns, nc, na, nt)
+    if int(range_error.item()):
+        raise ValueError(_RANGE_MESSAGE[storage])
+    return packed
+
+
+class _Geometry:
+    """Float32 device copies of everything the ensemble entry points read."""
+
+    def __init__(self, angles, t_start, grid, geom: RCAGeometry, order=None):
+        t = dev.torch()
+        na = len(angles)
+        starts = np.ascontiguousarray(np.broadcast_to(np.asarray(t_start, dtype=np.float32), (na,)))
+        vectors = [np.asarray(v, dtype=np.float32) for v in (geom.x_el, geom.y_el, angles)]
+        if not all(np.isfinite(v).all() for v in (*vectors, starts)):
+            raise ValueError("Geometry and sample starts must be finite")
+        scan = _scan_coords(grid)
+        if order is not None:
+            scan = np.ascontiguousarray(scan[order])
+        self.nv = len(scan)
+        self.scan = dev.to_device(scan, t.float32)
+        self.xe, self.ye, self.angles = (dev.to_device(v, t.float32) for v in vectors)
+        self.starts = dev.to_device(starts, t.float32)
+        self.constants = (float(geom.c), float(geom.fs), float(geom.f_demod), -1.0 if geom.fnumber is None else float(geom.fnumber))
+        # The reference ABI requires a non-null geometry workspace; it is unused here.
+        self.geometry_workspace = dev.empty((16,), t.uint8)
+
+
+def run_ensemble(packed_rc, packed_cr, g: _Geometry, shape4, method: int, k: int, storage: str, pd, signal,
+                 weights=None, first: int = 0, count: int | None = None):
+    """Enqueue one reconstruction of voxels [first, first + count) over the C ABI."""
+    t = dev.torch()
+    ns, nc, na, nt = shape4
+    count = g.nv - first if count is None else count
+    c, fs, fd, fn = g.constants
+    scan = g.scan[first : first + count]
+    pd_view = pd[first : first + count]
+    planes = 2 if method == _METHODS["apertures"] else 1
+    sig_view = None if signal is None else signal[first * planes : (first + count) * planes]
+    common = (dev.ptr(g.xe), dev.ptr(g.ye), dev.ptr(g.angles), dev.ptr(g.starts), dev.ptr(pd_view), dev.ptr(sig_view))
+    if storage == "float32":
+        workspace = None
+        w_view = None
+        if method == _METHODS["st_sw"]:
+            workspace = dev.empty((count * k * k * nt,), t.complex64)
+            w_view = weights[first : first + count]
+        _lib.call(
+            "rcabeam_ensemble_device",
+            dev.ptr(packed_rc), dev.ptr(packed_cr), dev.ptr(scan), *common, dev.ptr(w_view), dev.ptr(workspace),
+            dev.ptr(g.geometry_workspace), ns, nc, na, nt, count, c, fs, fd, fn, method, k,
+        )
+    elif storage == "float16":
+        _lib.call(
+            "rcabeam_opw_fp16_device",
+            dev.ptr(packed_rc), dev.ptr(packed_cr), dev.ptr(scan), *common, dev.ptr(g.geometry_workspace),
+            ns, nc, na, nt, count, c, fs, fd, fn,
+        )
+    else:
+        _lib.call(
+            "rcabeam_opw_bf16_device",
+            dev.ptr(packed_rc), dev.ptr(packed_cr), dev.ptr(scan), *common, ns, nc, na, nt, count, c, fs, fd, fn,
+        )
+
+
+def _run(rc, cr, angles, t_start, grid, geom, method, k, return_signal, timings, iq_storage):
+    """Validate, upload, pack, reconstruct and download (ensemble.py:52-130)."""
+    _validate(rc, cr, angles, grid, geom, method, k, return_signal, iq_storage)
+    t = dev.torch()
+    host_input = not (dev.is_tensor(rc) and dev.is_tensor(cr))
+    dev.device()
+    _lib.load()
+    shape = (len(grid[0]), len(grid[1]), len(grid[2]))
+    g = _Geometry(angles, t_start, grid, geom)
+    ns, nc, na, nt = tuple(rc.shape)
+    nv = g.nv
+    code = _METHODS[method]
+    pd = dev.empty((nv, 2), t.float32)
+    weights = dev.empty((nv,), t.float32) if method == "st_sw" else None
+    planes = 2 if method == "apertures" else 1
+    signal = dev.empty((nv * planes, nt), t.complex64) if return_signal else None
+    dev.synchronize()
+    begin = perf_counter()
+    raw_rc = dev.to_device(rc, t.complex64)
+    raw_cr = dev.to_device(cr, t.complex64)
+    dev.synchronize()
+    upload = perf_counter() - begin if host_input else 0.0
+    packed_rc = pack_channels(raw_rc, iq_storage)
+    packed_cr = pack_channels(raw_cr, iq_storage)
+    del raw_rc, raw_cr
+    if method == "st_sw":
+        tile = max(1, _STSW_WORKSPACE_BYTES // (k * k * nt * 8))
+        for first in range(0, nv, tile):
+            run_ensemble(packed_rc, packed_cr, g, (ns, nc, na, nt), code, k, iq_storage, pd, signal, weights,
+                         first, min(tile, nv - first))
+    else:
+        run_ensemble(packed_rc, packed_cr, g, (ns, nc, na, nt), code, k, iq_storage, pd, signal, weights)
+    dev.synchronize()
+    if timings is not None:
+        timings["raw_upload_seconds"] = upload
+    to_host = dev.host if host_input else (lambda x: x)
+    if return_signal:
+        signal_shape = (*shape, 2, nt) if method == "apertures" else (*shape, nt)
+        return to_host(pd[:, 0].reshape(shape)), to_host(signal.reshape(signal_shape))
+    power = pd[:, 0].clone()
+    if method == "st_sw":
+        peak = weights.max()
+        power = power * (weights / peak) if float(peak) > 0 else power * 0
+    return to_host(power.reshape(shape)), None
+
+
+def ensemble_pd_from_channels(rc, cr, angles, t_start, grid, geom: RCAGeometry, *, method: Method = "opw",
+                              k: int = 2, timings: EnsembleTiming | None = None,
+                              iq_storage: IQStorage = "float32"):
+    """Reconstruct a channel ensemble and reduce it to a `(nx, nz, ny)` float32 volume.
+
+    OPW / RC-FMAS: mean squared magnitude; XDoppler: |mean cross-product|;
+    St-SW: XDoppler weighted by normalised ensemble correlations; DMAS: mean
+    real DMAS-CCF-ACF response.  Unfiltered (reference ensemble.py:133-218).
+    `iq_storage` "float16"/"bfloat16" packs channels at 16 bits (OPW only);
+    arithmetic stays float32.
+    """
+    if method == "st_sw_baseband":
+        raise ValueError("st_sw_baseband (two-stage approximation) is not provided by rcabeam_amd")
+    return _run(rc, cr, angles, t_start, grid, geom, method, k, False, timings, iq_storage)[0]
+
+
+def beamform_ensemble(rc, cr, angles, t_start, grid, geom: RCAGeometry, *,
+                      method: Literal["opw", "xdoppler", "rc_fmas"] = "opw", timings: EnsembleTiming | None = None,
+                      iq_storage: IQStorage = "float32"):
+    """Complex64 `(nx, nz, ny, frames)` OPW IQ or XDoppler/RC-FMAS slow-time signal (ensemble.py:221-270)."""
+    if method not in ("opw", "xdoppler", "rc_fmas"):
+        raise ValueError("Only OPW, XDoppler and RC-FMAS have complex signal outputs")
+    return _run(rc, cr, angles, t_start, grid, geom, method, 2, True, timings, iq_storage)[1]
+
+
+def beamform_apertures(rc, cr, angles, t_start, grid, geom: RCAGeometry):
+    """Separately compounded RC and CR IQ, each `(nx, nz, ny, frames)` (ensemble.py:273-287)."""
+    _, iq = _run(rc, cr, angles, t_start, grid, geom, "apertures", 2, True, None, "float32")
+    return iq[..., 0, :], iq[..., 1, :]

@@ -1,0 +1,171 @@
+"""GPU parity of the frame-batched ensemble path against the float64 oracle.
+
+Tolerance: relative L2 error <= 1e-5 for complex64 IQ and for power (the
+stated bar for float32 storage), on seeded inputs small enough for the oracle.
+"""
+
+import numpy as np
+import pytest
+from conftest import rel_l2
+
+import rcabeam_amd as rb
+from oracle import combiners, das, scene
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def _geom(acq):
+    return rb.RCAGeometry(acq.x_el, acq.y_el, c=acq.c, fs=acq.fs, f_demod=acq.fd, fnumber=acq.fnumber)
+
+
+def _oracle(acq, channels_out=False):
+    scan = das.scan_coords(acq.grid)
+    return das.delay_apertures(
+        acq.rc, acq.cr, scan, acq.x_el, acq.y_el, acq.angles, acq.t_start, acq.c, acq.fs, acq.fd, acq.fnumber,
+        channels_out=channels_out,
+    )
+
+
+def test_opw_point_scene_subgrid():
+    """64+64 elements, 256 samples, 3 angles, 16 frames, 5 scatterers + -30 dB noise, 16x32x16 voxels."""
+    acq = scene.point_scene(grid_shape=(16, 32, 16))
+    print("case: point_scene 16x32x16", flush=True)
+    (ra, _, _), (qa, _, _) = _oracle(acq)
+    expected = ra.sum(axis=1) + qa.sum(axis=1)
+    shape = tuple(len(g) for g in acq.grid)
+    iq = rb.beamform_ensemble(acq.rc, acq.cr, acq.angles, acq.t_start, acq.grid, _geom(acq))
+    assert iq.shape == (*shape, 16) and iq.dtype == np.complex64
+    assert rel_l2(iq.reshape(-1, 16), expected) < TOL
+    pd = rb.ensemble_pd_from_channels(acq.rc, acq.cr, acq.angles, acq.t_start, acq.grid, _geom(acq))
+    assert pd.dtype == np.float32
+    assert rel_l2(pd.ravel(), combiners.power_doppler(expected)) < TOL
+
+
+@pytest.mark.parametrize("frames", [1, 33, 200])
+@pytest.mark.parametrize("fnumber", [None, 1.0])
+def test_all_methods_match_oracle(frames, fnumber):
+    """Shape of reference tests/test_ensemble.py:31-87, checked at 1e-5 relative L2."""
+    print(f"case: methods frames={frames} fnumber={fnumber}", flush=True)
+    acq = scene.random_acquisition(frames=frames, fnumber=fnumber)
+    geom = _geom(acq)
+    starts = np.array([1e-6, 2e-6, 1e-6, 3e-6], np.float32)
+    acq = scene.Acquisition(acq.rc, acq.cr, acq.x_el, acq.y_el, acq.angles, starts, acq.grid, acq.c, acq.fs,
+                            acq.fd, acq.fnumber)
+    rc_res, cr_res = _oracle(acq, channels_out=True)
+    args = (acq.rc, acq.cr, acq.angles, starts, acq.grid, geom)
+    r, q = rb.beamform_apertures(*args)
+    assert rel_l2(r.reshape(-1, frames), rc_res[0].sum(axis=1)) < TOL
+    assert rel_l2(q.reshape(-1, frames), cr_res[0].sum(axis=1)) < TOL
+    for method in ("opw", "xdoppler", "rc_fmas"):
+        power, signal = combiners.ensemble_power(method, rc_res, cr_res)
+        timing = {"raw_upload_seconds": -1.0}
+        actual = rb.beamform_ensemble(*args, method=method, timings=timing)
+        assert timing["raw_upload_seconds"] > 0
+        assert rel_l2(actual.reshape(-1, frames), signal) < TOL, method
+        pd = rb.ensemble_pd_from_channels(*args, method=method)
+        assert rel_l2(pd.ravel(), power) < TOL, method
+    power, _ = combiners.ensemble_power("dmas", rc_res, cr_res)
+    assert rel_l2(rb.ensemble_pd_from_channels(*args, method="dmas").ravel(), power) < TOL
+    for k in (2, 3, 4):
+        power, _ = combiners.ensemble_power("st_sw", rc_res, cr_res, k=k)
+        assert rel_l2(rb.ensemble_pd_from_channels(*args, method="st_sw", k=k).ravel(), power) < TOL, k
+    zeros = np.zeros_like(acq.rc)
+    for method in ("opw", "xdoppler", "rc_fmas", "dmas", "st_sw"):
+        out = rb.ensemble_pd_from_channels(zeros, zeros, acq.angles, starts, acq.grid, geom, method=method)
+        assert not out.any(), method
+
+
+def test_closed_receive_window_known_answer():
+    """Reference tests/test_iq_sampling.py:42-53 through the GPU per-frame entry point."""
+    zero = np.zeros(1)
+    geom = rb.RCAGeometry(zero, zero, c=2, fs=1, f_demod=0, fnumber=None)
+    raw = np.arange(1, 4, dtype=np.complex64).reshape(3, 1, 1)
+    _, angles, _ = rb.delay_rca_channel_data(raw, zero, 0, (zero, np.array([0, 1.5, 2, 2.25]), zero), geom, "RC")
+    np.testing.assert_array_equal(angles.ravel(), [1, 2.5, 3, 0])
+    # The same window through the frame-batched ensemble (one frame, both apertures).
+    ens = raw[..., None]
+    iq = rb.beamform_apertures(ens, ens, zero, 0, (zero, np.array([0, 1.5, 2, 2.25]), zero), geom)[0]
+    np.testing.assert_array_equal(iq.ravel(), [1, 2.5, 3, 0])
+
+
+@pytest.mark.parametrize("ny", [4097, 16385])
+def test_voxel_tail_and_large_volume(ny):
+    """Voxel counts that are not multiples of the workgroup tile (reference tests/test_ensemble_targets.py:68-91)."""
+    el = np.array([-0.05e-3, 0.05e-3], np.float32)
+    grid = (np.zeros(1, np.float32), np.array([8e-3], np.float32), np.linspace(-1e-3, 1e-3, ny).astype(np.float32))
+    acq = scene.random_acquisition(samples=368, channels=2, angles=4, frames=3, grid=grid, fs=20e6, fd=15e6, seed=7)
+    acq = scene.Acquisition(acq.rc, acq.cr, el, el, acq.angles, 2e-6, grid, acq.c, acq.fs, acq.fd, 1.0)
+    (ra, _, _), (qa, _, _) = _oracle(acq)
+    expected = ra.sum(axis=1) + qa.sum(axis=1)
+    iq = rb.beamform_ensemble(acq.rc, acq.cr, acq.angles, 2e-6, grid, _geom(acq))
+    assert rel_l2(iq.reshape(-1, 3), expected) < TOL
+
+
+@pytest.mark.parametrize("frames", [2, 8, 64, 66, 72, 200, 300])
+@pytest.mark.parametrize("channels", [1, 17])
+def test_frame_and_channel_tails(frames, channels):
+    """Reference tests/test_ensemble_targets.py:94-120 (cooperative-kernel tails)."""
+    el = np.array([0.001], np.float32) if channels == 1 else np.linspace(-0.006, 0.001, channels).astype(np.float32)
+    grid = (np.zeros(1, np.float32), np.array([0.007], np.float32), np.array([-0.001, 0.0, 0.001], np.float32))
+    acq = scene.random_acquisition(channels=channels, angles=3, frames=frames, grid=grid, seed=19)
+    starts = np.array([1e-6, 2e-6, 3e-6], np.float32)
+    acq = scene.Acquisition(acq.rc, acq.cr, el, el, np.array([-0.08, 0.02, 0.09], np.float32), starts, grid,
+                            acq.c, acq.fs, acq.fd, 1.0)
+    (ra, _, _), (qa, _, _) = _oracle(acq)
+    expected = ra.sum(axis=1) + qa.sum(axis=1)
+    geom = _geom(acq)
+    iq = rb.beamform_ensemble(acq.rc, acq.cr, acq.angles, starts, grid, geom)
+    assert rel_l2(iq.reshape(-1, frames), expected) < TOL
+    pd = rb.ensemble_pd_from_channels(acq.rc, acq.cr, acq.angles, starts, grid, geom)
+    assert rel_l2(pd.ravel(), combiners.power_doppler(expected)) < TOL
+
+
+@pytest.mark.parametrize("storage,dtype", [("float16", np.float16), ("bfloat16", None)])
+@pytest.mark.parametrize("frames", [1, 3, 66, 200])
+def test_half_storage_matches_quantized_fp32(storage, dtype, frames):
+    """Reference tests/test_opw_fp16.py:11-44: 16-bit storage equals float32 on pre-rounded input."""
+    import torch
+
+    rng = np.random.default_rng(111)
+    shape = (48, 3, 4, frames)
+    rc = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
+    cr = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
+    el = np.linspace(-0.001, 0.001, 3)
+    geom = rb.RCAGeometry(el, el, fs=2e6, f_demod=1e6)
+    angles = np.linspace(-0.1, 0.1, 4)
+    grid = (np.linspace(-0.001, 0.001, 3), np.array([0.005, 0.007]), np.linspace(-0.001, 0.003, 5))
+
+    def quantize(x):
+        if dtype is not None:
+            return (x.real.astype(dtype).astype(np.float32) + 1j * x.imag.astype(dtype).astype(np.float32)).astype(
+                np.complex64
+            )
+        t = torch.from_numpy(np.ascontiguousarray(x.view(np.float32)))
+        return t.to(torch.bfloat16).to(torch.float32).numpy().view(np.complex64)
+
+    for scale in (0.0, 1.0, 1e-4):
+        a, b = rc * scale, cr * scale
+        expected = rb.beamform_ensemble(quantize(a), quantize(b), angles, 1e-6, grid, geom)
+        actual = rb.beamform_ensemble(a, b, angles, 1e-6, grid, geom, iq_storage=storage)
+        pd = rb.ensemble_pd_from_channels(a, b, angles, 1e-6, grid, geom, iq_storage=storage)
+        assert actual.dtype == np.complex64 and pd.dtype == np.float32
+        assert rel_l2(actual, expected) < 1e-6
+        assert rel_l2(pd, np.mean(np.abs(actual) ** 2, axis=-1)) < 1e-6
+
+
+def test_half_storage_rejects_invalid_values_and_methods():
+    """Reference tests/test_opw_fp16.py:47-60."""
+    el = np.array([0.0])
+    geom = rb.RCAGeometry(el, el, fs=2e6, f_demod=1e6)
+    grid = (el, np.array([0.005]), el)
+    angles = np.array([0.0, 0.05])
+    data = np.zeros((48, 1, 2, 3), dtype=np.complex64)
+    for value in (70000.0, 70000j, float("inf"), float("nan")):
+        bad = data.copy()
+        bad[0, 0, 0, 0] = value
+        with pytest.raises(ValueError, match="FP16 IQ requires"):
+            rb.ensemble_pd_from_channels(bad, data, angles, 1e-6, grid, geom, iq_storage="float16")
+    with pytest.raises(ValueError, match="OPW only"):
+        rb.ensemble_pd_from_channels(data, data, angles, 1e-6, grid, geom, method="xdoppler", iq_storage="float16")
