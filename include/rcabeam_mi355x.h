@@ -37,6 +37,24 @@ rcabeam_status rcabeam_opw_bf16_device(const void* rc, const void* cr, const flo
                                        void* signal, size_t ns, size_t nc, size_t na, size_t nt, size_t nv, float c,
                                        float fs, float fd, float fn);
 
+/* OPW with any channel storage and any beamformed-IQ storage (rcabeam_storage
+ * codes); float32 interpolation and accumulation.  signal (optional) holds
+ * (nv, nt) samples of signal_storage; pd as rcabeam_ensemble_device.  The
+ * 16-bit signal is the mixed-precision form of the clutter-filter input. */
+rcabeam_status rcabeam_opw_mixed_device(const void* rc, const void* cr, const float* scan, const float* xe,
+                                        const float* ye, const float* angles, const float* starts, float* pd,
+                                        void* signal, int channel_storage, int signal_storage, size_t ns, size_t nc,
+                                        size_t na, size_t nt, size_t nv, float c, float fs, float fd, float fn);
+
+/* Adds to *out (a device counter the caller zeroes) the number of
+ * (voxel, aperture, channel, angle) taps inside the dynamic aperture and the
+ * closed sample window: the work the ensemble kernels perform.  Used for
+ * algorithmic-flop accounting; arguments as rcabeam_ensemble_device. */
+rcabeam_status rcabeam_count_active_taps_device(const float* scan, const float* xe, const float* ye,
+                                                const float* angles, const float* starts, size_t ns, size_t nc,
+                                                size_t na, size_t nv, float c, float fs, float fd, float fn,
+                                                unsigned long long* out);
+
 /* ---- temporal clutter filter (replaces the CuPy/cuBLAS/cuSOLVER path of
  * reference src/rcabeam/gpu_clutter.py:112-273) ----------------------------
  * iq is a row-major Casorati matrix (rows, frames) of `storage` samples. */
@@ -61,6 +79,14 @@ rcabeam_status rcabeam_temporal_gram_device(const void* iq, int storage, size_t 
                                             const float* inv_scale, double* real_gram, float* workspace,
                                             size_t workspace_bytes, int accumulate);
 
+/* Exact real Gram of complex64 input in float64 on v_mfma_f64_16x16x4_f64
+ * (products exact, float64 accumulation): the complex128 Gram of reference
+ * gpu_clutter.py:140-161 without prescaling (float64 cannot overflow on
+ * complex64 products).  Same real_gram layout as above; frames <= 640. */
+size_t rcabeam_gram_f64_workspace_bytes(size_t rows, size_t frames);
+rcabeam_status rcabeam_temporal_gram_f64_device(const void* iq, size_t rows, size_t frames, double* real_gram,
+                                                double* workspace, size_t workspace_bytes, int accumulate);
+
 /* Complex Hermitian Gram G = scale2 * X^H X (complex128, column-major
  * frames x frames) from the real Gram above (reference gpu_clutter.py:160-161). */
 rcabeam_status rcabeam_gram_hermitian_device(const double* real_gram, size_t frames, double scale2, void* gram);
@@ -84,6 +110,24 @@ rcabeam_status rcabeam_filtered_power_device(const void* iq, int storage, size_t
  * (reference gpu_clutter.py:212-241). */
 rcabeam_status rcabeam_temporal_filter_device(const void* iq, int storage, size_t rows, size_t frames,
                                               const void* vectors, size_t r, int subtract, void* out);
+
+/* ---- synthetic acquisitions (GPU counterpart of reference
+ * src/rcabeam/sim.py:131-189, scaled to full-size scenes) ------------------
+ * Gaussian-envelope echoes exp(-((t - tau)/sigma_t)^2) exp(-2j pi fd tau) of
+ * static scatterers (x, z, y; complex amplitude; group index into the per-frame
+ * complex factors group_factor[nt][n_groups]) and moving scatterers (position
+ * per frame moving_pos[nt][n_moving][3]), plus circular complex Gaussian noise
+ * of per-component standard deviation noise_std (counter-based, seeded).
+ * out: complex64 (ns, nc, na, nt).  config 0 = RC, 1 = CR.  Dynamic LDS is
+ * 8*ns*(n_groups + 16) bytes and must stay <= 160 KiB. */
+size_t rcabeam_simulate_lds_bytes(size_t ns, int n_groups);
+rcabeam_status rcabeam_simulate_channels_device(const double* static_pos, const double* static_amp,
+                                                const int* static_group, size_t n_static, const double* group_factor,
+                                                int n_groups, const double* moving_pos, const double* moving_amp,
+                                                size_t n_moving, const float* elements, const float* angles,
+                                                void* out, size_t ns, size_t nc, size_t na, size_t nt, int config,
+                                                float t_start, float c, float fs, float fd, float sigma_t,
+                                                float noise_std, unsigned long long seed);
 
 #ifdef __cplusplus
 }

@@ -11,18 +11,16 @@ runtime errors `RuntimeError`.
 from __future__ import annotations
 
 import ctypes
+import os
 import threading
-from ctypes import POINTER, c_char_p, c_double, c_float, c_int, c_size_t, c_void_p
+from ctypes import c_char_p, c_double, c_float, c_int, c_size_t, c_ulonglong, c_void_p
 from pathlib import Path
 
-LIBRARY_PATH = Path(__file__).resolve().parent / "lib" / "librcabeam_mi355x.so"
+# RCABEAM_LIBRARY may name another build of the same ABI (e.g. a tuning variant).
+LIBRARY_PATH = Path(os.environ.get("RCABEAM_LIBRARY", Path(__file__).resolve().parent / "lib" / "librcabeam_mi355x.so"))
 
 SUCCESS, ERROR_DEVICE, ERROR_ARGUMENT = 0, 1, 2
 STORAGE = {"float32": 0, "complex64": 0, "float16": 1, "bfloat16": 2}
-
-_F = POINTER(c_float)
-_D = POINTER(c_double)
-_I = POINTER(c_int)
 
 # name -> argument types (all return rcabeam_status as int unless listed in _RESTYPE)
 _SIGNATURES: dict[str, list] = {
@@ -43,10 +41,18 @@ _SIGNATURES: dict[str, list] = {
     "rcabeam_ensemble_device": [c_void_p] * 12 + [c_size_t] * 5 + [c_float] * 4 + [c_int, c_int],
     "rcabeam_opw_fp16_device": [c_void_p] * 10 + [c_size_t] * 5 + [c_float] * 4,
     "rcabeam_opw_bf16_device": [c_void_p] * 9 + [c_size_t] * 5 + [c_float] * 4,
+    "rcabeam_opw_mixed_device": [c_void_p] * 9 + [c_int, c_int] + [c_size_t] * 5 + [c_float] * 4,
+    "rcabeam_count_active_taps_device": [c_void_p] * 5 + [c_size_t] * 4 + [c_float] * 4 + [c_void_p],
     "rcabeam_iq_absmax_device": [c_void_p, c_int, c_size_t, c_void_p, c_int],
     "rcabeam_gram_workspace_bytes": [c_size_t, c_size_t],
     "rcabeam_gram_real_elements": [c_size_t],
     "rcabeam_temporal_gram_device": [c_void_p, c_int, c_size_t, c_size_t, c_void_p, c_void_p, c_void_p, c_size_t, c_int],
+    "rcabeam_gram_f64_workspace_bytes": [c_size_t, c_size_t],
+    "rcabeam_temporal_gram_f64_device": [c_void_p, c_size_t, c_size_t, c_void_p, c_void_p, c_size_t, c_int],
+    "rcabeam_simulate_lds_bytes": [c_size_t, c_int],
+    "rcabeam_simulate_channels_device": [c_void_p, c_void_p, c_void_p, c_size_t, c_void_p, c_int, c_void_p, c_void_p,
+                                         c_size_t, c_void_p, c_void_p, c_void_p] + [c_size_t] * 4
+    + [c_int] + [c_float] * 7 + [c_ulonglong],
     "rcabeam_gram_hermitian_device": [c_void_p, c_size_t, c_double, c_void_p],
     "rcabeam_eigh_device": [c_void_p, c_void_p, c_void_p, c_void_p, c_size_t],
     "rcabeam_filtered_power_device": [c_void_p, c_int, c_size_t, c_size_t, c_void_p, c_size_t, c_int, c_void_p],
@@ -56,6 +62,8 @@ _RESTYPE = {
     "rcabeam_status_string": c_char_p,
     "rcabeam_gram_workspace_bytes": c_size_t,
     "rcabeam_gram_real_elements": c_size_t,
+    "rcabeam_gram_f64_workspace_bytes": c_size_t,
+    "rcabeam_simulate_lds_bytes": c_size_t,
 }
 
 # Symbols declared in include/*.h that the library must export.

@@ -8,10 +8,12 @@ of src/rcabeam/clutter.py:65-71.  Inputs are torch GPU tensors (complex64, or
 and answered with NumPy.  A tuple of arrays estimates one shared basis from
 all rows (RC/CR).
 
-Numerics: the Gram uses split bfloat16 (hi + lo) products on MFMA with
-float32 runs reduced in float64 (16-bit storage is exact in one product); the
-T x T eigendecomposition is float64 (rocSOLVER); the projector and power
-arithmetic is float64.  Values are sqrt(lambda) * scale * sqrt(rows) in
+Numerics: precision "complex128" forms the Gram exactly on float64 MFMA
+(complex64 products are exact in float64), "complex64" uses split bfloat16
+(hi + lo) products on MFMA with short float32 runs reduced in float64, and
+16-bit packed IQ is exact in one fp16/bf16 MFMA product.  The T x T
+eigendecomposition is float64 (rocSOLVER); the projector and power arithmetic
+is float64.  Values are sqrt(lambda) * scale * sqrt(rows) in
 descending order and the basis columns follow that order.
 """
 
@@ -109,17 +111,28 @@ def temporal_basis(iq, *, method: Solver = "gram", precision: str = "complex128"
     rows = sum(a.numel() // frames for a in arrays)
     lib = _lib.load()
     real = dev.empty((int(lib.rcabeam_gram_real_elements(frames)),), t.float64)
+    exact = code == 0 and precision == "complex128"
     for i, a in enumerate(arrays):
         r = a.numel() // frames
-        nbytes = int(lib.rcabeam_gram_workspace_bytes(r, frames))
-        if nbytes == 0:
-            raise ValueError("Frame count out of range for the GPU Gram (1 <= T <= 1024)")
-        work = dev.empty((nbytes // 4,), t.float32)
-        inv_scale_ptr = dev.ptr(stats) + 8  # out[2]
-        _lib.call("rcabeam_temporal_gram_device", dev.ptr(a), code, r, frames, inv_scale_ptr, dev.ptr(real),
-                  dev.ptr(work), nbytes, int(i > 0))
+        if exact:
+            nbytes = int(lib.rcabeam_gram_f64_workspace_bytes(r, frames))
+            if nbytes == 0:
+                raise ValueError("Frame count out of range for the float64 GPU Gram (1 <= T <= 640)")
+            work = dev.empty((nbytes // 8,), t.float64)
+            _lib.call("rcabeam_temporal_gram_f64_device", dev.ptr(a), r, frames, dev.ptr(real), dev.ptr(work),
+                      nbytes, int(i > 0))
+        else:
+            nbytes = int(lib.rcabeam_gram_workspace_bytes(r, frames))
+            if nbytes == 0:
+                raise ValueError("Frame count out of range for the GPU Gram (1 <= T <= 1024)")
+            work = dev.empty((nbytes // 4,), t.float32)
+            inv_scale_ptr = dev.ptr(stats) + 8  # out[2]
+            _lib.call("rcabeam_temporal_gram_device", dev.ptr(a), code, r, frames, inv_scale_ptr, dev.ptr(real),
+                      dev.ptr(work), nbytes, int(i > 0))
+    # The float64 path forms the unscaled Gram; fold the 1/scale^2 of the reference's scaled Gram in here.
+    scale2 = (1.0 / (scale * scale) if (exact and scale > 0) else 1.0) / rows
     gram = dev.empty((frames, frames), t.complex128)
-    _lib.call("rcabeam_gram_hermitian_device", dev.ptr(real), frames, 1.0 / rows, dev.ptr(gram))
+    _lib.call("rcabeam_gram_hermitian_device", dev.ptr(real), frames, scale2, dev.ptr(gram))
     if timings is not None:
         dev.synchronize()
         timings["matrix_seconds"] = perf_counter() - begin

@@ -190,8 +190,10 @@ __global__ void k_gram_reduce(const float* __restrict__ partials, int groups, Gr
   *dst = accumulate ? *dst + sum : sum;
 }
 
+// Upper-triangular blocks are stored (diagonal blocks in full), so any entry
+// with i <= j is present and C is symmetric.
 __device__ __forceinline__ double c_at(const double* c, long long ldc, int i, int j) {
-  return (i / 32 <= j / 32) ? c[i * ldc + j] : c[j * ldc + i];
+  return i <= j ? c[i * ldc + j] : c[j * ldc + i];
 }
 
 // G (complex128, column-major T x T) = scale2 * complex Gram from C.
@@ -204,6 +206,102 @@ __global__ void k_gram_form(const double* __restrict__ c, int nb, int frames, do
   const double re = c_at(c, ldc, 2 * t, 2 * u) + c_at(c, ldc, 2 * t + 1, 2 * u + 1);
   const double im = c_at(c, ldc, 2 * t, 2 * u + 1) - c_at(c, ldc, 2 * t + 1, 2 * u);
   g[idx] = make_double2(re * scale2, im * scale2);
+}
+
+// ---- exact Gram: float64 MFMA ----------------------------------------------
+// complex64 samples widen exactly to float64; v_mfma_f64_16x16x4_f64 forms the
+// products exactly and accumulates in float64, so C equals the complex128
+// reference Gram up to float64 summation order.  Blocks are 16x16; a pass
+// covers up to kG64Waves * kG64MaxBlocks upper-triangular blocks and the
+// row chunk staged in LDS is shared by every block of the pass.
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+constexpr int kG64Threads = 1024;
+constexpr int kG64Waves = kG64Threads / kWave;
+constexpr int kG64MaxBlocks = 7;
+constexpr int kG64BlocksPerPass = kG64Waves * kG64MaxBlocks;  // 112
+constexpr int kG64Rows = 16;                                  // rows staged per step (4 MFMA K-steps)
+constexpr int kG64RowsPerGroup = 8192;
+
+struct Gram64Plan {
+  int nb;  // 16-column blocks of the real Gram (2T padded to 16)
+  int wave_start[kG64Waves];
+  int wave_count[kG64Waves];
+  unsigned char bi[kG64BlocksPerPass];
+  unsigned char bj[kG64BlocksPerPass];
+  int pass_blocks;
+};
+
+__global__ __launch_bounds__(kG64Threads) void k_gram64(const float2* __restrict__ x, long long rows, int frames,
+                                                       Gram64Plan plan, double* __restrict__ partials) {
+  extern __shared__ __align__(16) double stage[];  // [kG64Rows][ld]
+  const int ld = plan.nb * 16;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x / kWave;
+  const long long row0 = static_cast<long long>(blockIdx.x) * kG64RowsPerGroup;
+  const long long row_end = row0 + kG64RowsPerGroup < rows ? row0 + kG64RowsPerGroup : rows;
+  const int steps = static_cast<int>((row_end - row0 + kG64Rows - 1) / kG64Rows);
+  for (int idx = threadIdx.x; idx < kG64Rows * (ld - 2 * frames); idx += blockDim.x) {
+    const int k = idx / (ld - 2 * frames), c = idx % (ld - 2 * frames);
+    stage[k * ld + 2 * frames + c] = 0.0;
+  }
+  doublex4 acc[kG64MaxBlocks];
+#pragma unroll
+  for (int b = 0; b < kG64MaxBlocks; ++b) acc[b] = doublex4{0.0, 0.0, 0.0, 0.0};
+  const int my_start = plan.wave_start[wave];
+  const int my_count = plan.wave_count[wave];
+  const int per_step = kG64Rows * frames;
+  for (int s = 0; s < steps; ++s) {
+    const long long rbase = row0 + static_cast<long long>(s) * kG64Rows;
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < per_step; idx += blockDim.x) {
+      const int k = idx / frames, t = idx - k * frames;
+      const long long r = rbase + k;
+      const float2 v = r < row_end ? x[r * frames + t] : make_float2(0.f, 0.f);
+      *reinterpret_cast<double2*>(stage + k * ld + 2 * t) = make_double2(v.x, v.y);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < kG64Rows; kk += 4) {
+      const double* rowp = stage + (kk + lane / 16) * ld + (lane & 15);
+      int prev = -1;
+      double a = 0.0;
+#pragma unroll
+      for (int b = 0; b < kG64MaxBlocks; ++b) {
+        if (b < my_count) {
+          const int I = plan.bi[my_start + b], J = plan.bj[my_start + b];
+          if (I != prev) {
+            a = rowp[16 * I];
+            prev = I;
+          }
+          acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, rowp[16 * J], acc[b], 0, 0, 0);
+        }
+      }
+    }
+  }
+  double* out = partials + static_cast<long long>(blockIdx.x) * plan.pass_blocks * 256;
+#pragma unroll
+  for (int b = 0; b < kG64MaxBlocks; ++b)
+    if (b < my_count) {
+      double* dst = out + static_cast<long long>(my_start + b) * 256 + lane * 4;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) dst[e] = acc[b][e];
+    }
+}
+
+// Sum float64 partials into the real Gram C (16x16 blocks, ldc = 32 * ceil(2T / 32)).
+__global__ void k_gram64_reduce(const double* __restrict__ partials, int groups, Gram64Plan plan, int ldc,
+                                double* __restrict__ c, int accumulate) {
+  const long long idx = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (idx >= static_cast<long long>(plan.pass_blocks) * 256) return;
+  const int b = static_cast<int>(idx / 256), e = static_cast<int>(idx % 256);
+  double sum = 0.0;
+  for (int g = 0; g < groups; ++g) sum += partials[(static_cast<long long>(g) * plan.pass_blocks + b) * 256 + e];
+  const int l = e / 4, r = e % 4;
+  // 16x16 float64 MFMA accumulator layout: lane l, register r -> (row 4*(l/16) + r, column l%16).
+  const int i = 16 * plan.bi[b] + 4 * (l / 16) + r;
+  const int j = 16 * plan.bj[b] + (l % 16);
+  double* dst = c + static_cast<long long>(i) * ldc + j;
+  *dst = accumulate ? *dst + sum : sum;
 }
 
 // ---- max(|Re|, |Im|) and finiteness ---------------------------------------
@@ -373,6 +471,34 @@ static hipError_t gram_launch(const void* x, long long rows, int frames, const f
   return hipSuccess;
 }
 
+static void make_plan64(int nb, int pass, Gram64Plan& plan) {
+  plan.nb = nb;
+  const long long total = triangle_blocks(nb);
+  const long long first = static_cast<long long>(pass) * kG64BlocksPerPass;
+  const int count = static_cast<int>(total - first < kG64BlocksPerPass ? total - first : kG64BlocksPerPass);
+  plan.pass_blocks = count;
+  long long idx = 0;
+  int filled = 0;
+  for (int I = 0; I < nb; ++I)
+    for (int J = I; J < nb; ++J, ++idx)
+      if (idx >= first && idx < first + count) {
+        plan.bi[filled] = static_cast<unsigned char>(I);
+        plan.bj[filled] = static_cast<unsigned char>(J);
+        ++filled;
+      }
+  const int base = count / kG64Waves, extra = count % kG64Waves;
+  int start = 0;
+  for (int w = 0; w < kG64Waves; ++w) {
+    const int n = base + (w < extra ? 1 : 0);
+    plan.wave_start[w] = start;
+    plan.wave_count[w] = n;
+    start += n;
+  }
+}
+
+static int column_blocks16(size_t frames) { return static_cast<int>((2 * frames + 15) / 16); }
+static size_t lds64_bytes(size_t frames) { return static_cast<size_t>(kG64Rows) * 16 * column_blocks16(frames) * sizeof(double); }
+
 struct SolverState {
   std::mutex lock;
   rocblas_handle handle[64] = {};
@@ -510,4 +636,42 @@ extern "C" rcabeam_status rcabeam_temporal_filter_device(const void* iq, int sto
                                                          const void* vectors, size_t r, int subtract, void* out) {
   if (!out) return RCABEAM_ERROR_ARGUMENT;
   return project<true>(iq, storage, rows, frames, vectors, r, subtract, nullptr, out);
+}
+
+extern "C" size_t rcabeam_gram_f64_workspace_bytes(size_t rows, size_t frames) {
+  if (!rows || !frames || lds64_bytes(frames) > 160 * 1024) return 0;
+  const long long nb = column_blocks16(frames);
+  const long long groups = (static_cast<long long>(rows) + kG64RowsPerGroup - 1) / kG64RowsPerGroup;
+  const long long per_pass = triangle_blocks(static_cast<int>(nb)) < kG64BlocksPerPass ? triangle_blocks(static_cast<int>(nb))
+                                                                                       : kG64BlocksPerPass;
+  return static_cast<size_t>(groups * per_pass * 256 * sizeof(double));
+}
+
+extern "C" rcabeam_status rcabeam_temporal_gram_f64_device(const void* iq, size_t rows, size_t frames,
+                                                           double* real_gram, double* workspace,
+                                                           size_t workspace_bytes, int accumulate) {
+  if (!iq || !real_gram || !workspace || !rows || !frames) return RCABEAM_ERROR_ARGUMENT;
+  const size_t need = rcabeam_gram_f64_workspace_bytes(rows, frames);
+  if (!need || workspace_bytes < need) return RCABEAM_ERROR_ARGUMENT;
+  const int nb = column_blocks16(frames);
+  const long long groups = (static_cast<long long>(rows) + kG64RowsPerGroup - 1) / kG64RowsPerGroup;
+  const int passes = static_cast<int>((triangle_blocks(nb) + kG64BlocksPerPass - 1) / kG64BlocksPerPass);
+  const size_t lds = lds64_bytes(frames);
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_gram64),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+  if (e != hipSuccess) return RCABEAM_ERROR_CUDA;
+  const int ldc = 32 * column_blocks(frames);
+  for (int pass = 0; pass < passes; ++pass) {
+    Gram64Plan plan;
+    make_plan64(nb, pass, plan);
+    k_gram64<<<dim3(static_cast<unsigned>(groups)), kG64Threads, lds>>>(static_cast<const float2*>(iq),
+                                                                         static_cast<long long>(rows),
+                                                                         static_cast<int>(frames), plan, workspace);
+    if (hipGetLastError() != hipSuccess) return RCABEAM_ERROR_CUDA;
+    const long long n = static_cast<long long>(plan.pass_blocks) * 256;
+    k_gram64_reduce<<<dim3(static_cast<unsigned>((n + 255) / 256)), 256>>>(workspace, static_cast<int>(groups), plan,
+                                                                            ldc, real_gram, accumulate);
+    if (hipGetLastError() != hipSuccess) return RCABEAM_ERROR_CUDA;
+  }
+  return RCABEAM_SUCCESS;
 }

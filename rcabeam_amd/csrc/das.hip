@@ -25,6 +25,10 @@ namespace rcb {
 
 constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / kWave;
+// Voxels per wave of the OPW kernel (compile-time tuning knob).
+#ifndef RCB_OPW_VPW
+#define RCB_OPW_VPW 2
+#endif
 constexpr size_t kLdsLimit = 64 * 1024;
 
 struct EnsembleArgs {
@@ -80,46 +84,72 @@ __device__ void fill_terms(const EnsembleArgs& p, int cfg, long long v0, int vb_
   }
 }
 
+// A tap ready for the inner loop: element offset of the first of the two
+// sample rows (channel, angle, i0) in the packed layout, interpolation weight
+// and phasor.  A tap that contributes nothing has a zero phasor (a real tap's
+// phasor has unit modulus) and points at row 0 of its plane with zero weight.
+struct RowTap {
+  unsigned off;  // ((ch * na + a) * ns + i0) * nt
+  float w;
+  float c;
+  float s;
+};
+
+__device__ __forceinline__ RowTap row_tap(const Tap& t, int ch, int na, int a, int ns, int nt) {
+  const unsigned plane = (static_cast<unsigned>(ch) * na + a) * static_cast<unsigned>(ns);
+  if (t.i0 < 0) return RowTap{plane * static_cast<unsigned>(nt), 0.f, 0.f, 0.f};
+  return RowTap{(plane + static_cast<unsigned>(t.i0)) * static_cast<unsigned>(nt), t.w, t.c, t.s};
+}
+
 // Accumulate one angle of one aperture into part[VPW][FPL] for the frames of
-// this lane.  `taps` holds VB * nc taps of the current angle.
+// this lane.  `taps` holds the VB * nc row taps of the current angle.  All
+// 2*VPW*FPL sample loads of a channel are issued before any arithmetic (one
+// wait per channel, not per load).  Lanes whose frame is past the end load the
+// last frame instead (a valid address) and their sums are never stored.  When
+// w == 0 the second row is not needed (this covers i0 == ns - 1, where the
+// closed window admits only w == 0) and the first row is read twice.
 template <typename T, int VPW, int FPL>
-__device__ __forceinline__ void accumulate_angle(const T* data, const Tap* taps, int nc, int na, int ns, int nt,
-                                                 int a, int f0, int wave, int lane, float2 (&part)[VPW][FPL]) {
-  const int t0 = f0 + lane;
+__device__ __forceinline__ void accumulate_angle(const T* __restrict__ data, const RowTap* taps, int nc, int nt,
+                                                 int f0, int wave, int lane, float2 (&part)[VPW][FPL]) {
+  int tcl[FPL];
+#pragma unroll
+  for (int j = 0; j < FPL; ++j) {
+    const int t = f0 + lane + kWave * j;
+    tcl[j] = t < nt ? t : nt - 1;
+  }
   for (int ch = 0; ch < nc; ++ch) {
-    int i0[VPW];
+    unsigned off[VPW];
     float w[VPW], pc[VPW], ps[VPW];
-    int any = -1;
+    bool any = false;
 #pragma unroll
     for (int vv = 0; vv < VPW; ++vv) {
-      const Tap tp = taps[(wave * VPW + vv) * nc + ch];
-      i0[vv] = uniform(tp.i0);
+      const RowTap tp = taps[(wave * VPW + vv) * nc + ch];
+      off[vv] = static_cast<unsigned>(uniform(static_cast<int>(tp.off)));
       w[vv] = uniform(tp.w);
       pc[vv] = uniform(tp.c);
       ps[vv] = uniform(tp.s);
-      any = i0[vv] > any ? i0[vv] : any;
+      any |= (pc[vv] != 0.f) | (ps[vv] != 0.f);
     }
-    if (any < 0) continue;  // the whole wave's voxels are outside aperture/window
-    const size_t plane = (static_cast<size_t>(ch) * na + a) * ns;
+    if (!any) continue;  // every voxel of this wave is outside aperture/window
+    float2 A[VPW][FPL], B[VPW][FPL];
 #pragma unroll
     for (int vv = 0; vv < VPW; ++vv) {
-      // An inactive tap of an active group reuses the group's row with zero weight.
-      const bool on = i0[vv] >= 0;
-      const int row = on ? i0[vv] : any;
-      const float ww = on ? w[vv] : 0.f, cc = on ? pc[vv] : 0.f, ss = on ? ps[vv] : 0.f;
-      const T* base = data + (plane + row) * nt;
-      const size_t step = row == ns - 1 ? 0 : static_cast<size_t>(nt);
+      const T* base = data + off[vv];
+      const T* next = base + (w[vv] != 0.f ? nt : 0);
 #pragma unroll
       for (int j = 0; j < FPL; ++j) {
-        const int t = t0 + kWave * j;
-        if (t < nt) {
-          const float2 A = load_sample(base + t);
-          const float2 B = load_sample(base + step + t);
-          const float vx = fmaf(ww, B.x - A.x, A.x);
-          const float vy = fmaf(ww, B.y - A.y, A.y);
-          part[vv][j].x = fmaf(cc, vx, fmaf(-ss, vy, part[vv][j].x));
-          part[vv][j].y = fmaf(cc, vy, fmaf(ss, vx, part[vv][j].y));
-        }
+        A[vv][j] = load_sample(base + tcl[j]);
+        B[vv][j] = load_sample(next + tcl[j]);
+      }
+    }
+#pragma unroll
+    for (int vv = 0; vv < VPW; ++vv) {
+#pragma unroll
+      for (int j = 0; j < FPL; ++j) {
+        const float vx = fmaf(w[vv], B[vv][j].x - A[vv][j].x, A[vv][j].x);
+        const float vy = fmaf(w[vv], B[vv][j].y - A[vv][j].y, A[vv][j].y);
+        part[vv][j].x = fmaf(pc[vv], vx, fmaf(-ps[vv], vy, part[vv][j].x));
+        part[vv][j].y = fmaf(pc[vv], vy, fmaf(ps[vv], vx, part[vv][j].y));
       }
     }
   }
@@ -128,20 +158,21 @@ __device__ __forceinline__ void accumulate_angle(const T* data, const Tap* taps,
 // Methods 0 (OPW), 1 (XDoppler), 2 (RC-FMAS), 4 (St-SW), 5 (apertures):
 // angle-major traversal (per-angle channel sums are complete before any
 // per-angle nonlinearity).
-template <int METHOD, typename T, int VPW, int FPL, int K>
+template <int METHOD, typename T, int VPW, int FPL, int K, typename O = float2>
 __global__ __launch_bounds__(kBlock) void das_angle_major(EnsembleArgs p, long long tiles) {
   constexpr int VB = kWavesPerBlock * VPW;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   Term* rx = reinterpret_cast<Term*>(lds_raw);
   Term* tx = rx + VB * p.nc;
-  Tap* taps = reinterpret_cast<Tap*>(tx + VB * p.na);
+  RowTap* taps = reinterpret_cast<RowTap*>(tx + VB * p.na);
 
   const long long tile = xcd_tile(tiles);
   if (tile >= tiles) return;
   const long long v0 = tile * VB;
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = threadIdx.x / kWave;
-  const T* data[2] = {static_cast<const T*>(p.rc), static_cast<const T*>(p.cr)};
+  const T* __restrict__ rc_data = static_cast<const T*>(p.rc);
+  const T* __restrict__ cr_data = static_cast<const T*>(p.cr);
 
   float pw[VPW], pi[VPW];
 #pragma unroll
@@ -168,8 +199,8 @@ __global__ __launch_bounds__(kBlock) void das_angle_major(EnsembleArgs p, long l
       for (int a = 0; a < p.na; ++a) {
         __syncthreads();
         for (int idx = threadIdx.x; idx < VB * p.nc; idx += blockDim.x) {
-          const int vb = idx / p.nc;
-          taps[idx] = combine(rx[idx], tx[vb * p.na + a], p.ns);
+          const int vb = idx / p.nc, ch = idx - vb * p.nc;
+          taps[idx] = row_tap(combine(rx[idx], tx[vb * p.na + a], p.ns), ch, p.na, a, p.ns, p.nt);
         }
         __syncthreads();
         float2 part[VPW][FPL];
@@ -177,7 +208,7 @@ __global__ __launch_bounds__(kBlock) void das_angle_major(EnsembleArgs p, long l
         for (int vv = 0; vv < VPW; ++vv)
 #pragma unroll
           for (int j = 0; j < FPL; ++j) part[vv][j] = make_float2(0.f, 0.f);
-        accumulate_angle<T, VPW, FPL>(data[cfg], taps, p.nc, p.na, p.ns, p.nt, a, f0, wave, lane, part);
+        accumulate_angle<T, VPW, FPL>(cfg == 0 ? rc_data : cr_data, taps, p.nc, p.nt, f0, wave, lane, part);
 #pragma unroll
         for (int vv = 0; vv < VPW; ++vv)
 #pragma unroll
@@ -215,7 +246,7 @@ __global__ __launch_bounds__(kBlock) void das_angle_major(EnsembleArgs p, long l
           p.signal[(2 * v) * p.nt + t] = r[vv][j];
           p.signal[(2 * v + 1) * p.nt + t] = q[vv][j];
         } else if constexpr (METHOD == kOpw) {
-          if (p.signal) p.signal[v * p.nt + t] = r[vv][j];
+          if (p.signal) store_sample(reinterpret_cast<O*>(p.signal) + v * p.nt + t, r[vv][j]);
           pw[vv] += cnorm(r[vv][j]);
         } else if constexpr (METHOD == kRcFmas) {
           const float2 value = cmul(r[vv][j], q[vv][j]);
@@ -432,11 +463,40 @@ __global__ __launch_bounds__(kBlock) void stsw_finish(const float2* ccf, float* 
   if (lane == 0) weights[v] = hypotf(acc.x, acc.y);
 }
 
+// Number of (voxel, aperture, channel, angle) taps that contribute (inside
+// the dynamic aperture and the closed sample window).  Instrumentation for
+// algorithmic-flop accounting; one thread per (voxel, aperture, channel).
+__global__ void k_count_taps(EnsembleArgs p, unsigned long long* out) {
+  const long long idx = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  unsigned long long n = 0;
+  if (idx < p.nv * 2 * p.nc) {
+    const long long v = idx / (2 * p.nc);
+    const int cfg = static_cast<int>((idx / p.nc) % 2);
+    const int ch = static_cast<int>(idx % p.nc);
+    const double x = p.scan[3 * v], z = p.scan[3 * v + 1], y = p.scan[3 * v + 2];
+    const float* el = cfg == 0 ? p.xe : p.ye;
+    const Term r = receive_term(cfg == 0 ? x : y, z, static_cast<double>(el[ch]), p.c, p.fs, p.fd, p.fn);
+    if (r.i != kInactive) {
+      for (int a = 0; a < p.na; ++a) {
+        double sn, cs;
+        sincos(static_cast<double>(p.angles[a]), &sn, &cs);
+        const Term t = transmit_term(cfg == 0 ? y : x, z, cs, sn, static_cast<double>(p.starts[a]), p.c, p.fs, p.fd);
+        n += combine(r, t, p.ns).i0 >= 0 ? 1 : 0;
+      }
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) n += __shfl_xor(n, off, kWave);
+  if ((threadIdx.x & (kWave - 1)) == 0 && n) atomicAdd(out, n);
+}
+
 // ---- host dispatch ---------------------------------------------------------
 
-template <int METHOD, typename T, int VPW, int K>
+template <int METHOD, typename T, int VPW, int K, typename O = float2>
 static hipError_t launch_angle_major(const EnsembleArgs& p) {
   constexpr int VB = kWavesPerBlock * VPW;
+  // Row offsets are 32-bit element indices into one packed aperture.
+  if (static_cast<unsigned long long>(p.ns) * p.nc * p.na * p.nt > 0xffffffffULL) return hipErrorInvalidValue;
   const size_t lds = static_cast<size_t>(VB) * (2 * p.nc + p.na) * 16;
   if (lds > kLdsLimit) return hipErrorInvalidValue;
   const long long tiles = (p.nv + VB - 1) / VB;
@@ -445,13 +505,13 @@ static hipError_t launch_angle_major(const EnsembleArgs& p) {
   const int fpl = p.nt <= kWave ? 1 : (p.nt <= 2 * kWave ? 2 : 4);
   switch (fpl) {
     case 1:
-      das_angle_major<METHOD, T, VPW, 1, K><<<dim3(static_cast<unsigned>(grid)), kBlock, lds>>>(p, tiles);
+      das_angle_major<METHOD, T, VPW, 1, K, O><<<dim3(static_cast<unsigned>(grid)), kBlock, lds>>>(p, tiles);
       break;
     case 2:
-      das_angle_major<METHOD, T, VPW, 2, K><<<dim3(static_cast<unsigned>(grid)), kBlock, lds>>>(p, tiles);
+      das_angle_major<METHOD, T, VPW, 2, K, O><<<dim3(static_cast<unsigned>(grid)), kBlock, lds>>>(p, tiles);
       break;
     default:
-      das_angle_major<METHOD, T, VPW, 4, K><<<dim3(static_cast<unsigned>(grid)), kBlock, lds>>>(p, tiles);
+      das_angle_major<METHOD, T, VPW, 4, K, O><<<dim3(static_cast<unsigned>(grid)), kBlock, lds>>>(p, tiles);
       break;
   }
   return hipGetLastError();
@@ -473,7 +533,7 @@ template <typename T>
 static hipError_t dispatch(const EnsembleArgs& p, int method, int k) {
   switch (method) {
     case kOpw:
-      return launch_angle_major<kOpw, T, 2, 1>(p);
+      return launch_angle_major<kOpw, T, RCB_OPW_VPW, 1>(p);
     case kXdoppler:
       return launch_angle_major<kXdoppler, T, 2, 1>(p);
     case kRcFmas:
@@ -597,4 +657,57 @@ extern "C" rcabeam_status rcabeam_opw_bf16_device(const void* rc, const void* cr
   EnsembleArgs p = make_args(rc, cr, scan, xe, ye, angles, starts, pd, signal, nullptr, nullptr, ns, nc, na, nt, nv,
                              c, fs, fd, fn);
   return to_status(dispatch<bf16x2>(p, kOpw, 2));
+}
+
+extern "C" rcabeam_status rcabeam_count_active_taps_device(const float* scan, const float* xe, const float* ye,
+                                                           const float* angles, const float* starts, size_t ns,
+                                                           size_t nc, size_t na, size_t nv, float c, float fs,
+                                                           float fd, float fn, unsigned long long* out) {
+  if (!scan || !xe || !ye || !angles || !starts || !out || ns < 2 || !nc || !na || !nv) return RCABEAM_ERROR_ARGUMENT;
+  EnsembleArgs p = make_args(nullptr, nullptr, scan, xe, ye, angles, starts, nullptr, nullptr, nullptr, nullptr, ns,
+                             nc, na, 1, nv, c, fs, fd, fn);
+  const long long n = static_cast<long long>(nv) * 2 * static_cast<long long>(nc);
+  k_count_taps<<<dim3(static_cast<unsigned>((n + 255) / 256)), 256>>>(p, out);
+  return launch_status();
+}
+
+template <typename T>
+static hipError_t dispatch_opw_out(const EnsembleArgs& p, int out_storage) {
+  switch (out_storage) {
+    case RCABEAM_STORAGE_C64:
+      return launch_angle_major<kOpw, T, RCB_OPW_VPW, 1, float2>(p);
+    case RCABEAM_STORAGE_FP16:
+      return launch_angle_major<kOpw, T, RCB_OPW_VPW, 1, __half2>(p);
+    case RCABEAM_STORAGE_BF16:
+      return launch_angle_major<kOpw, T, RCB_OPW_VPW, 1, bf16x2>(p);
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+extern "C" rcabeam_status rcabeam_opw_mixed_device(const void* rc, const void* cr, const float* scan, const float* xe,
+                                                   const float* ye, const float* angles, const float* starts,
+                                                   float* pd, void* signal, int channel_storage, int signal_storage,
+                                                   size_t ns, size_t nc, size_t na, size_t nt, size_t nv, float c,
+                                                   float fs, float fd, float fn) {
+  if (reinterpret_cast<uintptr_t>(rc) % 4 || reinterpret_cast<uintptr_t>(cr) % 4) return RCABEAM_ERROR_ARGUMENT;
+  if (!valid_ensemble(rc, cr, scan, xe, ye, angles, starts, pd, ns, nc, na, nt, nv, c, fs, fd, fn))
+    return RCABEAM_ERROR_ARGUMENT;
+  EnsembleArgs p = make_args(rc, cr, scan, xe, ye, angles, starts, pd, signal, nullptr, nullptr, ns, nc, na, nt, nv,
+                             c, fs, fd, fn);
+  hipError_t e;
+  switch (channel_storage) {
+    case RCABEAM_STORAGE_C64:
+      e = dispatch_opw_out<float2>(p, signal_storage);
+      break;
+    case RCABEAM_STORAGE_FP16:
+      e = dispatch_opw_out<__half2>(p, signal_storage);
+      break;
+    case RCABEAM_STORAGE_BF16:
+      e = dispatch_opw_out<bf16x2>(p, signal_storage);
+      break;
+    default:
+      e = hipErrorInvalidValue;
+  }
+  return to_status(e);
 }
