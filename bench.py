@@ -117,8 +117,11 @@ class Pipeline:
             self._call("rcabeam_ensemble_device", *common, dev.ptr(self.signal), 0, 0, dev.ptr(self.g.geometry_workspace),
                        *shape, nv, c, fs, fd, fn, 0, 2)
         else:
-            self._call("rcabeam_opw_mixed_device", *common, dev.ptr(self.signal), self.code, self.code, *shape, nv,
-                       c, fs, fd, fn)
+            nx, nz, ny = self.g.shape
+            self._call("rcabeam_opw_grid_device", dev.ptr(self.packed[0]), dev.ptr(self.packed[1]), self.code,
+                       dev.ptr(self.g.axes[0]), dev.ptr(self.g.axes[1]), dev.ptr(self.g.axes[2]), nx, nz, ny,
+                       dev.ptr(self.g.xe), dev.ptr(self.g.ye), dev.ptr(self.g.angles), dev.ptr(self.g.starts),
+                       dev.ptr(self.pd), dev.ptr(self.signal), self.code, *shape, c, fs, fd, fn)
         if das_events is not None:
             das_events[1].record()
         self._call("rcabeam_iq_absmax_device", dev.ptr(self.signal), self.code, nv * self.nt, dev.ptr(self.stats), 0)

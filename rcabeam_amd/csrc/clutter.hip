@@ -297,8 +297,9 @@ __global__ void k_gram64_reduce(const double* __restrict__ partials, int groups,
   double sum = 0.0;
   for (int g = 0; g < groups; ++g) sum += partials[(static_cast<long long>(g) * plan.pass_blocks + b) * 256 + e];
   const int l = e / 4, r = e % 4;
-  // 16x16 float64 MFMA accumulator layout: lane l, register r -> (row 4*(l/16) + r, column l%16).
-  const int i = 16 * plan.bi[b] + 4 * (l / 16) + r;
+  // 16x16x4 float64 MFMA accumulator layout (measured on gfx950): lane l,
+  // register r -> (row l/16 + 4 r, column l%16).
+  const int i = 16 * plan.bi[b] + (l / 16) + 4 * r;
   const int j = 16 * plan.bj[b] + (l % 16);
   double* dst = c + static_cast<long long>(i) * ldc + j;
   *dst = accumulate ? *dst + sum : sum;

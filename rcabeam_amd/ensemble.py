@@ -112,6 +112,8 @@ class _Geometry:
         if order is not None:
             scan = np.ascontiguousarray(scan[order])
         self.nv = len(scan)
+        self.shape = tuple(len(axis) for axis in grid)
+        self.axes = tuple(dev.to_device(np.asarray(axis, dtype=np.float32), t.float32) for axis in grid)
         self.scan = dev.to_device(scan, t.float32)
         self.xe, self.ye, self.angles = (dev.to_device(v, t.float32) for v in vectors)
         self.starts = dev.to_device(starts, t.float32)
@@ -121,8 +123,12 @@ class _Geometry:
 
 
 def run_ensemble(packed_rc, packed_cr, g: _Geometry, shape4, method: int, k: int, storage: str, pd, signal,
-                 weights=None, first: int = 0, count: int | None = None):
-    """Enqueue one reconstruction of voxels [first, first + count) over the C ABI."""
+                 weights=None, first: int = 0, count: int | None = None, signal_storage: int = 0):
+    """Enqueue one reconstruction of voxels [first, first + count) over the C ABI.
+
+    float32 channels run the frame-batched VALU kernels (rcabeam_ensemble_device);
+    16-bit channels (OPW only) run the MFMA grid kernel (rcabeam_opw_grid_device).
+    """
     t = dev.torch()
     ns, nc, na, nt = shape4
     count = g.nv - first if count is None else count
@@ -143,16 +149,16 @@ def run_ensemble(packed_rc, packed_cr, g: _Geometry, shape4, method: int, k: int
             dev.ptr(packed_rc), dev.ptr(packed_cr), dev.ptr(scan), *common, dev.ptr(w_view), dev.ptr(workspace),
             dev.ptr(g.geometry_workspace), ns, nc, na, nt, count, c, fs, fd, fn, method, k,
         )
-    elif storage == "float16":
-        _lib.call(
-            "rcabeam_opw_fp16_device",
-            dev.ptr(packed_rc), dev.ptr(packed_cr), dev.ptr(scan), *common, dev.ptr(g.geometry_workspace),
-            ns, nc, na, nt, count, c, fs, fd, fn,
-        )
     else:
+        # 16-bit channels: delay-and-sum on MFMA over the Cartesian grid (4 x 4 lateral tiles).
+        if first != 0 or count != g.nv:
+            raise ValueError("16-bit OPW runs on the whole grid")
+        nx, nz, ny = g.shape
         _lib.call(
-            "rcabeam_opw_bf16_device",
-            dev.ptr(packed_rc), dev.ptr(packed_cr), dev.ptr(scan), *common, ns, nc, na, nt, count, c, fs, fd, fn,
+            "rcabeam_opw_grid_device", dev.ptr(packed_rc), dev.ptr(packed_cr), _lib.STORAGE[storage],
+            dev.ptr(g.axes[0]), dev.ptr(g.axes[1]), dev.ptr(g.axes[2]), nx, nz, ny,
+            dev.ptr(g.xe), dev.ptr(g.ye), dev.ptr(g.angles), dev.ptr(g.starts), dev.ptr(pd_view), dev.ptr(sig_view),
+            signal_storage, ns, nc, na, nt, c, fs, fd, fn,
         )
 
 
