@@ -49,7 +49,7 @@ def parse():
     p.add_argument("--precision", choices=("float32", "mixed"), default="float32")
     p.add_argument("--grid", type=int, nargs=3, default=(128, 256, 128))
     p.add_argument("--no-cpu-baseline", action="store_true")
-    p.add_argument("--cpu-voxels", type=int, default=96)
+    p.add_argument("--cpu-voxels", type=int, default=400)
     return p.parse_args()
 
 
@@ -130,7 +130,7 @@ class Pipeline:
                        dev.ptr(self.work), self.work_bytes, 0)
         else:
             self._call("rcabeam_temporal_gram_device", dev.ptr(self.signal), self.code, nv, self.nt,
-                       dev.ptr(self.stats) + 8, dev.ptr(self.real), dev.ptr(self.work), self.work_bytes, 0)
+                       dev.ptr(self.stats), dev.ptr(self.real), dev.ptr(self.work), self.work_bytes, 0)
         self._call("rcabeam_gram_hermitian_device", dev.ptr(self.real), self.nt, 1.0 / nv, dev.ptr(self.gram))
         self._call("rcabeam_eigh_device", dev.ptr(self.gram), dev.ptr(self.values), dev.ptr(self.e_work),
                    dev.ptr(self.info), self.nt)

@@ -120,9 +120,9 @@ rcabeam_status rcabeam_pack_ensemble_device(const void* source, void* target, si
  * other methods write zero there).  signal: optional complex64 (nv, nt) for
  * methods 0-3 (method 3: real DMAS response, imaginary zero); method 4 needs a
  * null signal; method 5 needs complex64 (nv, 2, nt), RC then CR, and sets pd
- * to zero.  St-SW needs weights[nv] and complex64 workspace[nv*k*k*nt] and
- * 2 <= k <= 4 <= ... i.e. k <= na; normalise the weights by the full-volume
- * maximum before multiplying the power.
+ * to zero.  St-SW needs weights[nv], complex64 workspace[nv*k*k*nt] and
+ * 2 <= k <= min(4, na); normalise the weights by the full-volume maximum
+ * before multiplying the power.
  * geometry_workspace: must be non-null (reference contract); it is neither
  * read nor written here.  Results do not depend on alignment or on the
  * parity of nt.  reference include/rcabeam.h:187-207 */

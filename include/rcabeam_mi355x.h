@@ -83,14 +83,15 @@ rcabeam_status rcabeam_iq_absmax_device(const void* iq, int storage, size_t n, u
 size_t rcabeam_gram_workspace_bytes(size_t rows, size_t frames);
 size_t rcabeam_gram_real_elements(size_t frames);
 
-/* Real Gram C = A^T A of A = (iq * inv_scale) viewed as (rows, 2*frames)
- * reals, upper 32x32 blocks only, float64, row-major with leading dimension
+/* Real Gram C = A^T A of A = (iq / scale) viewed as (rows, 2*frames) reals,
+ * upper 32x32 blocks only, float64, row-major with leading dimension
  * 32*ceil(2*frames/32); accumulate != 0 adds to real_gram.  complex64 input
- * uses bfloat16 hi+lo split products on MFMA (inv_scale: device float, e.g.
- * out + 2 of rcabeam_iq_absmax_device); FP16/BF16 input is exact in one
- * product and ignores inv_scale.  frames <= 1024. */
+ * is divided by *scale (device float, e.g. out + 0 of
+ * rcabeam_iq_absmax_device; 0 means 1) and split into bfloat16 hi + lo, all
+ * four products on MFMA; FP16/BF16 input is exact in one product and ignores
+ * scale.  frames <= 1024. */
 rcabeam_status rcabeam_temporal_gram_device(const void* iq, int storage, size_t rows, size_t frames,
-                                            const float* inv_scale, double* real_gram, float* workspace,
+                                            const float* scale, double* real_gram, float* workspace,
                                             size_t workspace_bytes, int accumulate);
 
 /* Exact real Gram of complex64 input in float64 on v_mfma_f64_16x16x4_f64

@@ -54,7 +54,7 @@ _SIGNATURES: dict[str, list] = {
     "rcabeam_simulate_lds_bytes": [c_size_t, c_int],
     "rcabeam_simulate_channels_device": [c_void_p, c_void_p, c_void_p, c_size_t, c_void_p, c_int, c_void_p, c_void_p,
                                          c_size_t, c_void_p, c_void_p, c_void_p] + [c_size_t] * 4
-    + [c_int] + [c_float] * 7 + [c_ulonglong],
+    + [c_int] + [c_float] * 6 + [c_ulonglong],
     "rcabeam_gram_hermitian_device": [c_void_p, c_size_t, c_double, c_void_p],
     "rcabeam_eigh_device": [c_void_p, c_void_p, c_void_p, c_void_p, c_size_t],
     "rcabeam_filtered_power_device": [c_void_p, c_int, c_size_t, c_size_t, c_void_p, c_size_t, c_int, c_void_p],

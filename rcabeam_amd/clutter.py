@@ -126,8 +126,7 @@ def temporal_basis(iq, *, method: Solver = "gram", precision: str = "complex128"
             if nbytes == 0:
                 raise ValueError("Frame count out of range for the GPU Gram (1 <= T <= 1024)")
             work = dev.empty((nbytes // 4,), t.float32)
-            inv_scale_ptr = dev.ptr(stats) + 8  # out[2]
-            _lib.call("rcabeam_temporal_gram_device", dev.ptr(a), code, r, frames, inv_scale_ptr, dev.ptr(real),
+            _lib.call("rcabeam_temporal_gram_device", dev.ptr(a), code, r, frames, dev.ptr(stats), dev.ptr(real),
                       dev.ptr(work), nbytes, int(i > 0))
     # The float64 path forms the unscaled Gram; fold the 1/scale^2 of the reference's scaled Gram in here.
     scale2 = (1.0 / (scale * scale) if (exact and scale > 0) else 1.0) / rows

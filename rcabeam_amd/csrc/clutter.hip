@@ -10,9 +10,9 @@
 //   Re G[t][u] = C[2t][2u] + C[2t+1][2u+1],  Im G[t][u] = C[2t][2u+1] - C[2t+1][2u].
 // C is tiled into 32x32 blocks; only the upper-triangular blocks are formed,
 // each on v_mfma_f32_32x32x16_{bf16,f16}.  complex64 input is scaled by
-// 1/max(|Re|,|Im|) and split into bfloat16 hi + lo (x = hi + lo to 2^-16
-// relative), accumulating hi*hi + hi*lo + lo*hi; FP16/BF16 input is exact in
-// one product.  Each workgroup sums a bounded run of rows in float32 (the
+// 1/max(|Re|,|Im|) and split into bfloat16 hi + lo (x = hi + lo to ~2^-18
+// relative), accumulating all four products hi*hi + hi*lo + lo*hi + lo*lo;
+// FP16/BF16 input is exact in one product.  Each workgroup sums a bounded run of rows in float32 (the
 // rounding of short runs stays ~1e-8 relative after the cross-workgroup sum)
 // and writes a float32 partial; partials are reduced in float64.
 #include <rocblas/rocblas.h>
@@ -62,7 +62,7 @@ __device__ __forceinline__ float bf16_bits_to_float(unsigned short b) { return _
 // MODE kExactBf16: T = bf16x2, one bf16 product.
 template <int MODE, typename T>
 __global__ __launch_bounds__(kGramThreads) void k_gram(const T* __restrict__ x, long long rows, int frames,
-                                                      const float* __restrict__ inv_scale_ptr, GramPlan plan,
+                                                      const float* __restrict__ scale_ptr, GramPlan plan,
                                                       float* __restrict__ partials) {
   constexpr bool kSplit = MODE == kSplitBf16;
   extern __shared__ __align__(16) unsigned short opnd[];  // [2 (hi, lo)][cols][kGramKPad]
@@ -71,7 +71,9 @@ __global__ __launch_bounds__(kGramThreads) void k_gram(const T* __restrict__ x, 
   unsigned short* lo = opnd + cols * kGramKPad;
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = threadIdx.x / kWave;
-  const float inv_scale = kSplit ? *inv_scale_ptr : 1.0f;
+  // 1 / max(|Re|, |Im|) in float64: the max may be a float32 subnormal whose inverse overflows float32.
+  const float maxabs = kSplit ? *scale_ptr : 1.0f;
+  const double inv_scale = maxabs > 0.f ? 1.0 / static_cast<double>(maxabs) : 1.0;
   const long long row0 = static_cast<long long>(blockIdx.x) * kGramRowsPerGroup;
   const long long row_end = row0 + kGramRowsPerGroup < rows ? row0 + kGramRowsPerGroup : rows;
   const int steps = static_cast<int>((row_end - row0 + kGramK - 1) / kGramK);
@@ -111,8 +113,8 @@ __global__ __launch_bounds__(kGramThreads) void k_gram(const T* __restrict__ x, 
             hr = __half_as_ushort(__float2half_rn(v.x));
             hi_i = __half_as_ushort(__float2half_rn(v.y));
           } else {
-            v.x *= inv_scale;
-            v.y *= inv_scale;
+            v.x = static_cast<float>(static_cast<double>(v.x) * inv_scale);
+            v.y = static_cast<float>(static_cast<double>(v.y) * inv_scale);
             hr = bf16_rn_bits(v.x);
             hi_i = bf16_rn_bits(v.y);
             if constexpr (kSplit) {
@@ -153,6 +155,8 @@ __global__ __launch_bounds__(kGramThreads) void k_gram(const T* __restrict__ x, 
             const bf16x8 b_lo = *reinterpret_cast<const bf16x8*>(lo + (32 * J + (lane & 31)) * kGramKPad + koff);
             acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_hi, b_lo, acc[b], 0, 0, 0);
             acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_lo, b_hi, acc[b], 0, 0, 0);
+            // lo * lo too: dropping it biases the diagonal by -sum lo^2 (always one sign).
+            acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_lo, b_lo, acc[b], 0, 0, 0);
           }
         }
       }
@@ -338,8 +342,10 @@ __global__ __launch_bounds__(256) void k_absmax(const T* __restrict__ x, long lo
 }
 
 // ---- projector + filtered power ---------------------------------------------
-// One wave per Casorati row.  coefficients c_k = sum_t x_t conj(V[t,k]) and
-// the residual / retained energy are formed in float64 (the operation is
+// One wave per Casorati row x (a row vector).  Coefficients c = x V, i.e.
+// c_k = sum_t x_t V[t,k], then x - c V^H (subtract form) or |c|^2 (retain
+// form), as block @ vectors @ vectors^H in reference gpu_clutter.py:215.
+// The residual / retained energy are formed in float64 (the operation is
 // HBM-bound; float64 removes the cancellation of x - P x for strong clutter).
 template <typename T, bool FILTER, int kMaxPerLane>
 __global__ __launch_bounds__(256) void k_project(const T* __restrict__ x, long long rows, int frames,
@@ -368,8 +374,8 @@ __global__ __launch_bounds__(256) void k_project(const T* __restrict__ x, long l
       const int t = lane + kWave * j;
       if (t < frames) {
         const double2 b = vk[t];
-        cr += xv[j].x * b.x + xv[j].y * b.y;  // x * conj(b)
-        ci += xv[j].y * b.x - xv[j].x * b.y;
+        cr += xv[j].x * b.x - xv[j].y * b.y;  // c_k = sum_t x_t V[t, k]
+        ci += xv[j].x * b.y + xv[j].y * b.x;
       }
     }
 #pragma unroll
@@ -386,9 +392,9 @@ __global__ __launch_bounds__(256) void k_project(const T* __restrict__ x, long l
     for (int j = 0; j < kMaxPerLane; ++j) {
       const int t = lane + kWave * j;
       if (t < frames) {
-        const double2 b = vk[t];
-        resid[j].x += sign * (cr * b.x - ci * b.y);
-        resid[j].y += sign * (cr * b.y + ci * b.x);
+        const double2 b = vk[t];  // projection c_k conj(V[t, k]) (row-vector convention x V V^H)
+        resid[j].x += sign * (cr * b.x + ci * b.y);
+        resid[j].y += sign * (ci * b.x - cr * b.y);
       }
     }
   }
@@ -448,7 +454,7 @@ static void make_plan(int nb, int pass, GramPlan& plan) {
 static size_t lds_bytes(int nb) { return static_cast<size_t>(2) * nb * 32 * kGramKPad * sizeof(unsigned short); }
 
 template <int MODE, typename T>
-static hipError_t gram_launch(const void* x, long long rows, int frames, const float* inv_scale, double* c,
+static hipError_t gram_launch(const void* x, long long rows, int frames, const float* scale, double* c,
                               float* workspace, int accumulate) {
   const int nb = column_blocks(frames);
   const long long groups = (rows + kGramRowsPerGroup - 1) / kGramRowsPerGroup;
@@ -462,7 +468,7 @@ static hipError_t gram_launch(const void* x, long long rows, int frames, const f
     GramPlan plan;
     make_plan(nb, pass, plan);
     k_gram<MODE, T><<<dim3(static_cast<unsigned>(groups)), kGramThreads, lds>>>(
-        static_cast<const T*>(x), rows, frames, inv_scale, plan, workspace);
+        static_cast<const T*>(x), rows, frames, scale, plan, workspace);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     const long long n = static_cast<long long>(plan.pass_blocks) * 1024;
     k_gram_reduce<<<dim3(static_cast<unsigned>((n + 255) / 256)), 256>>>(workspace, static_cast<int>(groups), plan, c,
@@ -544,10 +550,10 @@ extern "C" rcabeam_status rcabeam_iq_absmax_device(const void* iq, int storage, 
 }
 
 extern "C" rcabeam_status rcabeam_temporal_gram_device(const void* iq, int storage, size_t rows, size_t frames,
-                                                       const float* inv_scale, double* real_gram, float* workspace,
+                                                       const float* scale, double* real_gram, float* workspace,
                                                        size_t workspace_bytes, int accumulate) {
   if (!iq || !real_gram || !workspace || !rows || !frames || storage < 0 || storage > 2) return RCABEAM_ERROR_ARGUMENT;
-  if (storage == RCABEAM_STORAGE_C64 && !inv_scale) return RCABEAM_ERROR_ARGUMENT;
+  if (storage == RCABEAM_STORAGE_C64 && !scale) return RCABEAM_ERROR_ARGUMENT;
   const size_t need = rcabeam_gram_workspace_bytes(rows, frames);
   if (!need || workspace_bytes < need) return RCABEAM_ERROR_ARGUMENT;
   hipError_t e;
@@ -555,13 +561,13 @@ extern "C" rcabeam_status rcabeam_temporal_gram_device(const void* iq, int stora
   const int t = static_cast<int>(frames);
   switch (storage) {
     case RCABEAM_STORAGE_C64:
-      e = gram_launch<kSplitBf16, float2>(iq, n, t, inv_scale, real_gram, workspace, accumulate);
+      e = gram_launch<kSplitBf16, float2>(iq, n, t, scale, real_gram, workspace, accumulate);
       break;
     case RCABEAM_STORAGE_FP16:
-      e = gram_launch<kExactF16, __half2>(iq, n, t, inv_scale, real_gram, workspace, accumulate);
+      e = gram_launch<kExactF16, __half2>(iq, n, t, scale, real_gram, workspace, accumulate);
       break;
     default:
-      e = gram_launch<kExactBf16, bf16x2>(iq, n, t, inv_scale, real_gram, workspace, accumulate);
+      e = gram_launch<kExactBf16, bf16x2>(iq, n, t, scale, real_gram, workspace, accumulate);
   }
   if (e == hipErrorInvalidValue) return RCABEAM_ERROR_ARGUMENT;
   return e == hipSuccess ? RCABEAM_SUCCESS : RCABEAM_ERROR_CUDA;

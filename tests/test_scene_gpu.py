@@ -36,4 +36,5 @@ def test_mixture_scene_is_seeded_and_finite():
         assert x.shape == (128, 8, 3, 6)
         h = x.cpu().numpy()
         assert np.isfinite(h).all() and np.abs(h).max() > 0
-        np.testing.assert_array_equal(h, y.cpu().numpy())
+        # Echoes are summed with LDS float atomics, so the order of additions (and the last bit) varies.
+        np.testing.assert_allclose(h, y.cpu().numpy(), rtol=0, atol=1e-6 * np.abs(h).max())
