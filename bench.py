@@ -10,9 +10,10 @@ tube segments, -60 dB noise.
 
 One step = pack RC and CR -> frame-batched OPW delay-and-sum (complex64 IQ in
 HBM) -> scaled Gram on MFMA -> float64 eigendecomposition -> filtered power,
-starting from channel data resident in HBM.  `--precision mixed` stores
-channels and beamformed IQ as FP16 (float32 accumulation) and forms the Gram
-exactly from the 16-bit IQ.
+starting from channel data resident in HBM.  `--precision float16` (or
+`bfloat16`) is the mixed-precision form: channels and beamformed IQ stored as
+16-bit, delay-and-sum with one 16-bit coefficient per sample (exact products,
+float32 accumulation), Gram formed exactly from the 16-bit IQ.
 
 Prints one JSON line.  `value` is voxel-frames/s of the whole step.
 """
@@ -46,11 +47,28 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=3)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--precision", choices=("float32", "mixed"), default="float32")
+    p.add_argument("--precision", choices=("float32", "float16", "bfloat16"), default="float32",
+                   help="float32: complex64 channels and IQ; float16 / bfloat16: 16-bit channels and IQ (mixed precision)")
     p.add_argument("--grid", type=int, nargs=3, default=(128, 256, 128))
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-voxels", type=int, default=400)
+    p.add_argument("--copy-calibration", action="store_true",
+                   help="only run a plain 1 GiB device-to-device copy three times (HBM counter calibration)")
     return p.parse_args()
+
+
+def copy_calibration() -> int:
+    """Three plain copies of 1 GiB: the reference point for FETCH_SIZE / WRITE_SIZE scaling."""
+    import torch
+
+    n = 1 << 30
+    src = torch.ones(n // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n, "copy_bytes_written": n, "copies": 3}), flush=True)
+    return 0
 
 
 class Pipeline:
@@ -107,8 +125,8 @@ class Pipeline:
             if self.storage == "float32":
                 self._call("rcabeam_pack_ensemble_device", dev.ptr(raw), dev.ptr(packed), *shape)
             else:
-                self._call("rcabeam_pack_ensemble_fp16_device", dev.ptr(raw), dev.ptr(packed), dev.ptr(self.range_error),
-                           *shape)
+                pack = "rcabeam_pack_ensemble_fp16_device" if self.storage == "float16" else "rcabeam_pack_ensemble_bf16_device"
+                self._call(pack, dev.ptr(raw), dev.ptr(packed), dev.ptr(self.range_error), *shape)
         common = (dev.ptr(self.packed[0]), dev.ptr(self.packed[1]), dev.ptr(self.g.scan), dev.ptr(self.g.xe),
                   dev.ptr(self.g.ye), dev.ptr(self.g.angles), dev.ptr(self.g.starts), dev.ptr(self.pd))
         if das_events is not None:
@@ -117,11 +135,9 @@ class Pipeline:
             self._call("rcabeam_ensemble_device", *common, dev.ptr(self.signal), 0, 0, dev.ptr(self.g.geometry_workspace),
                        *shape, nv, c, fs, fd, fn, 0, 2)
         else:
-            nx, nz, ny = self.g.shape
-            self._call("rcabeam_opw_grid_device", dev.ptr(self.packed[0]), dev.ptr(self.packed[1]), self.code,
-                       dev.ptr(self.g.axes[0]), dev.ptr(self.g.axes[1]), dev.ptr(self.g.axes[2]), nx, nz, ny,
-                       dev.ptr(self.g.xe), dev.ptr(self.g.ye), dev.ptr(self.g.angles), dev.ptr(self.g.starts),
-                       dev.ptr(self.pd), dev.ptr(self.signal), self.code, *shape, c, fs, fd, fn)
+            # 16-bit channels and IQ, one 16-bit coefficient per sample (dot2, float32 accumulation).
+            self._call("rcabeam_opw_mixed_device", *common, dev.ptr(self.signal), self.code, self.code,
+                       _lib_coef_16bit(), *shape, nv, c, fs, fd, fn)
         if das_events is not None:
             das_events[1].record()
         self._call("rcabeam_iq_absmax_device", dev.ptr(self.signal), self.code, nv * self.nt, dev.ptr(self.stats), 0)
@@ -139,6 +155,12 @@ class Pipeline:
         self.vectors.copy_(self.gram[self.nt - self.rank_start:].flip(0))
         self._call("rcabeam_filtered_power_device", dev.ptr(self.signal), self.code, nv, self.nt,
                    dev.ptr(self.vectors), self.rank_start, 1, dev.ptr(self.power))
+
+
+def _lib_coef_16bit() -> int:
+    from rcabeam_amd import _lib
+
+    return _lib.COEF_16BIT
 
 
 def cpu_baseline(scene, rc, cr, rank_start, voxels: int):
@@ -181,6 +203,8 @@ def traffic_from_profiles(name: str):
 
 def main() -> int:
     args = parse()
+    if args.copy_calibration:
+        return copy_calibration()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); see the sharding note in DESIGN.md",
               file=sys.stderr)
@@ -195,7 +219,7 @@ def main() -> int:
     rank_start = 3
     log(f"bench: generating scene grid={tuple(args.grid)}")
     scene, rc, cr = mixture_scene(grid_shape=tuple(args.grid))
-    storage = "float32" if args.precision == "float32" else "float16"
+    storage = args.precision
     pipe = Pipeline(scene, rc, cr, storage, rank_start)
     nv, nt = pipe.g.nv, pipe.nt
     counter = dev.empty((1,), torch.int64)
@@ -233,7 +257,7 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if storage == "float32" else "f16-storage/f32",
+        "dtype": {"float32": "f32", "float16": "f16-storage/f32-accumulate", "bfloat16": "bf16-storage/f32-accumulate"}[storage],
         "data": "synthetic (GPU-simulated tissue + blood + noise scene, seed 42)",
         "config": {
             "workload": "filtered OPW PD: 128+128 el, IQ 1024x128x11x200 per aperture, grid "
@@ -245,7 +269,7 @@ def main() -> int:
         "roofline": {
             "bound": "mfma",
             "engine": "fp32 VALU (delay-and-sum has no GEMM form at this sparsity; fp32 peak used as the roof)",
-            "kernel": "das_angle_major (OPW)",
+            "kernel": "das_opw_tile (OPW delay-and-sum)",
             "achieved": achieved,
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",

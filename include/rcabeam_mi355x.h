@@ -37,14 +37,26 @@ rcabeam_status rcabeam_opw_bf16_device(const void* rc, const void* cr, const flo
                                        void* signal, size_t ns, size_t nc, size_t na, size_t nt, size_t nv, float c,
                                        float fs, float fd, float fn);
 
+/* Tap-coefficient precision of the 16-bit delay-and-sum. */
+typedef enum rcabeam_coefficients {
+    RCABEAM_COEF_FLOAT32 = 0, /* float32-accurate: samples widened to float32, or FP16 samples times
+                                 hi + lo FP16 coefficients (exact products, float32 accumulation) */
+    RCABEAM_COEF_16BIT = 1,   /* one coefficient of the channel storage type per sample
+                                 (interpolation weight x phasor), exact 16-bit products,
+                                 float32 accumulation: half the arithmetic of RCABEAM_COEF_FLOAT32 */
+} rcabeam_coefficients;
+
 /* OPW with any channel storage and any beamformed-IQ storage (rcabeam_storage
- * codes); float32 interpolation and accumulation.  signal (optional) holds
- * (nv, nt) samples of signal_storage; pd as rcabeam_ensemble_device.  The
- * 16-bit signal is the mixed-precision form of the clutter-filter input. */
+ * codes); float32 accumulation, coefficient precision per rcabeam_coefficients
+ * (ignored for complex64 channels).  signal (optional) holds (nv, nt) samples
+ * of signal_storage; pd as rcabeam_ensemble_device.  The 16-bit signal is the
+ * mixed-precision form of the clutter-filter input.  Extends reference
+ * src/rcabeam/ensemble.cu:798-809 (rcabeam_opw_fp16_device). */
 rcabeam_status rcabeam_opw_mixed_device(const void* rc, const void* cr, const float* scan, const float* xe,
                                         const float* ye, const float* angles, const float* starts, float* pd,
-                                        void* signal, int channel_storage, int signal_storage, size_t ns, size_t nc,
-                                        size_t na, size_t nt, size_t nv, float c, float fs, float fd, float fn);
+                                        void* signal, int channel_storage, int signal_storage, int coefficients,
+                                        size_t ns, size_t nc, size_t na, size_t nt, size_t nv, float c, float fs,
+                                        float fd, float fn);
 
 /* OPW on a Cartesian grid from 16-bit packed channels (channel_storage FP16
  * or BF16), delay-and-sum on MFMA: 4 x 4 lateral voxel tiles, per

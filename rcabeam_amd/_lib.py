@@ -21,6 +21,9 @@ LIBRARY_PATH = Path(os.environ.get("RCABEAM_LIBRARY", Path(__file__).resolve().p
 
 SUCCESS, ERROR_DEVICE, ERROR_ARGUMENT = 0, 1, 2
 STORAGE = {"float32": 0, "complex64": 0, "float16": 1, "bfloat16": 2}
+# rcabeam_coefficients (include/rcabeam_mi355x.h)
+COEF_FLOAT32 = 0
+COEF_16BIT = 1
 
 # name -> argument types (all return rcabeam_status as int unless listed in _RESTYPE)
 _SIGNATURES: dict[str, list] = {
@@ -41,7 +44,7 @@ _SIGNATURES: dict[str, list] = {
     "rcabeam_ensemble_device": [c_void_p] * 12 + [c_size_t] * 5 + [c_float] * 4 + [c_int, c_int],
     "rcabeam_opw_fp16_device": [c_void_p] * 10 + [c_size_t] * 5 + [c_float] * 4,
     "rcabeam_opw_bf16_device": [c_void_p] * 9 + [c_size_t] * 5 + [c_float] * 4,
-    "rcabeam_opw_mixed_device": [c_void_p] * 9 + [c_int, c_int] + [c_size_t] * 5 + [c_float] * 4,
+    "rcabeam_opw_mixed_device": [c_void_p] * 9 + [c_int, c_int, c_int] + [c_size_t] * 5 + [c_float] * 4,
     "rcabeam_opw_grid_device": [c_void_p, c_void_p, c_int] + [c_void_p] * 3 + [c_size_t] * 3 + [c_void_p] * 6
     + [c_int] + [c_size_t] * 4 + [c_float] * 4,
     "rcabeam_count_active_taps_device": [c_void_p] * 5 + [c_size_t] * 4 + [c_float] * 4 + [c_void_p],

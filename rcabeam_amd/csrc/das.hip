@@ -19,7 +19,10 @@
 // Blocks are mapped XCD-aware: block b runs on XCD b % 8, so each XCD gets a
 // contiguous run of voxel tiles and neighbouring tiles share that XCD's L2.
 #include "common.hpp"
+#include "ensemble.hpp"
 #include "rcabeam_mi355x.h"
+
+#include <type_traits>
 
 namespace rcb {
 
@@ -30,31 +33,6 @@ constexpr int kWavesPerBlock = kBlock / kWave;
 #define RCB_OPW_VPW 2
 #endif
 constexpr size_t kLdsLimit = 64 * 1024;
-
-struct EnsembleArgs {
-  const void* rc;
-  const void* cr;
-  const float* scan;
-  const float* xe;
-  const float* ye;
-  const float* angles;
-  const float* starts;
-  float* pd;
-  float2* signal;
-  float* weights;
-  float2* workspace;
-  int ns, nc, na, nt;
-  long long nv;
-  double c, fs, fd, fn;
-};
-
-enum Method { kOpw = 0, kXdoppler = 1, kRcFmas = 2, kDmas = 3, kStsw = 4, kApertures = 5 };
-
-__device__ __forceinline__ long long xcd_tile(long long tiles) {
-  long long per_xcd = (tiles + 7) / 8;
-  long long b = blockIdx.x;
-  return (b % 8) * per_xcd + b / 8;
-}
 
 // Fill rx[vb * nc + ch] and tx[vb * na + a] for one aperture configuration.
 __device__ void fill_terms(const EnsembleArgs& p, int cfg, long long v0, int vb_count, Term* rx, Term* tx) {
@@ -530,7 +508,22 @@ static hipError_t launch_dmas(const EnsembleArgs& p) {
 }
 
 template <typename T>
+constexpr int storage_code() {
+  return std::is_same<T, float2>::value ? RCABEAM_STORAGE_C64
+                                        : (std::is_same<T, __half2>::value ? RCABEAM_STORAGE_FP16 : RCABEAM_STORAGE_BF16);
+}
+
+// Float32-accurate coefficients for a channel storage: FP16 samples use dot2
+// with hi + lo coefficients, the others widen samples to float32.
+static int accurate_coefficients(int storage) { return storage == RCABEAM_STORAGE_FP16 ? kCoefSplit : kCoefFloat; }
+
+template <typename T>
 static hipError_t dispatch(const EnsembleArgs& p, int method, int k) {
+  if (method == kOpw || method == kApertures) {
+    const hipError_t e = launch_opw_tile(p, storage_code<T>(), accurate_coefficients(storage_code<T>()),
+                                         RCABEAM_STORAGE_C64, method == kApertures);
+    if (e != hipErrorNotSupported) return e;
+  }
   switch (method) {
     case kOpw:
       return launch_angle_major<kOpw, T, RCB_OPW_VPW, 1>(p);
@@ -688,14 +681,22 @@ static hipError_t dispatch_opw_out(const EnsembleArgs& p, int out_storage) {
 extern "C" rcabeam_status rcabeam_opw_mixed_device(const void* rc, const void* cr, const float* scan, const float* xe,
                                                    const float* ye, const float* angles, const float* starts,
                                                    float* pd, void* signal, int channel_storage, int signal_storage,
-                                                   size_t ns, size_t nc, size_t na, size_t nt, size_t nv, float c,
-                                                   float fs, float fd, float fn) {
+                                                   int coefficients, size_t ns, size_t nc, size_t na, size_t nt,
+                                                   size_t nv, float c, float fs, float fd, float fn) {
   if (reinterpret_cast<uintptr_t>(rc) % 4 || reinterpret_cast<uintptr_t>(cr) % 4) return RCABEAM_ERROR_ARGUMENT;
   if (!valid_ensemble(rc, cr, scan, xe, ye, angles, starts, pd, ns, nc, na, nt, nv, c, fs, fd, fn))
     return RCABEAM_ERROR_ARGUMENT;
+  if (channel_storage < RCABEAM_STORAGE_C64 || channel_storage > RCABEAM_STORAGE_BF16 ||
+      signal_storage < RCABEAM_STORAGE_C64 || signal_storage > RCABEAM_STORAGE_BF16 ||
+      (coefficients != RCABEAM_COEF_FLOAT32 && coefficients != RCABEAM_COEF_16BIT))
+    return RCABEAM_ERROR_ARGUMENT;
   EnsembleArgs p = make_args(rc, cr, scan, xe, ye, angles, starts, pd, signal, nullptr, nullptr, ns, nc, na, nt, nv,
                              c, fs, fd, fn);
-  hipError_t e;
+  const int coef = coefficients == RCABEAM_COEF_16BIT && channel_storage != RCABEAM_STORAGE_C64
+                       ? kCoefSingle
+                       : accurate_coefficients(channel_storage);
+  hipError_t e = launch_opw_tile(p, channel_storage, coef, signal_storage, false);
+  if (e != hipErrorNotSupported) return to_status(e);
   switch (channel_storage) {
     case RCABEAM_STORAGE_C64:
       e = dispatch_opw_out<float2>(p, signal_storage);

@@ -1,0 +1,50 @@
+// Arguments and helpers shared by the frame-batched delay-and-sum kernels.
+#pragma once
+
+#include "common.hpp"
+
+namespace rcb {
+
+struct EnsembleArgs {
+  const void* rc;  // packed (channels, angles, samples, frames) RC channels
+  const void* cr;  // same for CR
+  const float* scan;
+  const float* xe;
+  const float* ye;
+  const float* angles;
+  const float* starts;
+  float* pd;
+  float2* signal;
+  float* weights;
+  float2* workspace;
+  int ns, nc, na, nt;
+  long long nv;
+  double c, fs, fd, fn;
+};
+
+enum Method { kOpw = 0, kXdoppler = 1, kRcFmas = 2, kDmas = 3, kStsw = 4, kApertures = 5 };
+
+// How the LDS-tiled kernel multiplies samples by tap coefficients.
+enum Coefficients {
+  kCoefFloat = 0,   // samples widened to float32, lerp + phasor rotation in float32
+  kCoefSplit = 1,   // 16-bit samples: dot2 with hi + lo 16-bit coefficients (float32-accurate)
+  kCoefSingle = 2,  // 16-bit samples: dot2 with one 16-bit coefficient (mixed-precision fast form)
+};
+
+// Blocks are dispatched round-robin over the 8 XCDs of an MI355X; give every
+// XCD a contiguous run of tiles so that concurrent neighbouring tiles share
+// that XCD's L2.  The grid is padded to a multiple of 8.
+__device__ __forceinline__ long long xcd_tile(long long tiles) {
+  long long per_xcd = (tiles + 7) / 8;
+  long long b = blockIdx.x;
+  return (b % 8) * per_xcd + b / 8;
+}
+
+// LDS-tiled OPW / separate-aperture IQ (das_tile.hip).  Returns
+// hipErrorNotSupported when the shape is outside what the tiled kernel
+// handles (the caller then uses the angle-major kernel), hipErrorInvalidValue
+// for invalid storage codes.
+hipError_t launch_opw_tile(const EnsembleArgs& p, int channel_storage, int coefficients, int signal_storage,
+                           bool apertures);
+
+}  // namespace rcb

@@ -126,8 +126,8 @@ def run_ensemble(packed_rc, packed_cr, g: _Geometry, shape4, method: int, k: int
                  weights=None, first: int = 0, count: int | None = None, signal_storage: int = 0):
     """Enqueue one reconstruction of voxels [first, first + count) over the C ABI.
 
-    float32 channels run the frame-batched VALU kernels (rcabeam_ensemble_device);
-    16-bit channels (OPW only) run the MFMA grid kernel (rcabeam_opw_grid_device).
+    float32 channels run rcabeam_ensemble_device; 16-bit channels (OPW only) run
+    rcabeam_opw_mixed_device with float32-accurate coefficients.
     """
     t = dev.torch()
     ns, nc, na, nt = shape4
@@ -150,15 +150,11 @@ def run_ensemble(packed_rc, packed_cr, g: _Geometry, shape4, method: int, k: int
             dev.ptr(g.geometry_workspace), ns, nc, na, nt, count, c, fs, fd, fn, method, k,
         )
     else:
-        # 16-bit channels: delay-and-sum on MFMA over the Cartesian grid (4 x 4 lateral tiles).
-        if first != 0 or count != g.nv:
-            raise ValueError("16-bit OPW runs on the whole grid")
-        nx, nz, ny = g.shape
+        # 16-bit channels (OPW only): float32-accurate coefficients, float32 accumulation.
         _lib.call(
-            "rcabeam_opw_grid_device", dev.ptr(packed_rc), dev.ptr(packed_cr), _lib.STORAGE[storage],
-            dev.ptr(g.axes[0]), dev.ptr(g.axes[1]), dev.ptr(g.axes[2]), nx, nz, ny,
-            dev.ptr(g.xe), dev.ptr(g.ye), dev.ptr(g.angles), dev.ptr(g.starts), dev.ptr(pd_view), dev.ptr(sig_view),
-            signal_storage, ns, nc, na, nt, c, fs, fd, fn,
+            "rcabeam_opw_mixed_device",
+            dev.ptr(packed_rc), dev.ptr(packed_cr), dev.ptr(scan), *common, _lib.STORAGE[storage], signal_storage,
+            _lib.COEF_FLOAT32, ns, nc, na, nt, count, c, fs, fd, fn,
         )
 
 

@@ -151,9 +151,8 @@ def test_half_storage_matches_quantized_fp32(storage, dtype, frames):
         actual = rb.beamform_ensemble(a, b, angles, 1e-6, grid, geom, iq_storage=storage)
         pd = rb.ensemble_pd_from_channels(a, b, angles, 1e-6, grid, geom, iq_storage=storage)
         assert actual.dtype == np.complex64 and pd.dtype == np.float32
-        # 16-bit channels run the MFMA kernel: coefficients are split hi + lo, i.e. 22 bits (FP16)
-        # or 16 bits (BF16), on top of float32 accumulation.
-        tol = 1e-6 if storage == "float16" else 3e-5
+        # FP16 samples times hi + lo FP16 coefficients (exact products); BF16 samples widened to float32.
+        tol = 1e-6
         assert rel_l2(actual, expected) < tol
         assert rel_l2(pd, np.mean(np.abs(actual) ** 2, axis=-1)) < 1e-6
 
@@ -175,8 +174,8 @@ def test_half_storage_rejects_invalid_values_and_methods():
 
 
 @pytest.mark.parametrize("grid_shape", [(16, 32, 16), (7, 9, 6)])
-def test_mfma_opw_on_quantized_point_scene(grid_shape):
-    """MFMA delay-and-sum (FP16 channels) equals the float64 oracle on the FP16-rounded channels."""
+def test_fp16_opw_on_quantized_point_scene(grid_shape):
+    """FP16 channels equal the float64 oracle on the FP16-rounded channels."""
     acq = scene.point_scene(grid_shape=grid_shape)
     q16 = [(x.real.astype(np.float16).astype(np.float32) + 1j * x.imag.astype(np.float16).astype(np.float32)).astype(
         np.complex64) for x in (acq.rc, acq.cr)]
@@ -193,3 +192,74 @@ def test_mfma_opw_on_quantized_point_scene(grid_shape):
     (ra, _, _), (qa, _, _) = _oracle(acq)
     exact = ra.sum(axis=1) + qa.sum(axis=1)
     assert rel_l2(pd.ravel(), combiners.power_doppler(exact)) < 1e-3
+
+
+def _device_run(acq, storage, coefficients, order=None):
+    """rcabeam_opw_mixed_device through the C ABI; returns (IQ (nv, nt) complex64, power (nv,))."""
+    import torch
+
+    from rcabeam_amd import _device as dev
+    from rcabeam_amd import _lib
+    from rcabeam_amd.ensemble import _Geometry, pack_channels
+
+    lib = _lib.load()
+    g = _Geometry(acq.angles, acq.t_start, acq.grid, _geom(acq), order=order)
+    ns, nc, na, nt = acq.rc.shape
+    packed = [pack_channels(dev.to_device(x, torch.complex64), storage) for x in (acq.rc, acq.cr)]
+    pd = dev.empty((g.nv, 2), torch.float32)
+    signal = dev.empty((g.nv, nt), torch.complex64)
+    c, fs, fd, fn = g.constants
+    _lib.check(lib.rcabeam_opw_mixed_device(
+        dev.ptr(packed[0]), dev.ptr(packed[1]), dev.ptr(g.scan), dev.ptr(g.xe), dev.ptr(g.ye), dev.ptr(g.angles),
+        dev.ptr(g.starts), dev.ptr(pd), dev.ptr(signal), _lib.STORAGE[storage], _lib.STORAGE["complex64"],
+        coefficients, ns, nc, na, nt, g.nv, c, fs, fd, fn), "opw_mixed")
+    dev.synchronize()
+    return signal.cpu().numpy(), pd.cpu().numpy()[:, 0]
+
+
+@pytest.mark.parametrize("fnumber", [None, 1.0])
+def test_scattered_voxel_order_stages_windows_in_chunks(fnumber):
+    """Voxels in random order: a 64-voxel tile spans the whole volume, so its sample window is
+    longer than one LDS chunk and is staged in overlapping pieces (some without any tap)."""
+    from rcabeam_amd import _lib
+
+    acq = scene.point_scene(grid_shape=(12, 40, 12))
+    acq = scene.Acquisition(acq.rc, acq.cr, acq.x_el, acq.y_el, acq.angles, acq.t_start, acq.grid, acq.c, acq.fs,
+                            acq.fd, fnumber)
+    nv = 12 * 40 * 12
+    order = np.random.default_rng(5).permutation(nv)
+    print(f"case: scattered order fnumber={fnumber}", flush=True)
+    scan = das.scan_coords(acq.grid)[order]
+    (ra, _, _), (qa, _, _) = das.delay_apertures(acq.rc, acq.cr, scan, acq.x_el, acq.y_el, acq.angles, acq.t_start,
+                                                 acq.c, acq.fs, acq.fd, fnumber)
+    expected = ra.sum(axis=1) + qa.sum(axis=1)
+    iq, power = _device_run(acq, "float32", _lib.COEF_FLOAT32, order=order)
+    assert rel_l2(iq, expected) < TOL
+    assert rel_l2(power, combiners.power_doppler(expected)) < TOL
+
+
+@pytest.mark.parametrize("storage", ["float16", "bfloat16"])
+def test_sixteen_bit_coefficients_error(storage):
+    """RCABEAM_COEF_16BIT: one 16-bit coefficient per sample.  Against the float64 oracle of the
+    rounded channels the only error is the coefficient rounding (2^-11 FP16, 2^-8 BF16 relative
+    per tap, averaged over taps); the float32-accurate form stays at float32 rounding."""
+    import torch
+
+    from rcabeam_amd import _lib
+
+    acq = scene.point_scene(grid_shape=(16, 32, 16))
+    t = [torch.from_numpy(np.ascontiguousarray(x.view(np.float32))) for x in (acq.rc, acq.cr)]
+    dtype = torch.float16 if storage == "float16" else torch.bfloat16
+    rounded = [x.to(dtype).to(torch.float32).numpy().view(np.complex64) for x in t]
+    acq_r = scene.Acquisition(rounded[0], rounded[1], acq.x_el, acq.y_el, acq.angles, acq.t_start, acq.grid, acq.c,
+                              acq.fs, acq.fd, acq.fnumber)
+    (ra, _, _), (qa, _, _) = _oracle(acq_r)
+    expected = ra.sum(axis=1) + qa.sum(axis=1)
+    accurate, p_acc = _device_run(acq, storage, _lib.COEF_FLOAT32)
+    fast, p_fast = _device_run(acq, storage, _lib.COEF_16BIT)
+    assert rel_l2(accurate, expected) < TOL
+    e_iq = rel_l2(fast, expected)
+    e_pd = rel_l2(p_fast, combiners.power_doppler(expected))
+    print(f"{storage}: 16-bit coefficients IQ rel L2 {e_iq:.2e}, power {e_pd:.2e}", flush=True)
+    assert e_iq < (5e-4 if storage == "float16" else 4e-3)
+    assert e_pd < (1e-3 if storage == "float16" else 8e-3)
