@@ -4,25 +4,23 @@
 // src/rcabeam/ensemble.cu:341-454 computes the same sums with per-tap global
 // gathers).
 //
-// Work decomposition (one 512-thread workgroup = 8 wave64s):
+// Work decomposition (one 512-thread workgroup = 8 wave64s, two per CU):
 //   * a workgroup owns a tile of 64 consecutive voxels (y fastest: on a
 //     Cartesian grid, half a y line at fixed x and z) and every frame, so the
 //     power reduction is complete inside the workgroup (deterministic pd);
 //   * wave w owns voxels 8w .. 8w+7; the 8 lanes of a voxel own frames
-//     f0 + lane%8 + 8j, j < FPL, for NFB frame blocks of 8*FPL frames;
-//   * loop order aperture -> channel -> angle.  For every (channel, angle) the
-//     workgroup stages the sample rows that its 64 voxels touch -- the window
-//     [min tap row, max tap row + 1] over the frames of a block -- from the
+//     lane%8 + 8j, j < FPL (all frames at once, up to 208);
+//   * loop order aperture -> channel -> angle.  Per channel the workgroup
+//     builds the tap table of its 64 voxels x angles once (receive and
+//     transmit terms in float64, combined into row, weight and phasor) and,
+//     per angle, the exact row window [min first row, max second row] by
+//     wave reductions;
+//   * per (channel, angle) the window rows x all frames are staged from the
 //     packed (channels, angles, samples, frames) layout into LDS with 16-byte
-//     loads, then every lane interpolates its own voxel's tap from LDS;
-//   * the window is bounded without touching samples: the receive term of an
-//     RC voxel depends on (x, z) and the transmit term on (y, z), so over a
-//     y line one of the two is constant and min/max of (receive row) plus
-//     min/max of (transmit row) is the exact window.  Windows longer than
-//     max_rows (40, fewer when LDS is short) are staged in chunks that overlap
-//     by one row; a chunk no tap of the tile falls in is skipped (block vote);
-//   * receive terms (64 voxels x 8 channels) and transmit terms (64 voxels x
-//     angles) are built in float64 by the workgroup and kept in LDS.
+//     loads (one row holds every frame: contiguous in HBM), then every lane
+//     interpolates its voxel's tap for its frames from LDS.  Windows longer
+//     than max_rows (40 rows, fewer when LDS is short) are staged in chunks
+//     that overlap by one row.
 //
 // Arithmetic per tap and frame:
 //   kCoefFloat  : v = a + w (b - a); acc += v * e^{i phi}        (8 FMA)
@@ -47,7 +45,9 @@ constexpr int kGroup = kThreads / kTile;        // channels per receive-term gro
 constexpr int kMaxAngles = 16;
 constexpr int kMaxRows = 40;  // rows of one staged window chunk (fewer when LDS is short)
 constexpr int kMinRows = 4;
-constexpr int kLdsBytes = 64 * 1024;
+constexpr int kMaxFramesPerLane = 26;
+// Dynamic LDS per workgroup: two workgroups share a CU's 160 KiB.
+constexpr int kLdsBytes = 80 * 1024;
 constexpr int kBig = 0x7fffffff;
 
 typedef _Float16 half2v __attribute__((ext_vector_type(2)));
@@ -186,25 +186,28 @@ __device__ __forceinline__ int wave_max(int x) {
 }
 
 struct TileLayout {  // byte offsets inside the dynamic LDS allocation
-  int rx, trig, bounds, win, bytes;
+  int trig, bounds, win, bytes;
 };
 
+// taps: Tap[na][kTile] at offset 0; trig: double[na][2]; bounds: int2[na];
+// win: W[max_rows][stride].
 __host__ __device__ inline TileLayout tile_layout(int na, int stride, int word_bytes, int max_rows) {
   TileLayout l;
-  l.rx = 2 * kTile * na * static_cast<int>(sizeof(Term));
-  l.trig = l.rx + kGroup * kTile * static_cast<int>(sizeof(Term));
+  l.trig = na * kTile * static_cast<int>(sizeof(Tap));
   l.bounds = l.trig + kMaxAngles * 2 * static_cast<int>(sizeof(double));
-  l.win = (l.bounds + 2 * (kGroup + 2 * kMaxAngles) * 4 + 15) & ~15;
+  l.win = (l.bounds + kMaxAngles * 2 * 4 + 15) & ~15;
   l.bytes = l.win + max_rows * stride * word_bytes;
   return l;
 }
 
-// Stage rows [r0, r0 + rows) x frames [f0, f0 + FB) of one (channel, angle)
-// plane into win; frames past nt are zero.  vec: 16-byte units (rows and f0
-// are 16-byte aligned in global memory and nt is a multiple of the unit).
+// Stage rows [r0, r0 + rows) x frames [0, nt) of one (channel, angle) plane
+// into win (frames nt .. FB - 1 of the window stay zero from the kernel's
+// initial fill).  vec: 16-byte units (the plane and every row start are
+// 16-byte aligned, nt is a multiple of the unit).
 template <typename W, int FB>
-__device__ __forceinline__ void stage(const W* __restrict__ plane, int nt, int r0, int rows, int f0, W* win,
-                                      int stride, bool vec) {
+__device__ __forceinline__ void stage(const W* __restrict__ plane, int nt, int r0, int rows, W* win, int stride,
+                                      bool vec) {
+  const W* __restrict__ src = plane + static_cast<size_t>(r0) * nt;
   if (vec) {
     constexpr int U = 16 / static_cast<int>(sizeof(W));
     constexpr int UPR = FB / U;  // units per row
@@ -212,19 +215,16 @@ __device__ __forceinline__ void stage(const W* __restrict__ plane, int nt, int r
     for (int q = threadIdx.x; q < n; q += kThreads) {
       const int row = q / UPR;
       const int col = (q - row * UPR) * U;
-      uint4 val = make_uint4(0u, 0u, 0u, 0u);
-      if (f0 + col < nt)
-        val = *reinterpret_cast<const uint4*>(plane + static_cast<size_t>(r0 + row) * nt + f0 + col);
-      *reinterpret_cast<uint4*>(win + row * stride + col) = val;
+      if (col < nt)
+        *reinterpret_cast<uint4*>(win + row * stride + col) =
+            *reinterpret_cast<const uint4*>(src + row * nt + col);
     }
   } else {
     const int n = rows * FB;
     for (int q = threadIdx.x; q < n; q += kThreads) {
       const int row = q / FB;
       const int col = q - row * FB;
-      W val{};
-      if (f0 + col < nt) val = plane[static_cast<size_t>(r0 + row) * nt + f0 + col];
-      win[row * stride + col] = val;
+      if (col < nt) win[row * stride + col] = src[row * nt + col];
     }
   }
 }
@@ -234,21 +234,17 @@ __device__ __forceinline__ void store_out(void* signal, long long index, float2 
   store_sample(reinterpret_cast<O*>(signal) + index, v);
 }
 
-template <typename T, int CP, int FPL, int NFB, bool APERTURES>
+template <typename T, int CP, int FPL, bool APERTURES>
 __global__ __launch_bounds__(kThreads) void das_opw_tile(EnsembleArgs p, long long tiles, int stride, int max_rows,
                                                          bool vec, int out_storage) {
   using W = typename Word<T>::type;
   constexpr int FB = kLanesPerVoxel * FPL;
   extern __shared__ __align__(16) unsigned char lds[];
   const TileLayout layout = tile_layout(p.na, stride, static_cast<int>(sizeof(W)), max_rows);
-  Term* tx_all = reinterpret_cast<Term*>(lds);               // [2][kTile][na]
-  Term* rx = reinterpret_cast<Term*>(lds + layout.rx);       // [kGroup][kTile]
-  double* trig = reinterpret_cast<double*>(lds + layout.trig);  // [na][cos, sin]
-  int* rx_lo = reinterpret_cast<int*>(lds + layout.bounds);  // [kGroup]
-  int* rx_hi = rx_lo + kGroup;
-  int* tx_lo = rx_hi + kGroup;  // [2][na]
-  int* tx_hi = tx_lo + 2 * kMaxAngles;
-  W* win = reinterpret_cast<W*>(lds + layout.win);  // [max_rows][stride]
+  Tap* taps = reinterpret_cast<Tap*>(lds);                        // [na][kTile]
+  double* trig = reinterpret_cast<double*>(lds + layout.trig);    // [na][cos, sin]
+  int2* bounds = reinterpret_cast<int2*>(lds + layout.bounds);    // [na]: first row, last row
+  W* win = reinterpret_cast<W*>(lds + layout.win);                // [max_rows][stride]
 
   const long long tile = xcd_tile(tiles);
   if (tile >= tiles) return;
@@ -260,7 +256,7 @@ __global__ __launch_bounds__(kThreads) void das_opw_tile(EnsembleArgs p, long lo
   const int fl = lane % kLanesPerVoxel;                          // frame lane
   const long long v = v0 + vl;
 
-  // Coordinates of voxel v0 + lane, for the receive terms this thread builds.
+  // Voxel v0 + lane: this thread builds its taps for angles wave, wave + 8.
   const long long vr = v0 + lane;
   const bool vr_ok = vr < p.nv;
   double xr = 0.0, zr = 0.0, yr = 0.0;
@@ -269,122 +265,82 @@ __global__ __launch_bounds__(kThreads) void das_opw_tile(EnsembleArgs p, long lo
     zr = p.scan[3 * vr + 1];
     yr = p.scan[3 * vr + 2];
   }
-
-  // Steering cosines and sines, then the transmit terms of both apertures and
-  // their row bounds, before the accumulators are live.
+  // Steering cosines and sines once per workgroup (the float64 sincos is
+  // register-hungry, so it runs before the accumulators are live); the
+  // window is zeroed once, so frames nt .. FB - 1 read as zero.
   if (tid < p.na) {
     double sn, cs;
     sincos(static_cast<double>(p.angles[tid]), &sn, &cs);
     trig[2 * tid] = cs;
     trig[2 * tid + 1] = sn;
   }
-  __syncthreads();
-  for (int idx = tid; idx < 2 * kTile * p.na; idx += kThreads) {
-    const int cfg = idx / (kTile * p.na);
-    const int rem = idx - cfg * kTile * p.na;
-    const int vox = rem / p.na;
-    const int a = rem - vox * p.na;
-    const long long vg = v0 + vox;
-    Term t{0, 0.f, 1.f, 0.f};
-    if (vg < p.nv) {
-      const double x = p.scan[3 * vg], z = p.scan[3 * vg + 1], y = p.scan[3 * vg + 2];
-      t = transmit_term(cfg == 0 ? y : x, z, trig[2 * a], trig[2 * a + 1], static_cast<double>(p.starts[a]), p.c,
-                        p.fs, p.fd);
-    }
-    tx_all[idx] = t;
-  }
-  __syncthreads();
-  for (int pair = wave; pair < 2 * p.na; pair += kWaves) {
-    const int cfg = pair / p.na;
-    const int a = pair - cfg * p.na;
-    int lo = kBig, hi = -kBig;
-    if (vr_ok) lo = hi = tx_all[(cfg * kTile + lane) * p.na + a].i;
-    lo = wave_min(lo);
-    hi = wave_max(hi);
-    if (lane == 0) {
-      tx_lo[pair] = lo;
-      tx_hi[pair] = hi;
-    }
+  {
+    uint4* w4 = reinterpret_cast<uint4*>(win);
+    const int n = max_rows * stride * static_cast<int>(sizeof(W)) / 16;
+    for (int i = tid; i < n; i += kThreads) w4[i] = make_uint4(0u, 0u, 0u, 0u);
   }
 
-  float2 acc[NFB][FPL];
+  float2 acc[FPL];
 #pragma unroll
-  for (int b = 0; b < NFB; ++b)
-#pragma unroll
-    for (int j = 0; j < FPL; ++j) acc[b][j] = make_float2(0.f, 0.f);
+  for (int j = 0; j < FPL; ++j) acc[j] = make_float2(0.f, 0.f);
 
   for (int cfg = 0; cfg < 2; ++cfg) {
     const W* __restrict__ data = static_cast<const W*>(cfg == 0 ? p.rc : p.cr);
     const float* el = cfg == 0 ? p.xe : p.ye;
-    const Term* tx = tx_all + cfg * kTile * p.na;
-    const int* txl = tx_lo + cfg * p.na;
-    const int* txh = tx_hi + cfg * p.na;
-    for (int g0 = 0; g0 < p.nc; g0 += kGroup) {
-      __syncthreads();
+    for (int ch = 0; ch < p.nc; ++ch) {
+      __syncthreads();  // the previous channel's angles are done with taps, bounds and window
       {
-        const int ch = g0 + wave;
+        // Tap table of this channel and the exact row window of every angle.
         Term r{kInactive, 0.f, 0.f, 0.f};
-        if (ch < p.nc && vr_ok)
-          r = receive_term(cfg == 0 ? xr : yr, zr, static_cast<double>(el[ch]), p.c, p.fs, p.fd, p.fn);
-        rx[wave * kTile + lane] = r;
-        const bool act = r.i != kInactive;
-        const int lo = wave_min(act ? r.i : kBig);
-        const int hi = wave_max(act ? r.i : -kBig);
-        if (lane == 0) {
-          rx_lo[wave] = lo;
-          rx_hi[wave] = hi;
+        if (vr_ok) r = receive_term(cfg == 0 ? xr : yr, zr, static_cast<double>(el[ch]), p.c, p.fs, p.fd, p.fn);
+        for (int a = wave; a < p.na; a += kWaves) {
+          Tap tp{-1, 0.f, 0.f, 0.f};
+          if (r.i != kInactive) {
+            const Term t = transmit_term(cfg == 0 ? yr : xr, zr, trig[2 * a], trig[2 * a + 1],
+                                         static_cast<double>(p.starts[a]), p.c, p.fs, p.fd);
+            tp = combine(r, t, p.ns);
+          }
+          taps[a * kTile + lane] = tp;
+          const bool live = tp.i0 >= 0;
+          const int lo = wave_min(live ? tp.i0 : kBig);
+          const int hi = wave_max(live ? tp.i0 + (tp.w != 0.f ? 1 : 0) : -kBig);
+          if (lane == 0) bounds[a] = make_int2(lo, hi);
         }
       }
       __syncthreads();
-      const int group = min(kGroup, p.nc - g0);
-      for (int m = 0; m < group; ++m) {
-        const int rlo = uniform(rx_lo[m]);
-        const int rhi = uniform(rx_hi[m]);
-        if (rlo > rhi) continue;  // no voxel of the tile inside this channel's aperture
-        const int ch = g0 + m;
-        const Term rr = rx[m * kTile + vl];
-        for (int a = 0; a < p.na; ++a) {
-          const int lo = max(rlo + uniform(txl[a]), 0);
-          const int hi = min(rhi + uniform(txh[a]) + 2, p.ns - 1);
-          if (lo > hi) continue;  // every tap of the tile is outside the sample window
-          const Tap tp = combine(rr, tx[vl * p.na + a], p.ns);
-          const bool live = tp.i0 >= 0;
-          const int i1 = tp.w != 0.f ? tp.i0 + 1 : tp.i0;
-          const Coef k = make_coef<T, CP>(tp);
-          const W* plane = data + static_cast<size_t>(ch * p.na + a) * p.ns * p.nt;
-          for (int r0 = lo;; r0 += max_rows - 1) {
-            const bool last = r0 + max_rows - 1 >= hi;
-            const int rows = last ? hi + 1 - r0 : max_rows;
-            const bool in_chunk = live && tp.i0 >= r0 && (last || tp.i0 < r0 + max_rows - 1);
-            if (__syncthreads_or(in_chunk)) {
-              const W* pa = win + (tp.i0 - r0) * stride + fl;
-              const W* pb = win + (i1 - r0) * stride + fl;
-#pragma unroll
-              for (int b = 0; b < NFB; ++b) {
-                const int f0 = b * FB;
-                if (f0 < p.nt) {
-                  if (b > 0) __syncthreads();
-                  stage<W, FB>(plane, p.nt, r0, rows, f0, win, stride, vec);
-                  __syncthreads();
-                  if (in_chunk) accumulate<T, CP, FPL>(pa, pb, k, acc[b]);
-                }
-              }
-            }
-            if (last) break;
-          }
+      bool staged = false;  // a stage since the last barrier: the next one must wait for its readers
+      for (int a = 0; a < p.na; ++a) {
+        const int2 b = bounds[a];
+        const int lo = uniform(b.x);
+        const int hi = uniform(b.y);
+        if (lo > hi) continue;  // no tap of the tile for this (channel, angle)
+        const Tap tp = taps[a * kTile + vl];
+        const bool live = tp.i0 >= 0;
+        const int i1 = tp.w != 0.f ? tp.i0 + 1 : tp.i0;
+        const Coef k = make_coef<T, CP>(tp);
+        const W* plane = data + static_cast<size_t>(ch * p.na + a) * p.ns * p.nt;
+        for (int r0 = lo;; r0 += max_rows - 1) {
+          const bool last = r0 + max_rows - 1 >= hi;
+          const int rows = last ? hi + 1 - r0 : max_rows;
+          const bool in_chunk = live && tp.i0 >= r0 && (last || tp.i0 < r0 + max_rows - 1);
+          if (staged) __syncthreads();
+          stage<W, FB>(plane, p.nt, r0, rows, win, stride, vec);
+          __syncthreads();
+          staged = true;
+          if (in_chunk)
+            accumulate<T, CP, FPL>(win + (tp.i0 - r0) * stride + fl, win + (i1 - r0) * stride + fl, k, acc);
+          if (last) break;
         }
       }
     }
     if constexpr (APERTURES) {
       if (v < p.nv) {
 #pragma unroll
-        for (int b = 0; b < NFB; ++b)
-#pragma unroll
-          for (int j = 0; j < FPL; ++j) {
-            const int t = b * FB + fl + kLanesPerVoxel * j;
-            if (t < p.nt) p.signal[(2 * v + cfg) * p.nt + t] = acc[b][j];
-            acc[b][j] = make_float2(0.f, 0.f);
-          }
+        for (int j = 0; j < FPL; ++j) {
+          const int t = fl + kLanesPerVoxel * j;
+          if (t < p.nt) p.signal[(2 * v + cfg) * p.nt + t] = acc[j];
+          acc[j] = make_float2(0.f, 0.f);
+        }
         if (fl == 0) {
           p.pd[2 * v] = 0.f;
           p.pd[2 * v + 1] = 0.f;
@@ -395,24 +351,22 @@ __global__ __launch_bounds__(kThreads) void das_opw_tile(EnsembleArgs p, long lo
   if constexpr (!APERTURES) {
     float power = 0.f;
 #pragma unroll
-    for (int b = 0; b < NFB; ++b)
-#pragma unroll
-      for (int j = 0; j < FPL; ++j) {
-        const int t = b * FB + fl + kLanesPerVoxel * j;
-        if (t < p.nt && v < p.nv) {
-          const float2 s = acc[b][j];
-          power = fmaf(s.x, s.x, fmaf(s.y, s.y, power));
-          if (p.signal) {
-            const long long index = v * p.nt + t;
-            if (out_storage == RCABEAM_STORAGE_FP16)
-              store_out<__half2>(p.signal, index, s);
-            else if (out_storage == RCABEAM_STORAGE_BF16)
-              store_out<bf16x2>(p.signal, index, s);
-            else
-              store_out<float2>(p.signal, index, s);
-          }
+    for (int j = 0; j < FPL; ++j) {
+      const int t = fl + kLanesPerVoxel * j;
+      if (t < p.nt && v < p.nv) {
+        const float2 s = acc[j];
+        power = fmaf(s.x, s.x, fmaf(s.y, s.y, power));
+        if (p.signal) {
+          const long long index = v * p.nt + t;
+          if (out_storage == RCABEAM_STORAGE_FP16)
+            store_out<__half2>(p.signal, index, s);
+          else if (out_storage == RCABEAM_STORAGE_BF16)
+            store_out<bf16x2>(p.signal, index, s);
+          else
+            store_out<float2>(p.signal, index, s);
         }
       }
+    }
     // The 8 lanes of a voxel are adjacent: butterfly inside the group.
 #pragma unroll
     for (int off = 1; off < kLanesPerVoxel; off <<= 1) power += __shfl_xor(power, off, kWave);
@@ -432,7 +386,7 @@ int window_stride(int fb, int word_bytes) {
   return dw * 4 / word_bytes;
 }
 
-template <typename T, int CP, int FPL, int NFB, bool APERTURES>
+template <typename T, int CP, int FPL, bool APERTURES>
 hipError_t launch_shape(const EnsembleArgs& p, int out_storage) {
   using W = typename Word<T>::type;
   constexpr int FB = kLanesPerVoxel * FPL;
@@ -447,24 +401,36 @@ hipError_t launch_shape(const EnsembleArgs& p, int out_storage) {
   constexpr int U = 16 / static_cast<int>(sizeof(W));
   const bool vec = reinterpret_cast<uintptr_t>(p.rc) % 16 == 0 && reinterpret_cast<uintptr_t>(p.cr) % 16 == 0 &&
                    p.nt % U == 0;
-  das_opw_tile<T, CP, FPL, NFB, APERTURES>
-      <<<dim3(static_cast<unsigned>(grid)), kThreads, layout.bytes>>>(p, tiles, stride, max_rows, vec, out_storage);
+  auto kernel = das_opw_tile<T, CP, FPL, APERTURES>;
+  static bool attribute_set = false;  // one opt-in per instantiation (beyond 64 KiB of dynamic LDS)
+  if (!attribute_set) {
+    const hipError_t e =
+        hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                            kLdsBytes);
+    if (e != hipSuccess) return e;
+    attribute_set = true;
+  }
+  kernel<<<dim3(static_cast<unsigned>(grid)), kThreads, layout.bytes>>>(p, tiles, stride, max_rows, vec,
+                                                                        out_storage);
   return hipGetLastError();
 }
 
 template <typename T, int CP, bool APERTURES>
 hipError_t launch_frames(const EnsembleArgs& p, int out_storage) {
-  if (p.nt <= 16) return launch_shape<T, CP, 2, 1, APERTURES>(p, out_storage);
-  if (p.nt <= 32) return launch_shape<T, CP, 4, 1, APERTURES>(p, out_storage);
-  if (p.nt <= 64) return launch_shape<T, CP, 8, 1, APERTURES>(p, out_storage);
-  return launch_shape<T, CP, 13, 2, APERTURES>(p, out_storage);
+  if (p.nt <= 16) return launch_shape<T, CP, 2, APERTURES>(p, out_storage);
+  if (p.nt <= 32) return launch_shape<T, CP, 4, APERTURES>(p, out_storage);
+  if (p.nt <= 64) return launch_shape<T, CP, 8, APERTURES>(p, out_storage);
+  if (p.nt <= 104) return launch_shape<T, CP, 13, APERTURES>(p, out_storage);
+  return launch_shape<T, CP, 26, APERTURES>(p, out_storage);
 }
 
 }  // namespace
 
 hipError_t launch_opw_tile(const EnsembleArgs& p, int channel_storage, int coefficients, int signal_storage,
                            bool apertures) {
-  if (p.na > kMaxAngles || p.nt > 2 * kLanesPerVoxel * 13) return hipErrorNotSupported;
+  if (p.na > kMaxAngles || p.nt > kLanesPerVoxel * kMaxFramesPerLane) return hipErrorNotSupported;
+  // Plane offsets and in-plane row offsets are int.
+  if (static_cast<long long>(p.ns) * p.nt > 0x7fffffffLL) return hipErrorNotSupported;
   if (signal_storage < RCABEAM_STORAGE_C64 || signal_storage > RCABEAM_STORAGE_BF16) return hipErrorInvalidValue;
   if (apertures) {
     if (channel_storage != RCABEAM_STORAGE_C64) return hipErrorInvalidValue;
