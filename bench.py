@@ -127,17 +127,16 @@ class Pipeline:
             else:
                 pack = "rcabeam_pack_ensemble_fp16_device" if self.storage == "float16" else "rcabeam_pack_ensemble_bf16_device"
                 self._call(pack, dev.ptr(raw), dev.ptr(packed), dev.ptr(self.range_error), *shape)
-        common = (dev.ptr(self.packed[0]), dev.ptr(self.packed[1]), dev.ptr(self.g.scan), dev.ptr(self.g.xe),
-                  dev.ptr(self.g.ye), dev.ptr(self.g.angles), dev.ptr(self.g.starts), dev.ptr(self.pd))
         if das_events is not None:
             das_events[0].record()
-        if self.storage == "float32":
-            self._call("rcabeam_ensemble_device", *common, dev.ptr(self.signal), 0, 0, dev.ptr(self.g.geometry_workspace),
-                       *shape, nv, c, fs, fd, fn, 0, 2)
-        else:
-            # 16-bit channels and IQ, one 16-bit coefficient per sample (dot2, float32 accumulation).
-            self._call("rcabeam_opw_mixed_device", *common, dev.ptr(self.signal), self.code, self.code,
-                       _lib_coef_16bit(), *shape, nv, c, fs, fd, fn)
+        # OPW on the Cartesian grid (8 x 8 lateral tiles).  float32: complex64 channels and IQ;
+        # 16-bit: 16-bit channels and IQ, one 16-bit coefficient per sample (dot2, float32 accumulation).
+        nx, nz, ny = self.g.shape
+        coef = 0 if self.storage == "float32" else _lib_coef_16bit()
+        self._call("rcabeam_opw_grid_device", dev.ptr(self.packed[0]), dev.ptr(self.packed[1]), self.code, coef,
+                   dev.ptr(self.g.axes[0]), dev.ptr(self.g.axes[1]), dev.ptr(self.g.axes[2]), nx, nz, ny,
+                   dev.ptr(self.g.xe), dev.ptr(self.g.ye), dev.ptr(self.g.angles), dev.ptr(self.g.starts),
+                   dev.ptr(self.pd), dev.ptr(self.signal), self.code, 0, *shape, c, fs, fd, fn)
         if das_events is not None:
             das_events[1].record()
         self._call("rcabeam_iq_absmax_device", dev.ptr(self.signal), self.code, nv * self.nt, dev.ptr(self.stats), 0)

@@ -58,19 +58,21 @@ rcabeam_status rcabeam_opw_mixed_device(const void* rc, const void* cr, const fl
                                         size_t ns, size_t nc, size_t na, size_t nt, size_t nv, float c, float fs,
                                         float fd, float fn);
 
-/* OPW on a Cartesian grid from 16-bit packed channels (channel_storage FP16
- * or BF16), delay-and-sum on MFMA: 4 x 4 lateral voxel tiles, per
- * (aperture, channel, angle) an interpolation/phasor matrix (hi + lo 16-bit
- * coefficients) times the sample window, float32 accumulation.  Axes x (nx),
- * z (nz), y (ny); voxel id (ix * nz + iz) * ny + iy as _scan_coords orders
- * them.  pd: 2 * nx*nz*ny floats (power in the even entries); signal
- * (optional): (nv, nt) samples of signal_storage.  Other arguments as
- * rcabeam_ensemble_device. */
-rcabeam_status rcabeam_opw_grid_device(const void* rc, const void* cr, int channel_storage, const float* x,
-                                       const float* z, const float* y, size_t nx, size_t nz, size_t ny,
-                                       const float* xe, const float* ye, const float* angles, const float* starts,
-                                       float* pd, void* signal, int signal_storage, size_t ns, size_t nc, size_t na,
-                                       size_t nt, float c, float fs, float fd, float fn);
+/* OPW (method 0) or separate-aperture IQ (method 5, complex64 channels, signal
+ * (nv, 2, nt) complex64 RC then CR) on a Cartesian grid: axes x (nx), z (nz),
+ * y (ny); voxel id (ix * nz + iz) * ny + iy as _scan_coords orders them, so
+ * the outputs are those of rcabeam_opw_mixed_device / rcabeam_ensemble_device
+ * on the scan list of the grid.  Tiles are 8 x 8 lateral patches at one depth
+ * (short sample windows; the fastest form).  pd: 2 * nx*nz*ny floats; signal
+ * (optional for method 0): (nv, nt) samples of signal_storage.  Requires
+ * na <= 16 and nt <= 208 (RCABEAM_ERROR_ARGUMENT otherwise).  Other arguments
+ * as rcabeam_opw_mixed_device. */
+rcabeam_status rcabeam_opw_grid_device(const void* rc, const void* cr, int channel_storage, int coefficients,
+                                       const float* x, const float* z, const float* y, size_t nx, size_t nz,
+                                       size_t ny, const float* xe, const float* ye, const float* angles,
+                                       const float* starts, float* pd, void* signal, int signal_storage, int method,
+                                       size_t ns, size_t nc, size_t na, size_t nt, float c, float fs, float fd,
+                                       float fn);
 
 /* Adds to *out (a device counter the caller zeroes) the number of
  * (voxel, aperture, channel, angle) taps inside the dynamic aperture and the

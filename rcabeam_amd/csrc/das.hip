@@ -712,3 +712,31 @@ extern "C" rcabeam_status rcabeam_opw_mixed_device(const void* rc, const void* c
   }
   return to_status(e);
 }
+
+extern "C" rcabeam_status rcabeam_opw_grid_device(const void* rc, const void* cr, int channel_storage, int coefficients,
+                                                  const float* x, const float* z, const float* y, size_t nx,
+                                                  size_t nz, size_t ny, const float* xe, const float* ye,
+                                                  const float* angles, const float* starts, float* pd, void* signal,
+                                                  int signal_storage, int method, size_t ns, size_t nc, size_t na,
+                                                  size_t nt, float c, float fs, float fd, float fn) {
+  if (!x || !z || !y || !nx || !nz || !ny || nx > 0x7fffffff || nz > 0x7fffffff || ny > 0x7fffffff)
+    return RCABEAM_ERROR_ARGUMENT;
+  if (reinterpret_cast<uintptr_t>(rc) % 4 || reinterpret_cast<uintptr_t>(cr) % 4) return RCABEAM_ERROR_ARGUMENT;
+  const size_t nv = nx * nz * ny;
+  // The scan list is not read on a grid; any non-null pointer passes validation.
+  if (!valid_ensemble(rc, cr, x, xe, ye, angles, starts, pd, ns, nc, na, nt, nv, c, fs, fd, fn))
+    return RCABEAM_ERROR_ARGUMENT;
+  if ((method != kOpw && method != kApertures) || (method == kApertures && (!signal || channel_storage != 0)) ||
+      (coefficients != RCABEAM_COEF_FLOAT32 && coefficients != RCABEAM_COEF_16BIT) || channel_storage < 0 ||
+      channel_storage > 2 || signal_storage < 0 || signal_storage > 2)
+    return RCABEAM_ERROR_ARGUMENT;
+  EnsembleArgs p = make_args(rc, cr, nullptr, xe, ye, angles, starts, pd, signal, nullptr, nullptr, ns, nc, na, nt, nv,
+                             c, fs, fd, fn);
+  const int coef = coefficients == RCABEAM_COEF_16BIT && channel_storage != RCABEAM_STORAGE_C64
+                       ? kCoefSingle
+                       : accurate_coefficients(channel_storage);
+  const hipError_t e = launch_opw_grid(p, x, z, y, static_cast<int>(nx), static_cast<int>(nz), static_cast<int>(ny),
+                                       channel_storage, coef, signal_storage, method == kApertures);
+  if (e == hipErrorNotSupported) return RCABEAM_ERROR_ARGUMENT;
+  return to_status(e);
+}

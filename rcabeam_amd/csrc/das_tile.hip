@@ -1,26 +1,29 @@
 // LDS-tiled, frame-batched OPW delay-and-sum for MI355X (gfx950): the main
 // path of rcabeam_ensemble_device methods 0 and 5, rcabeam_opw_fp16_device,
-// rcabeam_opw_bf16_device and rcabeam_opw_mixed_device (reference
-// src/rcabeam/ensemble.cu:341-454 computes the same sums with per-tap global
-// gathers).
+// rcabeam_opw_bf16_device, rcabeam_opw_mixed_device and
+// rcabeam_opw_grid_device (reference src/rcabeam/ensemble.cu:341-454 computes
+// the same sums with per-tap global gathers).
 //
 // Work decomposition (one 512-thread workgroup = 8 wave64s, two per CU):
-//   * a workgroup owns a tile of 64 consecutive voxels (y fastest: on a
-//     Cartesian grid, half a y line at fixed x and z) and every frame, so the
-//     power reduction is complete inside the workgroup (deterministic pd);
-//   * wave w owns voxels 8w .. 8w+7; the 8 lanes of a voxel own frames
-//     lane%8 + 8j, j < FPL (all frames at once, up to 208);
+//   * a workgroup owns a tile of 64 voxels and every frame, so the power
+//     reduction is complete inside the workgroup (deterministic pd).  On a
+//     Cartesian grid a tile is an 8 (x) by 8 (y) patch at one depth, tiles
+//     ordered depth-fastest; for a voxel list it is 64 consecutive voxels;
+//   * wave w owns voxels 8w .. 8w+7 (one x row of a grid tile); the 8 lanes
+//     of a voxel own frames lane%8 + 8j, j < FPL (all frames, up to 208);
 //   * loop order aperture -> channel -> angle.  Per channel the workgroup
 //     builds the tap table of its 64 voxels x angles once (receive and
 //     transmit terms in float64, combined into row, weight and phasor) and,
 //     per angle, the exact row window [min first row, max second row] by
-//     wave reductions;
-//   * per (channel, angle) the window rows x all frames are staged from the
-//     packed (channels, angles, samples, frames) layout into LDS with 16-byte
-//     loads (one row holds every frame: contiguous in HBM), then every lane
-//     interpolates its voxel's tap for its frames from LDS.  Windows longer
-//     than max_rows (40 rows, fewer when LDS is short) are staged in chunks
-//     that overlap by one row.
+//     wave reductions.  Receive rows move by < 1.2 samples per voxel in x
+//     and transmit rows by < 0.5 in y, so a patch's window is a few rows;
+//   * the windows of consecutive angles are packed into stages of at most
+//     max_rows rows (longer windows are split into chunks overlapping by one
+//     row).  A stage streams rows x frames from the packed (channels, angles,
+//     samples, frames) layout straight into LDS (global_load_lds_dwordx4, no
+//     register staging), double-buffered: the next stage is in flight while
+//     every lane interpolates its voxel's taps for its frames from the
+//     current one; one barrier per stage.
 //
 // Arithmetic per tap and frame:
 //   kCoefFloat  : v = a + w (b - a); acc += v * e^{i phi}        (8 FMA)
@@ -41,9 +44,9 @@ constexpr int kWaves = kThreads / kWave;
 constexpr int kLanesPerVoxel = 8;
 constexpr int kVoxelsPerWave = kWave / kLanesPerVoxel;
 constexpr int kTile = kWaves * kVoxelsPerWave;  // 64 voxels
-constexpr int kGroup = kThreads / kTile;        // channels per receive-term group
+constexpr int kPatch = 8;                       // grid tiles: kPatch x kPatch lateral voxels
 constexpr int kMaxAngles = 16;
-constexpr int kMaxRows = 40;  // rows of one staged window chunk (fewer when LDS is short)
+constexpr int kMaxRows = 40;  // rows of one stage (fewer when LDS is short)
 constexpr int kMinRows = 4;
 constexpr int kMaxFramesPerLane = 26;
 // Dynamic LDS per workgroup: two workgroups share a CU's 160 KiB.
@@ -185,89 +188,192 @@ __device__ __forceinline__ int wave_max(int x) {
   return x;
 }
 
+// Row stride of an LDS window (in samples): at least fb and such that the 8
+// rows a wave reads in one instruction fall on distinct LDS bank groups.
+constexpr int window_stride(int fb, int word_bytes) {
+  const int seg = kLanesPerVoxel * word_bytes / 4;  // dwords one voxel's lanes read
+  const int dw0 = (fb * word_bytes / 4 + seg - 1) / seg * seg;
+  const int dw = (dw0 / seg) % 2 == 0 ? dw0 + seg : dw0;
+  return dw * 4 / word_bytes;
+}
+
 struct TileLayout {  // byte offsets inside the dynamic LDS allocation
   int trig, bounds, win, bytes;
 };
 
 // taps: Tap[na][kTile] at offset 0; trig: double[na][2]; bounds: int2[na];
-// win: W[max_rows][stride].
+// win: two buffers W[max_rows][stride].
 __host__ __device__ inline TileLayout tile_layout(int na, int stride, int word_bytes, int max_rows) {
   TileLayout l;
   l.trig = na * kTile * static_cast<int>(sizeof(Tap));
   l.bounds = l.trig + kMaxAngles * 2 * static_cast<int>(sizeof(double));
   l.win = (l.bounds + kMaxAngles * 2 * 4 + 15) & ~15;
-  l.bytes = l.win + max_rows * stride * word_bytes;
+  l.bytes = l.win + 2 * max_rows * stride * word_bytes;
   return l;
 }
 
-// Stage rows [r0, r0 + rows) x frames [0, nt) of one (channel, angle) plane
-// into win (frames nt .. FB - 1 of the window stay zero from the kernel's
-// initial fill).  vec: 16-byte units (the plane and every row start are
-// 16-byte aligned, nt is a multiple of the unit).
-template <typename W, int FB>
-__device__ __forceinline__ void stage(const W* __restrict__ plane, int nt, int r0, int rows, W* win, int stride,
-                                      bool vec) {
-  const W* __restrict__ src = plane + static_cast<size_t>(r0) * nt;
+// Which voxels a tile holds.  grid: 8 x 8 lateral patches, one depth per tile,
+// tile = (bx * nby + by) * nz + iz; voxel id (ix * nz + iz) * ny + iy.
+// Otherwise voxels [64 t, 64 t + 64) of the scan list.
+struct TileMap {
+  const float* x;
+  const float* z;
+  const float* y;
+  int nx, nz, ny, nby;
+  bool grid;
+};
+
+// Voxel id of tile slot s (-1 outside the volume) and its float32 coordinates.
+__device__ __forceinline__ long long slot_voxel(const EnsembleArgs& p, const TileMap& m, long long tile, int s,
+                                                float& x, float& z, float& y) {
+  x = z = y = 0.f;
+  if (m.grid) {
+    const int iz = static_cast<int>(tile % m.nz);
+    const long long lateral = tile / m.nz;
+    const int ix = static_cast<int>(lateral / m.nby) * kPatch + s / kPatch;
+    const int iy = static_cast<int>(lateral % m.nby) * kPatch + s % kPatch;
+    if (ix >= m.nx || iy >= m.ny) return -1;
+    x = m.x[ix];
+    z = m.z[iz];
+    y = m.y[iy];
+    return (static_cast<long long>(ix) * m.nz + iz) * m.ny + iy;
+  }
+  const long long v = tile * kTile + s;
+  if (v >= p.nv) return -1;
+  x = p.scan[3 * v];
+  z = p.scan[3 * v + 1];
+  y = p.scan[3 * v + 2];
+  return v;
+}
+
+// One stage: angles [a0, a1) whose windows are packed back to back, or one
+// chunk (rows [r0, r0 + rows)) of a window longer than max_rows (r0 >= 0).
+struct Stage {
+  int a0, a1, r0, rows;
+  bool last, valid;
+};
+
+__device__ __forceinline__ int2 window(const int2* bounds, int a) {
+  const int2 b = bounds[a];
+  return make_int2(uniform(b.x), uniform(b.y));
+}
+
+// The stage that starts at the first live angle at or after a.
+__device__ __forceinline__ Stage plan_stage(const int2* bounds, int a, int na, int max_rows) {
+  for (; a < na; ++a) {
+    const int2 w = window(bounds, a);
+    if (w.x > w.y) continue;
+    const int rows = w.y - w.x + 1;
+    if (rows > max_rows) return Stage{a, a + 1, w.x, max_rows, false, true};
+    int total = rows, a1 = a + 1;
+    for (; a1 < na; ++a1) {
+      const int2 w1 = window(bounds, a1);
+      if (w1.x > w1.y) continue;
+      const int r1 = w1.y - w1.x + 1;
+      if (total + r1 > max_rows) break;
+      total += r1;
+    }
+    return Stage{a, a1, -1, total, true, true};
+  }
+  return Stage{0, 0, -1, 0, true, false};
+}
+
+__device__ __forceinline__ Stage next_stage(const Stage& s, const int2* bounds, int na, int max_rows) {
+  if (s.r0 >= 0 && !s.last) {
+    const int hi = window(bounds, s.a0).y;
+    const int r0 = s.r0 + max_rows - 1;
+    const bool last = r0 + max_rows - 1 >= hi;
+    return Stage{s.a0, s.a1, r0, last ? hi + 1 - r0 : max_rows, last, true};
+  }
+  return plan_stage(bounds, s.a1, na, max_rows);
+}
+
+// Copy rows [r_first, r_first + rows) x frames [0, nt) of one plane into LDS
+// rows starting at dst.  vec: each lane moves one 16-byte unit straight from
+// HBM into LDS (global_load_lds_dwordx4, asynchronous: the caller waits on
+// vmcnt before the barrier); the lanes of a wave write 64 consecutive units,
+// as the instruction requires.  Otherwise a synchronous element copy.
+template <typename W, int STRIDE>
+__device__ __forceinline__ void copy_rows(const W* __restrict__ plane, int nt, int r_first, int rows, W* dst,
+                                          bool vec) {
+  const W* __restrict__ src = plane + static_cast<size_t>(r_first) * nt;
   if (vec) {
     constexpr int U = 16 / static_cast<int>(sizeof(W));
-    constexpr int UPR = FB / U;  // units per row
+    constexpr int UPR = STRIDE / U;  // LDS units per window row
     const int n = rows * UPR;
-    for (int q = threadIdx.x; q < n; q += kThreads) {
-      const int row = q / UPR;
-      const int col = (q - row * UPR) * U;
-      if (col < nt)
-        *reinterpret_cast<uint4*>(win + row * stride + col) =
-            *reinterpret_cast<const uint4*>(src + row * nt + col);
+    for (int base = (threadIdx.x / kWave) * kWave; base < n; base += kThreads) {
+      const int u = base + static_cast<int>(threadIdx.x % kWave);
+      const int row = u / UPR;
+      const int col = (u - row * UPR) * U;
+      if (u < n && col < nt)
+        __builtin_amdgcn_global_load_lds(
+            reinterpret_cast<const void*>(src + row * nt + col),
+            reinterpret_cast<__attribute__((address_space(3))) void*>(reinterpret_cast<uintptr_t>(dst + base * U)),
+            16, 0, 0);
     }
   } else {
-    const int n = rows * FB;
+    const int n = rows * STRIDE;
     for (int q = threadIdx.x; q < n; q += kThreads) {
-      const int row = q / FB;
-      const int col = q - row * FB;
-      if (col < nt) win[row * stride + col] = src[row * nt + col];
+      const int row = q / STRIDE;
+      const int col = q - row * STRIDE;
+      if (col < nt) dst[q] = src[row * nt + col];
     }
   }
 }
+
+template <typename W, int STRIDE>
+__device__ __forceinline__ void issue_stage(const W* plane0, size_t plane_size, int nt, const Stage& s,
+                                            const int2* bounds, W* buf, bool vec) {
+  if (s.r0 >= 0) {
+    copy_rows<W, STRIDE>(plane0 + s.a0 * plane_size, nt, s.r0, s.rows, buf, vec);
+    return;
+  }
+  int off = 0;
+  for (int a = s.a0; a < s.a1; ++a) {
+    const int2 w = window(bounds, a);
+    if (w.x > w.y) continue;
+    copy_rows<W, STRIDE>(plane0 + a * plane_size, nt, w.x, w.y - w.x + 1, buf + off * STRIDE, vec);
+    off += w.y - w.x + 1;
+  }
+}
+
+// s_waitcnt vmcnt(0) (expcnt and lgkmcnt unconstrained) in the gfx9 encoding.
+constexpr int kWaitVmcnt0 = 0x0F70;
 
 template <typename O>
 __device__ __forceinline__ void store_out(void* signal, long long index, float2 v) {
   store_sample(reinterpret_cast<O*>(signal) + index, v);
 }
 
+// Two workgroups (16 waves) per CU: at most 128 VGPRs.
 template <typename T, int CP, int FPL, bool APERTURES>
-__global__ __launch_bounds__(kThreads) void das_opw_tile(EnsembleArgs p, long long tiles, int stride, int max_rows,
-                                                         bool vec, int out_storage) {
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void das_opw_tile(
+    EnsembleArgs p, TileMap map, long long tiles, int max_rows, bool vec, int out_storage) {
   using W = typename Word<T>::type;
   constexpr int FB = kLanesPerVoxel * FPL;
+  constexpr int STRIDE = window_stride(FB, static_cast<int>(sizeof(W)));
   extern __shared__ __align__(16) unsigned char lds[];
-  const TileLayout layout = tile_layout(p.na, stride, static_cast<int>(sizeof(W)), max_rows);
-  Tap* taps = reinterpret_cast<Tap*>(lds);                        // [na][kTile]
-  double* trig = reinterpret_cast<double*>(lds + layout.trig);    // [na][cos, sin]
-  int2* bounds = reinterpret_cast<int2*>(lds + layout.bounds);    // [na]: first row, last row
-  W* win = reinterpret_cast<W*>(lds + layout.win);                // [max_rows][stride]
+  const TileLayout layout = tile_layout(p.na, STRIDE, static_cast<int>(sizeof(W)), max_rows);
+  Tap* taps = reinterpret_cast<Tap*>(lds);                      // [na][kTile]
+  double* trig = reinterpret_cast<double*>(lds + layout.trig);  // [na][cos, sin]
+  int2* bounds = reinterpret_cast<int2*>(lds + layout.bounds);  // [na]: first row, last row
+  W* win = reinterpret_cast<W*>(lds + layout.win);              // [2][max_rows][STRIDE]
+  const int buffer = max_rows * STRIDE;
 
   const long long tile = xcd_tile(tiles);
   if (tile >= tiles) return;
-  const long long v0 = tile * kTile;
   const int tid = threadIdx.x;
   const int lane = tid & (kWave - 1);
   const int wave = tid / kWave;
-  const int vl = wave * kVoxelsPerWave + lane / kLanesPerVoxel;  // voxel of this lane's taps
+  const int vl = wave * kVoxelsPerWave + lane / kLanesPerVoxel;  // tile slot of this lane's taps
   const int fl = lane % kLanesPerVoxel;                          // frame lane
-  const long long v = v0 + vl;
 
-  // Voxel v0 + lane: this thread builds its taps for angles wave, wave + 8.
-  const long long vr = v0 + lane;
-  const bool vr_ok = vr < p.nv;
-  double xr = 0.0, zr = 0.0, yr = 0.0;
-  if (vr_ok) {
-    xr = p.scan[3 * vr];
-    zr = p.scan[3 * vr + 1];
-    yr = p.scan[3 * vr + 2];
-  }
+  // Tile slot `lane`: this thread builds its taps for angles wave, wave + 8.
+  float xr, zr, yr;
+  const bool vr_ok = slot_voxel(p, map, tile, lane, xr, zr, yr) >= 0;
   // Steering cosines and sines once per workgroup (the float64 sincos is
-  // register-hungry, so it runs before the accumulators are live); the
-  // window is zeroed once, so frames nt .. FB - 1 read as zero.
+  // register-hungry, so it runs before the accumulators are live); both
+  // window buffers are zeroed once, so frames that are never loaded read 0.
   if (tid < p.na) {
     double sn, cs;
     sincos(static_cast<double>(p.angles[tid]), &sn, &cs);
@@ -276,7 +382,7 @@ __global__ __launch_bounds__(kThreads) void das_opw_tile(EnsembleArgs p, long lo
   }
   {
     uint4* w4 = reinterpret_cast<uint4*>(win);
-    const int n = max_rows * stride * static_cast<int>(sizeof(W)) / 16;
+    const int n = 2 * buffer * static_cast<int>(sizeof(W)) / 16;
     for (int i = tid; i < n; i += kThreads) w4[i] = make_uint4(0u, 0u, 0u, 0u);
   }
 
@@ -284,19 +390,23 @@ __global__ __launch_bounds__(kThreads) void das_opw_tile(EnsembleArgs p, long lo
 #pragma unroll
   for (int j = 0; j < FPL; ++j) acc[j] = make_float2(0.f, 0.f);
 
+  const size_t plane_size = static_cast<size_t>(p.ns) * p.nt;
   for (int cfg = 0; cfg < 2; ++cfg) {
     const W* __restrict__ data = static_cast<const W*>(cfg == 0 ? p.rc : p.cr);
     const float* el = cfg == 0 ? p.xe : p.ye;
     for (int ch = 0; ch < p.nc; ++ch) {
-      __syncthreads();  // the previous channel's angles are done with taps, bounds and window
+      __syncthreads();  // the previous channel is done with taps, bounds and both buffers
       {
         // Tap table of this channel and the exact row window of every angle.
         Term r{kInactive, 0.f, 0.f, 0.f};
-        if (vr_ok) r = receive_term(cfg == 0 ? xr : yr, zr, static_cast<double>(el[ch]), p.c, p.fs, p.fd, p.fn);
+        if (vr_ok)
+          r = receive_term(static_cast<double>(cfg == 0 ? xr : yr), static_cast<double>(zr),
+                           static_cast<double>(el[ch]), p.c, p.fs, p.fd, p.fn);
         for (int a = wave; a < p.na; a += kWaves) {
           Tap tp{-1, 0.f, 0.f, 0.f};
           if (r.i != kInactive) {
-            const Term t = transmit_term(cfg == 0 ? yr : xr, zr, trig[2 * a], trig[2 * a + 1],
+            const Term t = transmit_term(static_cast<double>(cfg == 0 ? yr : xr), static_cast<double>(zr),
+                                         trig[2 * a], trig[2 * a + 1],
                                          static_cast<double>(p.starts[a]), p.c, p.fs, p.fd);
             tp = combine(r, t, p.ns);
           }
@@ -308,33 +418,50 @@ __global__ __launch_bounds__(kThreads) void das_opw_tile(EnsembleArgs p, long lo
         }
       }
       __syncthreads();
-      bool staged = false;  // a stage since the last barrier: the next one must wait for its readers
-      for (int a = 0; a < p.na; ++a) {
-        const int2 b = bounds[a];
-        const int lo = uniform(b.x);
-        const int hi = uniform(b.y);
-        if (lo > hi) continue;  // no tap of the tile for this (channel, angle)
-        const Tap tp = taps[a * kTile + vl];
-        const bool live = tp.i0 >= 0;
-        const int i1 = tp.w != 0.f ? tp.i0 + 1 : tp.i0;
-        const Coef k = make_coef<T, CP>(tp);
-        const W* plane = data + static_cast<size_t>(ch * p.na + a) * p.ns * p.nt;
-        for (int r0 = lo;; r0 += max_rows - 1) {
-          const bool last = r0 + max_rows - 1 >= hi;
-          const int rows = last ? hi + 1 - r0 : max_rows;
-          const bool in_chunk = live && tp.i0 >= r0 && (last || tp.i0 < r0 + max_rows - 1);
-          if (staged) __syncthreads();
-          stage<W, FB>(plane, p.nt, r0, rows, win, stride, vec);
-          __syncthreads();
-          staged = true;
-          if (in_chunk)
-            accumulate<T, CP, FPL>(win + (tp.i0 - r0) * stride + fl, win + (i1 - r0) * stride + fl, k, acc);
-          if (last) break;
+      // Software pipeline over the stages of this channel: stage k+1 streams
+      // into one buffer while stage k is interpolated from the other.
+      const W* plane0 = data + static_cast<size_t>(ch) * p.na * plane_size;
+      Stage cur = plan_stage(bounds, 0, p.na, max_rows);
+      if (!cur.valid) continue;  // no tap of the tile in this channel
+      issue_stage<W, STRIDE>(plane0, plane_size, p.nt, cur, bounds, win, vec);
+      __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+      __syncthreads();
+      int b = 0;
+      while (true) {
+        const Stage nxt = next_stage(cur, bounds, p.na, max_rows);
+        if (nxt.valid) issue_stage<W, STRIDE>(plane0, plane_size, p.nt, nxt, bounds, win + (b ^ 1) * buffer, vec);
+        const W* base = win + b * buffer + fl;
+        int off = 0;
+        for (int a = cur.a0; a < cur.a1; ++a) {
+          int first;  // window row held in this buffer's row 0 for angle a
+          bool take = true;
+          const Tap tp = taps[a * kTile + vl];
+          if (cur.r0 >= 0) {  // one chunk of a long window
+            first = cur.r0;
+            take = tp.i0 >= cur.r0 && (cur.last || tp.i0 < cur.r0 + max_rows - 1);
+          } else {  // whole windows packed back to back
+            const int2 w = window(bounds, a);
+            if (w.x > w.y) continue;
+            first = w.x - off;
+            off += w.y - w.x + 1;
+          }
+          if (tp.i0 >= 0 && take) {
+            const int i1 = tp.w != 0.f ? tp.i0 + 1 : tp.i0;
+            accumulate<T, CP, FPL>(base + (tp.i0 - first) * STRIDE, base + (i1 - first) * STRIDE,
+                                   make_coef<T, CP>(tp), acc);
+          }
         }
+        __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+        __syncthreads();
+        if (!nxt.valid) break;
+        cur = nxt;
+        b ^= 1;
       }
     }
     if constexpr (APERTURES) {
-      if (v < p.nv) {
+      float xo, zo, yo;
+      const long long v = slot_voxel(p, map, tile, vl, xo, zo, yo);
+      if (v >= 0) {
 #pragma unroll
         for (int j = 0; j < FPL; ++j) {
           const int t = fl + kLanesPerVoxel * j;
@@ -349,11 +476,13 @@ __global__ __launch_bounds__(kThreads) void das_opw_tile(EnsembleArgs p, long lo
     }
   }
   if constexpr (!APERTURES) {
+    float xo, zo, yo;
+    const long long v = slot_voxel(p, map, tile, vl, xo, zo, yo);
     float power = 0.f;
 #pragma unroll
     for (int j = 0; j < FPL; ++j) {
       const int t = fl + kLanesPerVoxel * j;
-      if (t < p.nt && v < p.nv) {
+      if (t < p.nt && v >= 0) {
         const float2 s = acc[j];
         power = fmaf(s.x, s.x, fmaf(s.y, s.y, power));
         if (p.signal) {
@@ -370,32 +499,22 @@ __global__ __launch_bounds__(kThreads) void das_opw_tile(EnsembleArgs p, long lo
     // The 8 lanes of a voxel are adjacent: butterfly inside the group.
 #pragma unroll
     for (int off = 1; off < kLanesPerVoxel; off <<= 1) power += __shfl_xor(power, off, kWave);
-    if (fl == 0 && v < p.nv) {
+    if (fl == 0 && v >= 0) {
       p.pd[2 * v] = power / static_cast<float>(p.nt);
       p.pd[2 * v + 1] = 0.f;
     }
   }
 }
 
-// Row stride of the LDS window (in samples): at least FB and such that the 8
-// rows a wave reads in one instruction fall on distinct LDS bank groups.
-int window_stride(int fb, int word_bytes) {
-  const int seg = kLanesPerVoxel * word_bytes / 4;  // dwords one voxel's lanes read
-  int dw = (fb * word_bytes / 4 + seg - 1) / seg * seg;
-  if ((dw / seg) % 2 == 0) dw += seg;
-  return dw * 4 / word_bytes;
-}
-
 template <typename T, int CP, int FPL, bool APERTURES>
-hipError_t launch_shape(const EnsembleArgs& p, int out_storage) {
+hipError_t launch_shape(const EnsembleArgs& p, const TileMap& map, long long tiles, int out_storage) {
   using W = typename Word<T>::type;
   constexpr int FB = kLanesPerVoxel * FPL;
-  const int stride = window_stride(FB, static_cast<int>(sizeof(W)));
-  const int fixed = tile_layout(p.na, stride, static_cast<int>(sizeof(W)), 0).bytes;
-  const int max_rows = std::min(kMaxRows, (kLdsBytes - fixed) / (stride * static_cast<int>(sizeof(W))));
+  constexpr int STRIDE = window_stride(FB, static_cast<int>(sizeof(W)));
+  const int fixed = tile_layout(p.na, STRIDE, static_cast<int>(sizeof(W)), 0).bytes;
+  const int max_rows = std::min(kMaxRows, (kLdsBytes - fixed) / (2 * STRIDE * static_cast<int>(sizeof(W))));
   if (max_rows < kMinRows) return hipErrorNotSupported;
-  const TileLayout layout = tile_layout(p.na, stride, static_cast<int>(sizeof(W)), max_rows);
-  const long long tiles = (p.nv + kTile - 1) / kTile;
+  const TileLayout layout = tile_layout(p.na, STRIDE, static_cast<int>(sizeof(W)), max_rows);
   const long long grid = (tiles + 7) / 8 * 8;
   if (grid > 0x7fffffffLL) return hipErrorInvalidValue;
   constexpr int U = 16 / static_cast<int>(sizeof(W));
@@ -410,45 +529,58 @@ hipError_t launch_shape(const EnsembleArgs& p, int out_storage) {
     if (e != hipSuccess) return e;
     attribute_set = true;
   }
-  kernel<<<dim3(static_cast<unsigned>(grid)), kThreads, layout.bytes>>>(p, tiles, stride, max_rows, vec,
-                                                                        out_storage);
+  kernel<<<dim3(static_cast<unsigned>(grid)), kThreads, layout.bytes>>>(p, map, tiles, max_rows, vec, out_storage);
   return hipGetLastError();
 }
 
 template <typename T, int CP, bool APERTURES>
-hipError_t launch_frames(const EnsembleArgs& p, int out_storage) {
-  if (p.nt <= 16) return launch_shape<T, CP, 2, APERTURES>(p, out_storage);
-  if (p.nt <= 32) return launch_shape<T, CP, 4, APERTURES>(p, out_storage);
-  if (p.nt <= 64) return launch_shape<T, CP, 8, APERTURES>(p, out_storage);
-  if (p.nt <= 104) return launch_shape<T, CP, 13, APERTURES>(p, out_storage);
-  return launch_shape<T, CP, 26, APERTURES>(p, out_storage);
+hipError_t launch_frames(const EnsembleArgs& p, const TileMap& map, long long tiles, int out_storage) {
+  if (p.nt <= 16) return launch_shape<T, CP, 2, APERTURES>(p, map, tiles, out_storage);
+  if (p.nt <= 32) return launch_shape<T, CP, 4, APERTURES>(p, map, tiles, out_storage);
+  if (p.nt <= 64) return launch_shape<T, CP, 8, APERTURES>(p, map, tiles, out_storage);
+  if (p.nt <= 104) return launch_shape<T, CP, 13, APERTURES>(p, map, tiles, out_storage);
+  return launch_shape<T, CP, 26, APERTURES>(p, map, tiles, out_storage);
 }
 
-}  // namespace
-
-hipError_t launch_opw_tile(const EnsembleArgs& p, int channel_storage, int coefficients, int signal_storage,
-                           bool apertures) {
+hipError_t launch_any(const EnsembleArgs& p, const TileMap& map, long long tiles, int channel_storage,
+                      int coefficients, int signal_storage, bool apertures) {
   if (p.na > kMaxAngles || p.nt > kLanesPerVoxel * kMaxFramesPerLane) return hipErrorNotSupported;
   // Plane offsets and in-plane row offsets are int.
   if (static_cast<long long>(p.ns) * p.nt > 0x7fffffffLL) return hipErrorNotSupported;
   if (signal_storage < RCABEAM_STORAGE_C64 || signal_storage > RCABEAM_STORAGE_BF16) return hipErrorInvalidValue;
   if (apertures) {
     if (channel_storage != RCABEAM_STORAGE_C64) return hipErrorInvalidValue;
-    return launch_frames<float2, kCoefFloat, true>(p, signal_storage);
+    return launch_frames<float2, kCoefFloat, true>(p, map, tiles, signal_storage);
   }
   switch (channel_storage) {
     case RCABEAM_STORAGE_C64:
-      return launch_frames<float2, kCoefFloat, false>(p, signal_storage);
+      return launch_frames<float2, kCoefFloat, false>(p, map, tiles, signal_storage);
     case RCABEAM_STORAGE_FP16:
-      if (coefficients == kCoefSingle) return launch_frames<__half2, kCoefSingle, false>(p, signal_storage);
-      if (coefficients == kCoefSplit) return launch_frames<__half2, kCoefSplit, false>(p, signal_storage);
-      return launch_frames<__half2, kCoefFloat, false>(p, signal_storage);
+      if (coefficients == kCoefSingle) return launch_frames<__half2, kCoefSingle, false>(p, map, tiles, signal_storage);
+      if (coefficients == kCoefSplit) return launch_frames<__half2, kCoefSplit, false>(p, map, tiles, signal_storage);
+      return launch_frames<__half2, kCoefFloat, false>(p, map, tiles, signal_storage);
     case RCABEAM_STORAGE_BF16:
-      if (coefficients == kCoefSingle) return launch_frames<bf16x2, kCoefSingle, false>(p, signal_storage);
-      return launch_frames<bf16x2, kCoefFloat, false>(p, signal_storage);
+      if (coefficients == kCoefSingle) return launch_frames<bf16x2, kCoefSingle, false>(p, map, tiles, signal_storage);
+      return launch_frames<bf16x2, kCoefFloat, false>(p, map, tiles, signal_storage);
     default:
       return hipErrorInvalidValue;
   }
+}
+
+}  // namespace
+
+hipError_t launch_opw_tile(const EnsembleArgs& p, int channel_storage, int coefficients, int signal_storage,
+                           bool apertures) {
+  const TileMap map{nullptr, nullptr, nullptr, 0, 0, 0, 0, false};
+  return launch_any(p, map, (p.nv + kTile - 1) / kTile, channel_storage, coefficients, signal_storage, apertures);
+}
+
+hipError_t launch_opw_grid(const EnsembleArgs& p, const float* x, const float* z, const float* y, int nx, int nz,
+                           int ny, int channel_storage, int coefficients, int signal_storage, bool apertures) {
+  const int nbx = (nx + kPatch - 1) / kPatch, nby = (ny + kPatch - 1) / kPatch;
+  const TileMap map{x, z, y, nx, nz, ny, nby, true};
+  return launch_any(p, map, static_cast<long long>(nbx) * nby * nz, channel_storage, coefficients, signal_storage,
+                    apertures);
 }
 
 }  // namespace rcb

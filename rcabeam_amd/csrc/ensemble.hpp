@@ -47,4 +47,9 @@ __device__ __forceinline__ long long xcd_tile(long long tiles) {
 hipError_t launch_opw_tile(const EnsembleArgs& p, int channel_storage, int coefficients, int signal_storage,
                            bool apertures);
 
+// The same on a Cartesian grid (axes x, z, y; voxel id (ix * nz + iz) * ny +
+// iy), tiled in 8 x 8 lateral patches; p.scan is not read.
+hipError_t launch_opw_grid(const EnsembleArgs& p, const float* x, const float* z, const float* y, int nx, int nz,
+                           int ny, int channel_storage, int coefficients, int signal_storage, bool apertures);
+
 }  // namespace rcb

@@ -31,6 +31,9 @@ _RANGE_MESSAGE = {
     "float16": "FP16 IQ requires finite components within +/-65504; rescale inputs explicitly",
     "bfloat16": "BF16 IQ requires finite components within the bfloat16 range; rescale inputs explicitly",
 }
+# Shape limits of rcabeam_opw_grid_device (include/rcabeam_mi355x.h).
+GRID_MAX_ANGLES = 16
+GRID_MAX_FRAMES = 208
 # Bound on the St-SW correlation workspace per launch (voxel tiles beyond it).
 _STSW_WORKSPACE_BYTES = 1 << 28
 
@@ -109,6 +112,7 @@ class _Geometry:
         if not all(np.isfinite(v).all() for v in (*vectors, starts)):
             raise ValueError("Geometry and sample starts must be finite")
         scan = _scan_coords(grid)
+        self.canonical = order is None  # scan follows the grid's own (x, z, y) order
         if order is not None:
             scan = np.ascontiguousarray(scan[order])
         self.nv = len(scan)
@@ -126,13 +130,26 @@ def run_ensemble(packed_rc, packed_cr, g: _Geometry, shape4, method: int, k: int
                  weights=None, first: int = 0, count: int | None = None, signal_storage: int = 0):
     """Enqueue one reconstruction of voxels [first, first + count) over the C ABI.
 
-    float32 channels run rcabeam_ensemble_device; 16-bit channels (OPW only) run
-    rcabeam_opw_mixed_device with float32-accurate coefficients.
+    OPW and separate-aperture IQ of a whole grid run rcabeam_opw_grid_device
+    (8 x 8 lateral tiles; same per-voxel arithmetic as the voxel-list path).
+    Otherwise float32 channels run rcabeam_ensemble_device and 16-bit channels
+    (OPW only) rcabeam_opw_mixed_device, with float32-accurate coefficients.
     """
     t = dev.torch()
     ns, nc, na, nt = shape4
     count = g.nv - first if count is None else count
     c, fs, fd, fn = g.constants
+    whole_grid = first == 0 and count == g.nv and g.canonical
+    if whole_grid and method in (_METHODS["opw"], _METHODS["apertures"]) and na <= GRID_MAX_ANGLES \
+            and nt <= GRID_MAX_FRAMES:
+        nx, nz, ny = g.shape
+        _lib.call(
+            "rcabeam_opw_grid_device", dev.ptr(packed_rc), dev.ptr(packed_cr), _lib.STORAGE[storage],
+            _lib.COEF_FLOAT32, dev.ptr(g.axes[0]), dev.ptr(g.axes[1]), dev.ptr(g.axes[2]), nx, nz, ny,
+            dev.ptr(g.xe), dev.ptr(g.ye), dev.ptr(g.angles), dev.ptr(g.starts), dev.ptr(pd), dev.ptr(signal),
+            signal_storage, method, ns, nc, na, nt, c, fs, fd, fn,
+        )
+        return
     scan = g.scan[first : first + count]
     pd_view = pd[first : first + count]
     planes = 2 if method == _METHODS["apertures"] else 1

@@ -18,7 +18,7 @@ from numpy.typing import NDArray
 
 from rcabeam_amd import _device as dev
 from rcabeam_amd import _lib
-from rcabeam_amd.ensemble import IQStorage, opw_voxel_order, pack_channels
+from rcabeam_amd.ensemble import GRID_MAX_ANGLES, GRID_MAX_FRAMES, IQStorage, opw_voxel_order, pack_channels
 from rcabeam_amd.sim import RCAGeometry, _scan_coords
 
 
@@ -101,6 +101,7 @@ class OpwWorkspace:
             inverse[order] = np.arange(nv)
             self._inverse = dev.to_device(inverse.astype(np.int64), t.int64)
         self._scan = dev.to_device(np.ascontiguousarray(scan, dtype=np.float32), t.float32)
+        self._axes = tuple(dev.to_device(np.asarray(axis, dtype=np.float32), t.float32) for axis in grid)
         self._xe, self._ye, self._angles, self._starts = (dev.to_device(np.ascontiguousarray(v), t.float32)
                                                           for v in vectors)
         words = t.complex64 if iq_storage == "float32" else t.int32
@@ -161,12 +162,22 @@ class OpwWorkspace:
             self._has_output = False
             ns, nc, na, nt = self._channel_shape
             c, fs, fd, fn = self._constants
-            _lib.call(
-                "rcabeam_opw_mixed_device", dev.ptr(self._rc), dev.ptr(self._cr), dev.ptr(self._scan),
-                dev.ptr(self._xe), dev.ptr(self._ye), dev.ptr(self._angles), dev.ptr(self._starts),
-                dev.ptr(self._pd), dev.ptr(self._signal), _lib.STORAGE[self._storage], _lib.STORAGE["complex64"],
-                _lib.COEF_FLOAT32, ns, nc, na, nt, self._nv, c, fs, fd, fn,
-            )
+            if self._inverse is None and na <= GRID_MAX_ANGLES and nt <= GRID_MAX_FRAMES:
+                nx, nz, ny = self._shape
+                _lib.call(
+                    "rcabeam_opw_grid_device", dev.ptr(self._rc), dev.ptr(self._cr), _lib.STORAGE[self._storage],
+                    _lib.COEF_FLOAT32, dev.ptr(self._axes[0]), dev.ptr(self._axes[1]), dev.ptr(self._axes[2]),
+                    nx, nz, ny, dev.ptr(self._xe), dev.ptr(self._ye), dev.ptr(self._angles), dev.ptr(self._starts),
+                    dev.ptr(self._pd), dev.ptr(self._signal), _lib.STORAGE["complex64"], 0, ns, nc, na, nt,
+                    c, fs, fd, fn,
+                )
+            else:
+                _lib.call(
+                    "rcabeam_opw_mixed_device", dev.ptr(self._rc), dev.ptr(self._cr), dev.ptr(self._scan),
+                    dev.ptr(self._xe), dev.ptr(self._ye), dev.ptr(self._angles), dev.ptr(self._starts),
+                    dev.ptr(self._pd), dev.ptr(self._signal), _lib.STORAGE[self._storage],
+                    _lib.STORAGE["complex64"], _lib.COEF_FLOAT32, ns, nc, na, nt, self._nv, c, fs, fd, fn,
+                )
             if self._inverse is not None:
                 t.index_select(self._signal, 0, self._inverse, out=self._iq.view(self._nv, nt))
                 t.index_select(self._pd, 0, self._inverse, out=self._canonical_pd)

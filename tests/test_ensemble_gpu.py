@@ -263,3 +263,55 @@ def test_sixteen_bit_coefficients_error(storage):
     print(f"{storage}: 16-bit coefficients IQ rel L2 {e_iq:.2e}, power {e_pd:.2e}", flush=True)
     assert e_iq < (5e-4 if storage == "float16" else 4e-3)
     assert e_pd < (1e-3 if storage == "float16" else 8e-3)
+
+
+@pytest.mark.parametrize("storage,coefficients", [("float32", 0), ("float16", 0), ("float16", 1), ("bfloat16", 1)])
+def test_grid_tiles_equal_voxel_list(storage, coefficients):
+    """rcabeam_opw_grid_device (8 x 8 lateral tiles) performs, per voxel, the same operations in the same
+    order as the voxel-list entry point, so both give bit-identical IQ and power; grid sizes that are not
+    multiples of the patch leave partial tiles."""
+    import torch
+
+    from rcabeam_amd import _device as dev
+    from rcabeam_amd import _lib
+    from rcabeam_amd.ensemble import _Geometry, pack_channels
+
+    acq = scene.point_scene(grid_shape=(13, 20, 11))
+    lib = _lib.load()
+    g = _Geometry(acq.angles, acq.t_start, acq.grid, _geom(acq))
+    ns, nc, na, nt = acq.rc.shape
+    packed = [pack_channels(dev.to_device(x, torch.complex64), storage) for x in (acq.rc, acq.cr)]
+    c, fs, fd, fn = g.constants
+    code = _lib.STORAGE[storage]
+    out = []
+    for grid in (False, True):
+        pd = torch.full((g.nv, 2), float("nan"), device="cuda")
+        sig = torch.full((g.nv, nt), float("nan"), dtype=torch.complex64, device="cuda")
+        if grid:
+            nx, nz, ny = g.shape
+            _lib.check(lib.rcabeam_opw_grid_device(
+                dev.ptr(packed[0]), dev.ptr(packed[1]), code, coefficients, dev.ptr(g.axes[0]), dev.ptr(g.axes[1]),
+                dev.ptr(g.axes[2]), nx, nz, ny, dev.ptr(g.xe), dev.ptr(g.ye), dev.ptr(g.angles), dev.ptr(g.starts),
+                dev.ptr(pd), dev.ptr(sig), 0, 0, ns, nc, na, nt, c, fs, fd, fn), "grid")
+        else:
+            _lib.check(lib.rcabeam_opw_mixed_device(
+                dev.ptr(packed[0]), dev.ptr(packed[1]), dev.ptr(g.scan), dev.ptr(g.xe), dev.ptr(g.ye),
+                dev.ptr(g.angles), dev.ptr(g.starts), dev.ptr(pd), dev.ptr(sig), code, 0, coefficients, ns, nc, na,
+                nt, g.nv, c, fs, fd, fn), "list")
+        dev.synchronize()
+        out.append((sig.cpu().numpy(), pd.cpu().numpy()))
+    assert np.isfinite(out[1][0]).all() and np.isfinite(out[1][1]).all()
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    np.testing.assert_array_equal(out[0][1], out[1][1])
+
+
+def test_grid_entry_rejects_unsupported_shapes():
+    from rcabeam_amd import _lib
+
+    lib = _lib.load()
+    one = 1
+    # 17 angles exceed the grid kernel; methods other than 0 and 5 are not grid methods.
+    for na, method in ((17, 0), (3, 1)):
+        status = lib.rcabeam_opw_grid_device(one * 16, one * 16, 0, 0, 16, 16, 16, 1, 1, 1, 16, 16, 16, 16, 16, 0,
+                                             0, method, 8, 1, na, 4, 1540.0, 20e6, 15e6, 1.0)
+        assert status == 2, (na, method)
