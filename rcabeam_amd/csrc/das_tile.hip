@@ -15,15 +15,16 @@
 //     builds the tap table of its 64 voxels x angles once (receive and
 //     transmit terms in float64, combined into row, weight and phasor) and,
 //     per angle, the exact row window [min first row, max second row] by
-//     wave reductions.  Receive rows move by < 1.2 samples per voxel in x
-//     and transmit rows by < 0.5 in y, so a patch's window is a few rows;
+//     wave reductions; the next channel's table is built while the current
+//     one streams.  Receive rows move by < 1.2 samples per voxel in x and
+//     transmit rows by < 0.5 in y, so a patch's window is a few rows;
 //   * the windows of consecutive angles are packed into stages of at most
 //     max_rows rows (longer windows are split into chunks overlapping by one
 //     row).  A stage streams rows x frames from the packed (channels, angles,
 //     samples, frames) layout straight into LDS (global_load_lds_dwordx4, no
-//     register staging), double-buffered: the next stage is in flight while
-//     every lane interpolates its voxel's taps for its frames from the
-//     current one; one barrier per stage.
+//     register staging), double-buffered across angles and channels: the
+//     next stage is in flight while every lane interpolates its voxel's taps
+//     for its frames from the current one; one barrier per stage.
 //
 // Arithmetic per tap and frame:
 //   kCoefFloat  : v = a + w (b - a); acc += v * e^{i phi}        (8 FMA)
@@ -201,13 +202,14 @@ struct TileLayout {  // byte offsets inside the dynamic LDS allocation
   int trig, bounds, win, bytes;
 };
 
-// taps: Tap[na][kTile] at offset 0; trig: double[na][2]; bounds: int2[na];
-// win: two buffers W[max_rows][stride].
+// taps: Tap[2][na][kTile] at offset 0 (channels alternate); trig:
+// double[kMaxAngles][2]; bounds: int2[2][kMaxAngles]; win: two buffers
+// W[max_rows][stride].
 __host__ __device__ inline TileLayout tile_layout(int na, int stride, int word_bytes, int max_rows) {
   TileLayout l;
-  l.trig = na * kTile * static_cast<int>(sizeof(Tap));
+  l.trig = 2 * na * kTile * static_cast<int>(sizeof(Tap));
   l.bounds = l.trig + kMaxAngles * 2 * static_cast<int>(sizeof(double));
-  l.win = (l.bounds + kMaxAngles * 2 * 4 + 15) & ~15;
+  l.win = (l.bounds + 2 * kMaxAngles * 2 * 4 + 15) & ~15;
   l.bytes = l.win + 2 * max_rows * stride * word_bytes;
   return l;
 }
@@ -246,6 +248,22 @@ __device__ __forceinline__ long long slot_voxel(const EnsembleArgs& p, const Til
   return v;
 }
 
+// Row windows of one channel held in registers: lane a (< na) holds angle
+// a's [first row, last row] (empty: first > last); any lane reads them with
+// v_readlane, so stage planning costs no LDS round trips.
+struct Windows {
+  int lo, hi;
+  __device__ __forceinline__ int2 at(int a) const {
+    return make_int2(__builtin_amdgcn_readlane(lo, a), __builtin_amdgcn_readlane(hi, a));
+  }
+};
+
+__device__ __forceinline__ Windows load_windows(const int2* bounds, int na) {
+  const int lane = static_cast<int>(threadIdx.x % kWave);
+  const int2 b = lane < na ? bounds[lane] : make_int2(kBig, -kBig);
+  return Windows{b.x, b.y};
+}
+
 // One stage: angles [a0, a1) whose windows are packed back to back, or one
 // chunk (rows [r0, r0 + rows)) of a window longer than max_rows (r0 >= 0).
 struct Stage {
@@ -253,23 +271,18 @@ struct Stage {
   bool last, valid;
 };
 
-__device__ __forceinline__ int2 window(const int2* bounds, int a) {
-  const int2 b = bounds[a];
-  return make_int2(uniform(b.x), uniform(b.y));
-}
-
 // The stage that starts at the first live angle at or after a.
-__device__ __forceinline__ Stage plan_stage(const int2* bounds, int a, int na, int max_rows) {
+__device__ __forceinline__ Stage plan_stage(const Windows& w, int a, int na, int max_rows) {
   for (; a < na; ++a) {
-    const int2 w = window(bounds, a);
-    if (w.x > w.y) continue;
-    const int rows = w.y - w.x + 1;
-    if (rows > max_rows) return Stage{a, a + 1, w.x, max_rows, false, true};
+    const int2 b = w.at(a);
+    if (b.x > b.y) continue;
+    const int rows = b.y - b.x + 1;
+    if (rows > max_rows) return Stage{a, a + 1, b.x, max_rows, false, true};
     int total = rows, a1 = a + 1;
     for (; a1 < na; ++a1) {
-      const int2 w1 = window(bounds, a1);
-      if (w1.x > w1.y) continue;
-      const int r1 = w1.y - w1.x + 1;
+      const int2 b1 = w.at(a1);
+      if (b1.x > b1.y) continue;
+      const int r1 = b1.y - b1.x + 1;
       if (total + r1 > max_rows) break;
       total += r1;
     }
@@ -278,14 +291,14 @@ __device__ __forceinline__ Stage plan_stage(const int2* bounds, int a, int na, i
   return Stage{0, 0, -1, 0, true, false};
 }
 
-__device__ __forceinline__ Stage next_stage(const Stage& s, const int2* bounds, int na, int max_rows) {
+__device__ __forceinline__ Stage next_stage(const Stage& s, const Windows& w, int na, int max_rows) {
   if (s.r0 >= 0 && !s.last) {
-    const int hi = window(bounds, s.a0).y;
+    const int hi = w.at(s.a0).y;
     const int r0 = s.r0 + max_rows - 1;
     const bool last = r0 + max_rows - 1 >= hi;
     return Stage{s.a0, s.a1, r0, last ? hi + 1 - r0 : max_rows, last, true};
   }
-  return plan_stage(bounds, s.a1, na, max_rows);
+  return plan_stage(w, s.a1, na, max_rows);
 }
 
 // Copy rows [r_first, r_first + rows) x frames [0, nt) of one plane into LDS
@@ -323,17 +336,17 @@ __device__ __forceinline__ void copy_rows(const W* __restrict__ plane, int nt, i
 
 template <typename W, int STRIDE>
 __device__ __forceinline__ void issue_stage(const W* plane0, size_t plane_size, int nt, const Stage& s,
-                                            const int2* bounds, W* buf, bool vec) {
+                                            const Windows& w, W* buf, bool vec) {
   if (s.r0 >= 0) {
     copy_rows<W, STRIDE>(plane0 + s.a0 * plane_size, nt, s.r0, s.rows, buf, vec);
     return;
   }
   int off = 0;
   for (int a = s.a0; a < s.a1; ++a) {
-    const int2 w = window(bounds, a);
-    if (w.x > w.y) continue;
-    copy_rows<W, STRIDE>(plane0 + a * plane_size, nt, w.x, w.y - w.x + 1, buf + off * STRIDE, vec);
-    off += w.y - w.x + 1;
+    const int2 b = w.at(a);
+    if (b.x > b.y) continue;
+    copy_rows<W, STRIDE>(plane0 + a * plane_size, nt, b.x, b.y - b.x + 1, buf + off * STRIDE, vec);
+    off += b.y - b.x + 1;
   }
 }
 
@@ -354,9 +367,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
   constexpr int STRIDE = window_stride(FB, static_cast<int>(sizeof(W)));
   extern __shared__ __align__(16) unsigned char lds[];
   const TileLayout layout = tile_layout(p.na, STRIDE, static_cast<int>(sizeof(W)), max_rows);
-  Tap* taps = reinterpret_cast<Tap*>(lds);                      // [na][kTile]
+  Tap* taps = reinterpret_cast<Tap*>(lds);                      // [2][na][kTile]
   double* trig = reinterpret_cast<double*>(lds + layout.trig);  // [na][cos, sin]
-  int2* bounds = reinterpret_cast<int2*>(lds + layout.bounds);  // [na]: first row, last row
+  int2* bounds = reinterpret_cast<int2*>(lds + layout.bounds);  // [2][kMaxAngles]: first row, last row
   W* win = reinterpret_cast<W*>(lds + layout.win);              // [2][max_rows][STRIDE]
   const int buffer = max_rows * STRIDE;
 
@@ -385,93 +398,127 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
     const int n = 2 * buffer * static_cast<int>(sizeof(W)) / 16;
     for (int i = tid; i < n; i += kThreads) w4[i] = make_uint4(0u, 0u, 0u, 0u);
   }
+  __syncthreads();
+
+  // Channels run as one sequence q = aperture * nc + channel.  build(q): tap
+  // table of the tile's voxels x angles for channel q (float64 receive and
+  // transmit terms) and the exact row window of each angle.
+  const int nq = 2 * p.nc;
+  auto build = [&](int q) {
+    const int cfg = q >= p.nc ? 1 : 0;
+    const int ch = q - cfg * p.nc;
+    Tap* tq = taps + (q & 1) * p.na * kTile;
+    int2* bq = bounds + (q & 1) * kMaxAngles;
+    Term r{kInactive, 0.f, 0.f, 0.f};
+    if (vr_ok)
+      r = receive_term(static_cast<double>(cfg == 0 ? xr : yr), static_cast<double>(zr),
+                       static_cast<double>((cfg == 0 ? p.xe : p.ye)[ch]), p.c, p.fs, p.fd, p.fn);
+    for (int a = wave; a < p.na; a += kWaves) {
+      Tap tp{-1, 0.f, 0.f, 0.f};
+      if (r.i != kInactive) {
+        const Term t = transmit_term(static_cast<double>(cfg == 0 ? yr : xr), static_cast<double>(zr),
+                                     trig[2 * a], trig[2 * a + 1], static_cast<double>(p.starts[a]), p.c, p.fs,
+                                     p.fd);
+        tp = combine(r, t, p.ns);
+      }
+      tq[a * kTile + lane] = tp;
+      const bool live = tp.i0 >= 0;
+      const int lo = wave_min(live ? tp.i0 : kBig);
+      const int hi = wave_max(live ? tp.i0 + (tp.w != 0.f ? 1 : 0) : -kBig);
+      if (lane == 0) bq[a] = make_int2(lo, hi);
+    }
+  };
+  auto plane_of = [&](int q) {
+    const int cfg = q >= p.nc ? 1 : 0;
+    const W* data = static_cast<const W*>(cfg == 0 ? p.rc : p.cr);
+    return data + static_cast<size_t>(q - cfg * p.nc) * p.na * p.ns * p.nt;
+  };
 
   float2 acc[FPL];
 #pragma unroll
   for (int j = 0; j < FPL; ++j) acc[j] = make_float2(0.f, 0.f);
 
   const size_t plane_size = static_cast<size_t>(p.ns) * p.nt;
-  for (int cfg = 0; cfg < 2; ++cfg) {
-    const W* __restrict__ data = static_cast<const W*>(cfg == 0 ? p.rc : p.cr);
-    const float* el = cfg == 0 ? p.xe : p.ye;
-    for (int ch = 0; ch < p.nc; ++ch) {
-      __syncthreads();  // the previous channel is done with taps, bounds and both buffers
-      {
-        // Tap table of this channel and the exact row window of every angle.
-        Term r{kInactive, 0.f, 0.f, 0.f};
-        if (vr_ok)
-          r = receive_term(static_cast<double>(cfg == 0 ? xr : yr), static_cast<double>(zr),
-                           static_cast<double>(el[ch]), p.c, p.fs, p.fd, p.fn);
-        for (int a = wave; a < p.na; a += kWaves) {
-          Tap tp{-1, 0.f, 0.f, 0.f};
-          if (r.i != kInactive) {
-            const Term t = transmit_term(static_cast<double>(cfg == 0 ? yr : xr), static_cast<double>(zr),
-                                         trig[2 * a], trig[2 * a + 1],
-                                         static_cast<double>(p.starts[a]), p.c, p.fs, p.fd);
-            tp = combine(r, t, p.ns);
-          }
-          taps[a * kTile + lane] = tp;
-          const bool live = tp.i0 >= 0;
-          const int lo = wave_min(live ? tp.i0 : kBig);
-          const int hi = wave_max(live ? tp.i0 + (tp.w != 0.f ? 1 : 0) : -kBig);
-          if (lane == 0) bounds[a] = make_int2(lo, hi);
+  build(0);
+  __syncthreads();
+  // Software pipeline: while stage k is interpolated from one buffer, stage
+  // k+1 (of this channel or, at a channel's end, the first of the next one)
+  // streams into the other, and channel q+1's taps are built during channel
+  // q's first stage.  One barrier per stage.
+  int b = 0;
+  Stage pending{0, 0, -1, 0, true, false};
+  for (int q = 0; q < nq; ++q) {
+    if (q + 1 < nq) build(q + 1);
+    const Windows w = load_windows(bounds + (q & 1) * kMaxAngles, p.na);
+    const W* plane0 = plane_of(q);
+    Stage cur = pending;
+    if (!cur.valid) {
+      cur = plan_stage(w, 0, p.na, max_rows);
+      if (cur.valid) issue_stage<W, STRIDE>(plane0, plane_size, p.nt, cur, w, win + b * buffer, vec);
+    }
+    __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+    __syncthreads();  // stage landed; channel q+1's taps and windows visible
+    pending.valid = false;
+    const Tap* tq = taps + (q & 1) * p.na * kTile;
+    while (cur.valid) {
+      Stage nxt = next_stage(cur, w, p.na, max_rows);
+      bool crossing = false;
+      if (nxt.valid) {
+        issue_stage<W, STRIDE>(plane0, plane_size, p.nt, nxt, w, win + (b ^ 1) * buffer, vec);
+      } else if (q + 1 < nq) {
+        const Windows w1 = load_windows(bounds + ((q + 1) & 1) * kMaxAngles, p.na);
+        nxt = plan_stage(w1, 0, p.na, max_rows);
+        if (nxt.valid) issue_stage<W, STRIDE>(plane_of(q + 1), plane_size, p.nt, nxt, w1, win + (b ^ 1) * buffer, vec);
+        crossing = true;
+      }
+      const W* base = win + b * buffer + fl;
+      int off = 0;
+      for (int a = cur.a0; a < cur.a1; ++a) {
+        int first;  // window row held in this buffer's row 0 for angle a
+        bool take = true;
+        const Tap tp = tq[a * kTile + vl];
+        if (cur.r0 >= 0) {  // one chunk of a long window
+          first = cur.r0;
+          take = tp.i0 >= cur.r0 && (cur.last || tp.i0 < cur.r0 + max_rows - 1);
+        } else {  // whole windows packed back to back
+          const int2 bw = w.at(a);
+          if (bw.x > bw.y) continue;
+          first = bw.x - off;
+          off += bw.y - bw.x + 1;
+        }
+        if (tp.i0 >= 0 && take) {
+          const int i1 = tp.w != 0.f ? tp.i0 + 1 : tp.i0;
+          accumulate<T, CP, FPL>(base + (tp.i0 - first) * STRIDE, base + (i1 - first) * STRIDE,
+                                 make_coef<T, CP>(tp), acc);
         }
       }
-      __syncthreads();
-      // Software pipeline over the stages of this channel: stage k+1 streams
-      // into one buffer while stage k is interpolated from the other.
-      const W* plane0 = data + static_cast<size_t>(ch) * p.na * plane_size;
-      Stage cur = plan_stage(bounds, 0, p.na, max_rows);
-      if (!cur.valid) continue;  // no tap of the tile in this channel
-      issue_stage<W, STRIDE>(plane0, plane_size, p.nt, cur, bounds, win, vec);
       __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
       __syncthreads();
-      int b = 0;
-      while (true) {
-        const Stage nxt = next_stage(cur, bounds, p.na, max_rows);
-        if (nxt.valid) issue_stage<W, STRIDE>(plane0, plane_size, p.nt, nxt, bounds, win + (b ^ 1) * buffer, vec);
-        const W* base = win + b * buffer + fl;
-        int off = 0;
-        for (int a = cur.a0; a < cur.a1; ++a) {
-          int first;  // window row held in this buffer's row 0 for angle a
-          bool take = true;
-          const Tap tp = taps[a * kTile + vl];
-          if (cur.r0 >= 0) {  // one chunk of a long window
-            first = cur.r0;
-            take = tp.i0 >= cur.r0 && (cur.last || tp.i0 < cur.r0 + max_rows - 1);
-          } else {  // whole windows packed back to back
-            const int2 w = window(bounds, a);
-            if (w.x > w.y) continue;
-            first = w.x - off;
-            off += w.y - w.x + 1;
-          }
-          if (tp.i0 >= 0 && take) {
-            const int i1 = tp.w != 0.f ? tp.i0 + 1 : tp.i0;
-            accumulate<T, CP, FPL>(base + (tp.i0 - first) * STRIDE, base + (i1 - first) * STRIDE,
-                                   make_coef<T, CP>(tp), acc);
-          }
-        }
-        __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
-        __syncthreads();
-        if (!nxt.valid) break;
-        cur = nxt;
-        b ^= 1;
+      b ^= 1;
+      if (crossing) {
+        pending = nxt;
+        break;
       }
+      cur = nxt;
     }
     if constexpr (APERTURES) {
-      float xo, zo, yo;
-      const long long v = slot_voxel(p, map, tile, vl, xo, zo, yo);
-      if (v >= 0) {
+      if (q == p.nc - 1 || q == nq - 1) {  // one aperture complete: store it, restart the sums
+        const int cfg = q == nq - 1 ? 1 : 0;
+        float xo, zo, yo;
+        const long long v = slot_voxel(p, map, tile, vl, xo, zo, yo);
+        if (v >= 0) {
 #pragma unroll
-        for (int j = 0; j < FPL; ++j) {
-          const int t = fl + kLanesPerVoxel * j;
-          if (t < p.nt) p.signal[(2 * v + cfg) * p.nt + t] = acc[j];
-          acc[j] = make_float2(0.f, 0.f);
+          for (int j = 0; j < FPL; ++j) {
+            const int t = fl + kLanesPerVoxel * j;
+            if (t < p.nt) p.signal[(2 * v + cfg) * p.nt + t] = acc[j];
+          }
+          if (fl == 0) {
+            p.pd[2 * v] = 0.f;
+            p.pd[2 * v + 1] = 0.f;
+          }
         }
-        if (fl == 0) {
-          p.pd[2 * v] = 0.f;
-          p.pd[2 * v + 1] = 0.f;
-        }
+#pragma unroll
+        for (int j = 0; j < FPL; ++j) acc[j] = make_float2(0.f, 0.f);
       }
     }
   }
