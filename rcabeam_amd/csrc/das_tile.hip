@@ -302,27 +302,28 @@ __device__ __forceinline__ Stage next_stage(const Stage& s, const Windows& w, in
 }
 
 // Copy rows [r_first, r_first + rows) x frames [0, nt) of one plane into LDS
-// rows starting at dst.  vec: each lane moves one 16-byte unit straight from
-// HBM into LDS (global_load_lds_dwordx4, asynchronous: the caller waits on
-// vmcnt before the barrier); the lanes of a wave write 64 consecutive units,
-// as the instruction requires.  Otherwise a synchronous element copy.
+// rows starting at dst.  vec: wave w copies rows w, w + 8, ...; each lane
+// moves one 16-byte unit of the row straight from HBM into LDS
+// (global_load_lds_dwordx4, asynchronous: the caller waits on vmcnt before
+// the barrier), so one instruction moves 1 KiB contiguous in both memories.
+// Otherwise a synchronous element copy.
 template <typename W, int STRIDE>
 __device__ __forceinline__ void copy_rows(const W* __restrict__ plane, int nt, int r_first, int rows, W* dst,
                                           bool vec) {
   const W* __restrict__ src = plane + static_cast<size_t>(r_first) * nt;
   if (vec) {
     constexpr int U = 16 / static_cast<int>(sizeof(W));
-    constexpr int UPR = STRIDE / U;  // LDS units per window row
-    const int n = rows * UPR;
-    for (int base = (threadIdx.x / kWave) * kWave; base < n; base += kThreads) {
-      const int u = base + static_cast<int>(threadIdx.x % kWave);
-      const int row = u / UPR;
-      const int col = (u - row * UPR) * U;
-      if (u < n && col < nt)
-        __builtin_amdgcn_global_load_lds(
-            reinterpret_cast<const void*>(src + row * nt + col),
-            reinterpret_cast<__attribute__((address_space(3))) void*>(reinterpret_cast<uintptr_t>(dst + base * U)),
-            16, 0, 0);
+    const int units = nt / U;  // 16-byte units of one row (nt is a multiple of U)
+    const int lane = static_cast<int>(threadIdx.x % kWave);
+    for (int row = static_cast<int>(threadIdx.x / kWave); row < rows; row += kWaves) {
+      const W* s_row = src + row * nt;
+      W* d_row = dst + row * STRIDE;
+      for (int c0 = 0; c0 < units; c0 += kWave)
+        if (c0 + lane < units)
+          __builtin_amdgcn_global_load_lds(
+              reinterpret_cast<const void*>(s_row + (c0 + lane) * U),
+              reinterpret_cast<__attribute__((address_space(3))) void*>(reinterpret_cast<uintptr_t>(d_row + c0 * U)),
+              16, 0, 0);
     }
   } else {
     const int n = rows * STRIDE;
