@@ -189,6 +189,72 @@ def cpu_baseline(scene, rc, cr, rank_start, voxels: int):
     }
 
 
+def adjoint_timing(reps: int = 3) -> dict:
+    """Differentiable-OPW workload: adjoint kernel time / forward kernel time.
+
+    64 + 64 elements at 200 um, c = 1540 m/s, fs = 20 MHz, fd = 15 MHz, 3 angles (-5, 0, 5 deg),
+    256 samples, 32 frames, grid 64 x 128 x 64 over +/-6.4 mm x 5-15 mm, F = 1; synthetic
+    (seeded Gaussian) complex64 channels, resident in HBM; both kernels timed with HIP events.
+    """
+    import torch
+
+    from rcabeam_amd import _device as dev
+    from rcabeam_amd import _lib
+    from rcabeam_amd.ensemble import _Geometry, pack_channels
+    from rcabeam_amd.sim import RCAGeometry
+
+    lib = _lib.load()
+    ns, nc, na, nt = 256, 64, 3, 32
+    el = ((np.arange(nc) - (nc - 1) / 2) * 200e-6).astype(np.float32)
+    geom = RCAGeometry(el, el.copy(), c=1540.0, fs=20e6, f_demod=15e6, fnumber=1.0)
+    grid = (np.linspace(-6.4e-3, 6.4e-3, 64), np.linspace(5e-3, 15e-3, 128), np.linspace(-6.4e-3, 6.4e-3, 64))
+    angles = np.deg2rad([-5.0, 0.0, 5.0])
+    g = _Geometry(angles, 0.0, grid, geom)
+    gen = torch.Generator(device="cuda").manual_seed(42)
+    raw = [torch.randn((ns, nc, na, nt), dtype=torch.complex64, device="cuda", generator=gen) for _ in range(2)]
+    packed = [pack_channels(x, "float32") for x in raw]
+    nx, nz, ny = g.shape
+    c, fs, fd, fn = g.constants
+    pd = dev.empty((g.nv, 2), torch.float32)
+    iq = dev.empty((g.nv, nt), torch.complex64)
+    grad_iq = torch.randn((g.nv, nt), dtype=torch.complex64, device="cuda", generator=gen)
+    grads = [torch.zeros((nc, na, ns, nt), dtype=torch.complex64, device="cuda") for _ in range(2)]
+    grad_c = torch.zeros((1,), dtype=torch.float64, device="cuda")
+    axes = [dev.ptr(a) for a in g.axes]
+    common = [dev.ptr(g.xe), dev.ptr(g.ye), dev.ptr(g.angles), dev.ptr(g.starts)]
+
+    def forward():
+        _lib.check(lib.rcabeam_opw_grid_device(dev.ptr(packed[0]), dev.ptr(packed[1]), 0, 0, *axes, nx, nz, ny,
+                                               *common, dev.ptr(pd), dev.ptr(iq), 0, 0, ns, nc, na, nt,
+                                               c, fs, fd, fn), "forward")
+
+    def adjoint():
+        _lib.check(lib.rcabeam_opw_adjoint_grid_device(dev.ptr(packed[0]), dev.ptr(packed[1]), *axes, nx, nz, ny,
+                                                       *common, dev.ptr(grad_iq), dev.ptr(grads[0]),
+                                                       dev.ptr(grads[1]), dev.ptr(grad_c), ns, nc, na, nt,
+                                                       c, fs, fd, fn), "adjoint")
+
+    times = {}
+    for name, fn_ in (("forward", forward), ("adjoint", adjoint)):
+        fn_()
+        torch.cuda.synchronize()
+        ms = []
+        for _ in range(reps):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            fn_()
+            b.record()
+            torch.cuda.synchronize()
+            ms.append(a.elapsed_time(b))
+        times[name] = float(np.median(ms))
+    return {
+        "workload": "64+64 el, IQ 256x64x3x32, grid 64x128x64, synthetic channels",
+        "forward_ms": times["forward"],
+        "adjoint_ms": times["adjoint"],
+        "ratio": times["adjoint"] / times["forward"],
+    }
+
+
 def traffic_from_profiles(name: str):
     summary = PROFILES / "traffic.json"
     if not summary.exists():
@@ -276,6 +342,8 @@ def main() -> int:
             "traffic": traffic_from_profiles("das_opw"),
         },
     }
+    log("bench: adjoint / forward on the differentiable-OPW workload")
+    record["adjoint"] = adjoint_timing()
     if not args.no_cpu_baseline:
         log("bench: CPU baseline (oracle) on a voxel sample")
         record["cpu_baseline"] = cpu_baseline(scene, rc, cr, rank_start, args.cpu_voxels)

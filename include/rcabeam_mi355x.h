@@ -74,6 +74,27 @@ rcabeam_status rcabeam_opw_grid_device(const void* rc, const void* cr, int chann
                                        size_t ns, size_t nc, size_t na, size_t nt, float c, float fs, float fd,
                                        float fn);
 
+/* Inverse of rcabeam_pack_ensemble_device: complex64 (channels, angles,
+ * samples, frames) back to (samples, channels, angles, frames).  Used to
+ * return channel gradients in the caller's layout. */
+rcabeam_status rcabeam_unpack_ensemble_device(const void* source, void* target, size_t ns, size_t nc, size_t na,
+                                              size_t nt);
+
+/* Adjoint (transpose) of rcabeam_opw_grid_device for complex64 channels, the
+ * backward pass of a differentiable OPW (no reference counterpart).  rc, cr:
+ * the forward's packed complex64 channels; grad_iq: (nv, nt) complex64
+ * gradient dL/dRe(IQ) + i dL/dIm(IQ) of a real loss.  Accumulates (the
+ * caller zeroes them) grad_rc, grad_cr: packed (channels, angles, samples,
+ * frames) complex64 gradients in the same convention, and *grad_c (float64):
+ * dL/dc.  Taps are recomputed (nothing is stored between the passes), frames
+ * in chunks of 32; na <= 16.  Other arguments as rcabeam_opw_grid_device. */
+rcabeam_status rcabeam_opw_adjoint_grid_device(const void* rc, const void* cr, const float* x, const float* z,
+                                               const float* y, size_t nx, size_t nz, size_t ny, const float* xe,
+                                               const float* ye, const float* angles, const float* starts,
+                                               const void* grad_iq, void* grad_rc, void* grad_cr, double* grad_c,
+                                               size_t ns, size_t nc, size_t na, size_t nt, float c, float fs,
+                                               float fd, float fn);
+
 /* Adds to *out (a device counter the caller zeroes) the number of
  * (voxel, aperture, channel, angle) taps inside the dynamic aperture and the
  * closed sample window: the work the ensemble kernels perform.  Used for

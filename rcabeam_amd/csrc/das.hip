@@ -740,3 +740,26 @@ extern "C" rcabeam_status rcabeam_opw_grid_device(const void* rc, const void* cr
   if (e == hipErrorNotSupported) return RCABEAM_ERROR_ARGUMENT;
   return to_status(e);
 }
+
+extern "C" rcabeam_status rcabeam_opw_adjoint_grid_device(const void* rc, const void* cr, const float* x,
+                                                          const float* z, const float* y, size_t nx, size_t nz,
+                                                          size_t ny, const float* xe, const float* ye,
+                                                          const float* angles, const float* starts,
+                                                          const void* grad_iq, void* grad_rc, void* grad_cr,
+                                                          double* grad_c, size_t ns, size_t nc, size_t na, size_t nt,
+                                                          float c, float fs, float fd, float fn) {
+  if (!x || !z || !y || !nx || !nz || !ny || nx > 0x7fffffff || nz > 0x7fffffff || ny > 0x7fffffff)
+    return RCABEAM_ERROR_ARGUMENT;
+  if (!grad_iq || !grad_rc || !grad_cr || !grad_c) return RCABEAM_ERROR_ARGUMENT;
+  const size_t nv = nx * nz * ny;
+  float dummy_pd = 0.f;  // validation only: pd is not written by the adjoint
+  if (!valid_ensemble(rc, cr, x, xe, ye, angles, starts, &dummy_pd, ns, nc, na, nt, nv, c, fs, fd, fn))
+    return RCABEAM_ERROR_ARGUMENT;
+  EnsembleArgs p = make_args(rc, cr, nullptr, xe, ye, angles, starts, nullptr, nullptr, nullptr, nullptr, ns, nc, na,
+                             nt, nv, c, fs, fd, fn);
+  const hipError_t e = launch_opw_adjoint_grid(p, x, z, y, static_cast<int>(nx), static_cast<int>(nz),
+                                               static_cast<int>(ny), static_cast<const float2*>(grad_iq),
+                                               static_cast<float2*>(grad_rc), static_cast<float2*>(grad_cr), grad_c);
+  if (e == hipErrorNotSupported) return RCABEAM_ERROR_ARGUMENT;
+  return to_status(e);
+}

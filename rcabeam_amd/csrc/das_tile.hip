@@ -615,6 +615,207 @@ hipError_t launch_any(const EnsembleArgs& p, const TileMap& map, long long tiles
   }
 }
 
+// ---- adjoint (transpose) of the OPW delay-and-sum ------------------------
+//
+// For IQ[v, t] = sum over taps of ((1 - w) S[i0, t] + w S[i0 + 1, t]) e^{i phi}
+// and a real loss with gradient g = dL/dRe IQ + i dL/dIm IQ, the channel
+// gradient is the adjoint applied to g: S[i0, t] += conj((1 - w) e^{i phi}) g,
+// S[i0 + 1, t] += conj(w e^{i phi}) g, and
+//   dL/dc = sum over taps, frames of Re(conj(g) dIQ/dc),
+//   dIQ/dc = -(tau / c) e^{i phi} ((S[i0+1] - S[i0]) fs + i 2 pi fd v),
+// v the interpolated sample, since every delay is a distance over c (the
+// sample start t0 and the aperture mask do not depend on c).  The tiles and
+// tap tables are the forward's (taps recomputed, nothing stored between the
+// passes); frames go in chunks of at most 32.
+//
+// Voxels of a tile share sample rows, so their contributions collide.  Each
+// half-wave (16 per workgroup) owns 32 frame columns and one private LDS
+// gradient window: it takes its 4 voxels one after another, every lane adding
+// to its own frame column with a plain read-modify-write (no LDS atomics, no
+// address conflicts).  Per (channel, angle) chunk the 16 windows are summed
+// and added to the packed gradient planes with global float atomics.
+struct AdjointArgs {
+  EnsembleArgs p;     // rc, cr: packed complex64 channels of the forward pass
+  const float2* giq;  // (nv, nt) gradient of the loss w.r.t. IQ
+  float2* grc;        // packed (channels, angles, samples, frames) gradients, accumulated
+  float2* gcr;
+  double* gc;         // accumulated dL/dc
+  int f0, nf;         // frame chunk [f0, f0 + nf) of this launch, nf <= 32
+};
+
+constexpr int kHalf = kWave / 2;                      // frame columns of a half-wave
+constexpr int kGroups = kThreads / kHalf;             // private gradient windows per workgroup
+constexpr int kVoxelsPerGroup = kTile / kGroups;      // voxels a half-wave takes in turn
+
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void das_opw_adjoint(
+    AdjointArgs args, TileMap map, long long tiles, int max_rows) {
+  const EnsembleArgs& p = args.p;
+  extern __shared__ __align__(16) unsigned char lds[];
+  Tap* taps = reinterpret_cast<Tap*>(lds);                                      // [na][kTile]
+  double* trig = reinterpret_cast<double*>(lds + p.na * kTile * sizeof(Tap));   // [kMaxAngles][2]
+  int2* bounds = reinterpret_cast<int2*>(trig + 2 * kMaxAngles);                // [kMaxAngles]
+  double* partial = reinterpret_cast<double*>(bounds + kMaxAngles);             // [kWaves]
+  float2* win_s = reinterpret_cast<float2*>(partial + kWaves);                  // [max_rows][kHalf] samples
+  float2* win_g = win_s + max_rows * kHalf;                                     // [kGroups][max_rows][kHalf]
+
+  const long long tile = xcd_tile(tiles);
+  if (tile >= tiles) return;
+  const int tid = threadIdx.x;
+  const int lane = tid & (kWave - 1);
+  const int wave = tid / kWave;
+  const int group = tid / kHalf;  // half-wave
+  const int t = tid % kHalf;      // frame column (chunk-relative)
+  float xr, zr, yr;
+  const bool vr_ok = slot_voxel(p, map, tile, lane, xr, zr, yr) >= 0;
+  if (tid < p.na) {
+    double sn, cs;
+    sincos(static_cast<double>(p.angles[tid]), &sn, &cs);
+    trig[2 * tid] = cs;
+    trig[2 * tid + 1] = sn;
+  }
+  // IQ gradients of this half-wave's voxels (slots group + 16 k) at frame t.
+  float2 g[kVoxelsPerGroup];
+#pragma unroll
+  for (int k = 0; k < kVoxelsPerGroup; ++k) {
+    float xo, zo, yo;
+    const long long v = slot_voxel(p, map, tile, group + kGroups * k, xo, zo, yo);
+    g[k] = (v >= 0 && t < args.nf) ? args.giq[v * p.nt + args.f0 + t] : make_float2(0.f, 0.f);
+  }
+  for (int i = tid; i < kGroups * max_rows * kHalf; i += kThreads) win_g[i] = make_float2(0.f, 0.f);
+  float2* my_g = win_g + group * max_rows * kHalf + t;
+  const float fs = static_cast<float>(p.fs);
+  const float omega = static_cast<float>(2.0 * 3.14159265358979323846 * p.fd);
+  double dc = 0.0;
+  const size_t plane_size = static_cast<size_t>(p.ns) * p.nt;
+
+  for (int q = 0; q < 2 * p.nc; ++q) {
+    const int cfg = q >= p.nc ? 1 : 0;
+    const int ch = q - cfg * p.nc;
+    __syncthreads();  // previous channel done with taps, bounds and windows
+    {
+      Term r{kInactive, 0.f, 0.f, 0.f};
+      if (vr_ok)
+        r = receive_term(static_cast<double>(cfg == 0 ? xr : yr), static_cast<double>(zr),
+                         static_cast<double>((cfg == 0 ? p.xe : p.ye)[ch]), p.c, p.fs, p.fd, p.fn);
+      for (int a = wave; a < p.na; a += kWaves) {
+        Tap tp{-1, 0.f, 0.f, 0.f};
+        if (r.i != kInactive) {
+          const Term tt = transmit_term(static_cast<double>(cfg == 0 ? yr : xr), static_cast<double>(zr),
+                                        trig[2 * a], trig[2 * a + 1], static_cast<double>(p.starts[a]), p.c, p.fs,
+                                        p.fd);
+          tp = combine(r, tt, p.ns);
+        }
+        taps[a * kTile + lane] = tp;
+        const bool live = tp.i0 >= 0;
+        const int lo = wave_min(live ? tp.i0 : kBig);
+        const int hi = wave_max(live ? tp.i0 + (tp.w != 0.f ? 1 : 0) : -kBig);
+        if (lane == 0) bounds[a] = make_int2(lo, hi);
+      }
+    }
+    __syncthreads();
+    const Windows wins = load_windows(bounds, p.na);
+    const float2* plane0 = static_cast<const float2*>(cfg == 0 ? p.rc : p.cr) + static_cast<size_t>(ch) * p.na * plane_size;
+    float2* gplane0 = (cfg == 0 ? args.grc : args.gcr) + static_cast<size_t>(ch) * p.na * plane_size;
+    for (int a = 0; a < p.na; ++a) {
+      const int2 bw = wins.at(a);
+      if (bw.x > bw.y) continue;
+      const float t0 = p.starts[a];
+      for (int r0 = bw.x;; r0 += max_rows - 1) {
+        const bool last = r0 + max_rows - 1 >= bw.y;
+        const int rows = last ? bw.y + 1 - r0 : max_rows;
+        __syncthreads();  // previous chunk's readers are done with win_s
+        {
+          const float2* src = plane0 + a * plane_size + static_cast<size_t>(r0) * p.nt + args.f0;
+          for (int i = tid; i < rows * kHalf; i += kThreads) {
+            const int row = i / kHalf;
+            const int col = i - row * kHalf;
+            win_s[i] = col < args.nf ? src[row * p.nt + col] : make_float2(0.f, 0.f);
+          }
+        }
+        __syncthreads();
+        float acc = 0.f;
+#pragma unroll
+        for (int k = 0; k < kVoxelsPerGroup; ++k) {
+          const Tap tp = taps[a * kTile + group + kGroups * k];
+          const bool in_chunk = tp.i0 >= 0 && tp.i0 >= r0 && (last || tp.i0 < r0 + max_rows - 1);
+          if (in_chunk) {
+            const int ra = tp.i0 - r0;
+            const int rb = tp.w != 0.f ? ra + 1 : ra;
+            const float u = 1.0f - tp.w;
+            const float2 gk = g[k];
+            const float2 c0 = cmulc(gk, make_float2(u * tp.c, u * tp.s));  // conj(P0) g
+            const float2 c1 = cmulc(gk, make_float2(tp.w * tp.c, tp.w * tp.s));
+            float2 wa = my_g[ra * kHalf];
+            my_g[ra * kHalf] = make_float2(wa.x + c0.x, wa.y + c0.y);
+            float2 wb = my_g[rb * kHalf];
+            my_g[rb * kHalf] = make_float2(wb.x + c1.x, wb.y + c1.y);  // w == 0: rb == ra, c1 == 0
+            const float2 s0 = win_s[ra * kHalf + t], s1 = win_s[rb * kHalf + t];
+            const float2 d = make_float2(s1.x - s0.x, s1.y - s0.y);
+            const float2 val = make_float2(fmaf(tp.w, d.x, s0.x), fmaf(tp.w, d.y, s0.y));
+            // X = d fs + i omega val, Y = e^{i phi} X, Re(conj(g) Y), scaled by -tau / c
+            const float2 x = make_float2(d.x * fs - omega * val.y, d.y * fs + omega * val.x);
+            const float2 y = make_float2(tp.c * x.x - tp.s * x.y, tp.c * x.y + tp.s * x.x);
+            const float tau = (static_cast<float>(tp.i0) + tp.w) / fs + t0;
+            acc = fmaf(-tau, fmaf(gk.x, y.x, gk.y * y.y), acc);
+          }
+        }
+        dc += static_cast<double>(acc) / p.c;
+        __syncthreads();
+        // Sum the 16 private windows, add to HBM, clear for the next chunk.
+        float2* gdst = gplane0 + a * plane_size + static_cast<size_t>(r0) * p.nt + args.f0;
+        for (int i = tid; i < rows * kHalf; i += kThreads) {
+          const int row = i / kHalf;
+          const int col = i - row * kHalf;
+          float2 sum = make_float2(0.f, 0.f);
+#pragma unroll
+          for (int gi = 0; gi < kGroups; ++gi) {
+            float2* cell = win_g + (gi * max_rows + row) * kHalf + col;
+            const float2 val = *cell;
+            sum.x += val.x;
+            sum.y += val.y;
+            *cell = make_float2(0.f, 0.f);
+          }
+          if (col < args.nf) {
+            if (sum.x != 0.f) unsafeAtomicAdd(&gdst[row * p.nt + col].x, sum.x);
+            if (sum.y != 0.f) unsafeAtomicAdd(&gdst[row * p.nt + col].y, sum.y);
+          }
+        }
+        if (last) break;
+      }
+    }
+  }
+  // dL/dc: block sum in float64, one atomic per workgroup.
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) dc += __shfl_xor(dc, off, kWave);
+  __syncthreads();
+  if (lane == 0) partial[wave] = dc;
+  __syncthreads();
+  if (tid == 0) {
+    double sum = 0.0;
+    for (int i = 0; i < kWaves; ++i) sum += partial[i];
+    unsafeAtomicAdd(args.gc, sum);
+  }
+}
+
+hipError_t launch_adjoint(const AdjointArgs& a, const TileMap& map, long long tiles) {
+  const int fixed = a.p.na * kTile * static_cast<int>(sizeof(Tap)) + kMaxAngles * 16 + kMaxAngles * 8 + kWaves * 8;
+  const int per_row = (1 + kGroups) * kHalf * 8;  // sample row + 16 gradient rows
+  const int max_rows = std::min(kMaxRows, (kLdsBytes - fixed) / per_row);
+  if (max_rows < kMinRows) return hipErrorNotSupported;
+  const int bytes = fixed + max_rows * per_row;
+  const long long grid = (tiles + 7) / 8 * 8;
+  if (grid > 0x7fffffffLL) return hipErrorInvalidValue;
+  static bool attribute_set = false;
+  if (!attribute_set) {
+    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(das_opw_adjoint),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes);
+    if (e != hipSuccess) return e;
+    attribute_set = true;
+  }
+  das_opw_adjoint<<<dim3(static_cast<unsigned>(grid)), kThreads, bytes>>>(a, map, tiles, max_rows);
+  return hipGetLastError();
+}
+
 }  // namespace
 
 hipError_t launch_opw_tile(const EnsembleArgs& p, int channel_storage, int coefficients, int signal_storage,
@@ -629,6 +830,27 @@ hipError_t launch_opw_grid(const EnsembleArgs& p, const float* x, const float* z
   const TileMap map{x, z, y, nx, nz, ny, nby, true};
   return launch_any(p, map, static_cast<long long>(nbx) * nby * nz, channel_storage, coefficients, signal_storage,
                     apertures);
+}
+
+}  // namespace rcb
+
+namespace rcb {
+
+hipError_t launch_opw_adjoint_grid(const EnsembleArgs& p, const float* x, const float* z, const float* y, int nx,
+                                   int nz, int ny, const float2* giq, float2* grc, float2* gcr, double* gc) {
+  if (p.na > kMaxAngles) return hipErrorNotSupported;
+  if (static_cast<long long>(p.ns) * p.nt > 0x7fffffffLL) return hipErrorNotSupported;
+  const int nbx = (nx + kPatch - 1) / kPatch, nby = (ny + kPatch - 1) / kPatch;
+  const TileMap map{x, z, y, nx, nz, ny, nby, true};
+  const long long tiles = static_cast<long long>(nbx) * nby * nz;
+  // Frame chunks of at most 32: each chunk recomputes the taps (nothing is
+  // stored between chunks or passes).
+  for (int f0 = 0; f0 < p.nt; f0 += kHalf) {
+    const AdjointArgs a{p, giq, grc, gcr, gc, f0, std::min(kHalf, p.nt - f0)};
+    const hipError_t e = launch_adjoint(a, map, tiles);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 }  // namespace rcb

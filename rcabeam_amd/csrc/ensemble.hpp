@@ -52,4 +52,10 @@ hipError_t launch_opw_tile(const EnsembleArgs& p, int channel_storage, int coeff
 hipError_t launch_opw_grid(const EnsembleArgs& p, const float* x, const float* z, const float* y, int nx, int nz,
                            int ny, int channel_storage, int coefficients, int signal_storage, bool apertures);
 
+// Adjoint of the grid OPW for complex64 channels (das_tile.hip): accumulates
+// the packed channel gradients grc / gcr and dL/dc (*gc) for the IQ gradient
+// giq (nv, nt); p.rc / p.cr hold the forward's packed channels.
+hipError_t launch_opw_adjoint_grid(const EnsembleArgs& p, const float* x, const float* z, const float* y, int nx,
+                                   int nz, int ny, const float2* giq, float2* grc, float2* gcr, double* gc);
+
 }  // namespace rcb

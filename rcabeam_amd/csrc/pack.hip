@@ -66,6 +66,22 @@ __global__ __launch_bounds__(256) void pack_rows(const float2* __restrict__ src,
   if (range_error && __any(bad) && lane == 0) *range_error = 1;
 }
 
+// Inverse of pack_rows for complex64: one wave per packed (channel, angle,
+// sample) row, written back to (samples, channels, angles, frames).
+__global__ __launch_bounds__(256) void unpack_rows(const float2* __restrict__ src, float2* __restrict__ dst, int ns,
+                                                   int nc, int na, int nt, long long rows) {
+  const long long row = static_cast<long long>(blockIdx.x) * 4 + threadIdx.x / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
+  if (row >= rows) return;
+  const long long s = row % ns;
+  const long long ca = row / ns;
+  const long long a = ca % na;
+  const long long ch = ca / na;
+  const float2* in = src + row * nt;
+  float2* out = dst + ((s * nc + ch) * na + a) * nt;
+  for (int t = lane; t < nt; t += kWave) out[t] = in[t];
+}
+
 template <typename T>
 static rcabeam_status launch_pack(const void* source, void* target, int* range_error, size_t ns, size_t nc,
                                   size_t na, size_t nt) {
@@ -111,4 +127,17 @@ extern "C" rcabeam_status rcabeam_pack_ensemble_bf16_device(const void* source, 
       reinterpret_cast<uintptr_t>(range_error) % 4)
     return RCABEAM_ERROR_ARGUMENT;
   return launch_pack<bf16x2>(source, target, range_error, ns, nc, na, nt);
+}
+
+extern "C" rcabeam_status rcabeam_unpack_ensemble_device(const void* source, void* target, size_t ns, size_t nc,
+                                                         size_t na, size_t nt) {
+  if (!source || !target || !valid_dims(ns, nc, na, nt)) return RCABEAM_ERROR_ARGUMENT;
+  const long long rows = static_cast<long long>(ns) * nc * na;
+  const long long blocks = (rows + 3) / 4;
+  if (blocks > 0x7fffffffLL) return RCABEAM_ERROR_ARGUMENT;
+  unpack_rows<<<dim3(static_cast<unsigned>(blocks)), 256>>>(static_cast<const float2*>(source),
+                                                             static_cast<float2*>(target), static_cast<int>(ns),
+                                                             static_cast<int>(nc), static_cast<int>(na),
+                                                             static_cast<int>(nt), rows);
+  return launch_status();
 }
