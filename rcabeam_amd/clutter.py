@@ -148,7 +148,10 @@ def temporal_basis(iq, *, method: Solver = "gram", precision: str = "complex128"
     ascending = values.cpu().numpy()
     if not np.isfinite(ascending).all():
         raise np.linalg.LinAlgError("Nonfinite Gram matrix; rescale the IQ input")
-    tolerance = float(64 * np.finfo(np.float64).eps * frames * np.max(np.abs(ascending)))
+    # Negative-eigenvalue tolerance of the reference (gpu_clutter.py:171) at the working precision of the
+    # Gram: float64 for the exact Gram, float32 where products or runs are accumulated in float32.
+    eps = np.finfo(np.float64 if exact else np.float32).eps
+    tolerance = float(64 * eps * frames * np.max(np.abs(ascending)))
     minimum = float(ascending[0])
     if diagnostics is not None:
         diagnostics.update(

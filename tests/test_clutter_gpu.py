@@ -162,3 +162,23 @@ def test_invalid_inputs():
     for start, stop in ((True, None), (-1, 4), (0, 5), (3, 2)):
         with pytest.raises(ValueError, match=r"Rank|rank"):
             gc.svd_clutter_filter(iq, rank_start=start, rank_stop=stop)
+
+
+@pytest.mark.parametrize("frames", [8, 24, 64, 200])
+@pytest.mark.parametrize("dynamic_db", [0, 60])
+def test_split_precision_gram_frame_counts(frames, dynamic_db):
+    """complex64 precision (split bfloat16 Gram) across frame counts, with and without a rank-3 clutter
+    component 60 dB above white noise; the leading singular values against the oracle."""
+    gc = _gpu()
+    rng = np.random.default_rng(frames + dynamic_db)
+    rows = 1536
+    x = (rng.normal(size=(rows, frames)) + 1j * rng.normal(size=(rows, frames))) * 10 ** (-dynamic_db / 20)
+    clutter = (rng.normal(size=(rows, 3)) + 1j * rng.normal(size=(rows, 3))) @ np.exp(
+        1j * rng.uniform(0, 2 * np.pi, size=(3, frames)))
+    x = (x + clutter).astype(np.complex64)
+    values, basis = gc.temporal_basis(x, precision="complex64")
+    expected, _ = oc.temporal_basis(x.astype(np.complex128), method="svd")
+    assert np.isfinite(values).all()
+    np.testing.assert_allclose(values[:3], expected[:3], rtol=1e-5)
+    # Gram-level error bound for the rest (see test_split_precision_gram_on_many_rows).
+    np.testing.assert_allclose(values ** 2, expected ** 2, rtol=0, atol=1e-6 * expected[0] ** 2)
