@@ -51,7 +51,8 @@ def parse():
                    help="float32: complex64 channels and IQ; float16 / bfloat16: 16-bit channels and IQ (mixed precision)")
     p.add_argument("--grid", type=int, nargs=3, default=(128, 256, 128))
     p.add_argument("--no-cpu-baseline", action="store_true")
-    p.add_argument("--cpu-voxels", type=int, default=400)
+    p.add_argument("--no-mixed", action="store_true", help="skip the float16 comparison run")
+    p.add_argument("--cpu-voxels", type=int, default=2400)
     p.add_argument("--copy-calibration", action="store_true",
                    help="only run a plain 1 GiB device-to-device copy three times (HBM counter calibration)")
     return p.parse_args()
@@ -255,14 +256,34 @@ def adjoint_timing(reps: int = 3) -> dict:
     }
 
 
-def traffic_from_profiles(name: str):
-    summary = PROFILES / "traffic.json"
+def time_pipeline(pipe, warmup: int, steps: int):
+    """W untimed steps, then K timed steps between synchronizations: (ms per step, mean delay-and-sum ms)."""
+    import torch
+
+    for _ in range(warmup):
+        pipe.step()
+    torch.cuda.synchronize()
+    events = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    log(f"bench: timing {steps} steps ({pipe.storage})")
+    torch.cuda.synchronize()
+    begin = time.perf_counter()
+    for ev in events:
+        pipe.step(ev)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - begin
+    return wall * 1e3 / steps, float(np.mean([a.elapsed_time(b) for a, b in events]))
+
+
+def traffic_from_profiles(precision: str):
+    """HBM bytes per delay-and-sum launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+    (profiles/das_grid_hbm_traffic.json), already scaled by the copy-kernel calibration; None if absent."""
+    summary = PROFILES / "das_grid_hbm_traffic.json"
     if not summary.exists():
         return None
     try:
-        data = json.loads(summary.read_text())
-        return data.get(name)
-    except (OSError, ValueError):
+        entry = json.loads(summary.read_text()).get(precision)
+        return None if entry is None else float(entry["hbm_bytes_per_launch"])
+    except (OSError, ValueError, KeyError, TypeError):
         return None
 
 
@@ -296,19 +317,7 @@ def main() -> int:
     taps = int(counter.item())
     das_flops = FLOPS_PER_TAP * taps * nt
     log(f"bench: {taps} active taps ({taps / nv:.1f} per voxel), warmup {args.warmup}")
-    for _ in range(args.warmup):
-        pipe.step()
-    torch.cuda.synchronize()
-    events = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
-    log(f"bench: timing {args.steps} steps")
-    torch.cuda.synchronize()
-    begin = time.perf_counter()
-    for ev in events:
-        pipe.step(ev)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - begin
-    das_ms = float(np.mean([a.elapsed_time(b) for a, b in events]))
-    ms_per_step = wall * 1e3 / args.steps
+    ms_per_step, das_ms = time_pipeline(pipe, args.warmup, args.steps)
     value = nv * nt / (ms_per_step / 1e3)
     achieved = das_flops / (das_ms / 1e3) / 1e12
     record = {
@@ -339,9 +348,25 @@ def main() -> int:
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP32_PEAK_TFLOPS,
-            "traffic": traffic_from_profiles("das_opw"),
+            "traffic": traffic_from_profiles(storage),
         },
     }
+    if storage == "float32" and not args.no_mixed:
+        # The mixed-precision form on the same scene: FP16 channels and IQ, 16-bit coefficients.
+        del pipe
+        torch.cuda.empty_cache()
+        mixed = Pipeline(scene, rc, cr, "float16", rank_start)
+        m_ms, m_das = time_pipeline(mixed, args.warmup, args.steps)
+        record["mixed"] = {
+            "precision": "float16 channels and IQ, one 16-bit coefficient per sample, float32 accumulation",
+            "value": nv * nt / (m_ms / 1e3),
+            "ms_per_step": m_ms,
+            "das_ms": m_das,
+            "speedup_vs_float32": ms_per_step / m_ms,
+            "traffic": traffic_from_profiles("float16"),
+        }
+        del mixed
+        torch.cuda.empty_cache()
     log("bench: adjoint / forward on the differentiable-OPW workload")
     record["adjoint"] = adjoint_timing()
     if not args.no_cpu_baseline:
