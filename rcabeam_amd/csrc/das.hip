@@ -735,8 +735,15 @@ extern "C" rcabeam_status rcabeam_opw_grid_device(const void* rc, const void* cr
   const int coef = coefficients == RCABEAM_COEF_16BIT && channel_storage != RCABEAM_STORAGE_C64
                        ? kCoefSingle
                        : accurate_coefficients(channel_storage);
-  const hipError_t e = launch_opw_grid(p, x, z, y, static_cast<int>(nx), static_cast<int>(nz), static_cast<int>(ny),
-                                       channel_storage, coef, signal_storage, method == kApertures);
+  hipError_t e = hipErrorNotSupported;
+  // 16-bit channels: MFMA interpolation; complex64 channels, BF16 channels with
+  // float32-accurate coefficients and separate apertures: the VALU tile kernel.
+  if (method == kOpw && channel_storage != RCABEAM_STORAGE_C64)
+    e = launch_opw_grid_mfma(p, x, z, y, static_cast<int>(nx), static_cast<int>(nz), static_cast<int>(ny),
+                             channel_storage, coef, signal_storage);
+  if (e == hipErrorNotSupported)
+    e = launch_opw_grid(p, x, z, y, static_cast<int>(nx), static_cast<int>(nz), static_cast<int>(ny),
+                        channel_storage, coef, signal_storage, method == kApertures);
   if (e == hipErrorNotSupported) return RCABEAM_ERROR_ARGUMENT;
   return to_status(e);
 }

@@ -1,0 +1,644 @@
+// OPW delay-and-sum on MFMA for 16-bit channel storage on a Cartesian grid:
+// the 16-bit path of rcabeam_opw_grid_device (reference
+// src/rcabeam/ensemble.cu:341-454 computes the same sums with per-tap
+// float32 gathers).
+//
+// For one (aperture, channel, angle) a voxel adds P0 S[i0, t] + P1 S[i0+1, t]
+// to its IQ, P0 = (1 - w) e^{i phi}, P1 = w e^{i phi}.  For a group of 16
+// voxels (a 4 x 4 lateral patch at one depth) the rows i0, i0 + 1 of the
+// group lie in a short window (4.1 rows on average for 128+128 elements, 11
+// angles over 4-20 mm), so the update is a small dense product
+//   OUT (16 voxels x 32 frames) += C (16 voxels x 8 rows) S (8 rows x 32 frames)
+// with two non-zero coefficients per voxel.  v_mfma_f32_32x32x16_{f16,bf16}
+// computes it in real form: M = 32 = (voxel, Re / Im of the output),
+// N = 32 frames, K = 16 = (window row, Re / Im of the sample); lanes 0-31
+// hold window rows 0-3 and lanes 32-63 rows 4-7.  Products of 16-bit values
+// are exact in the float32 accumulator:
+//   kCoefSingle : one 16-bit coefficient per sample (1 MFMA per K-step and block)
+//   kCoefSplit  : hi + lo FP16 coefficients, float32-accurate (2 MFMAs)
+//
+// Work decomposition: one 256-thread workgroup (4 wave64s, two per CU) owns
+// an 8 x 8 lateral patch at one depth and all frames (tiles ordered
+// depth-fastest, XCD-aware); wave g owns the 16 voxels of group g and every
+// 32-frame block (NB accumulator blocks of 16 floats per lane).
+//
+// On a grid the delay separates: the receive term of RC depends on (x, z,
+// channel), the transmit term on (y, z, angle), and the other way round for
+// CR.  A tile therefore needs 8 receive terms per channel and 8 transmit terms
+// per angle (float64, split into integer sample, float32 fraction and unit
+// phasor).  Transmit terms are formed once per tile into LDS; each wave forms
+// the 8 receive terms of a channel itself.  Because the first tap row is
+// monotonic in both terms, the row window of a set of voxels is exactly
+// [first row (min receive + min transmit), last row (max receive + max
+// transmit)] clipped to the closed sample window, so the windows of the tile
+// (staging) and of each group (MFMA K-steps) come from min / max over 8 or 4
+// terms, without a tap table.  Each lane combines its A-operand voxel's two
+// terms into its tap.  Channels whose dynamic aperture misses the tile are
+// skipped.
+//
+// Windows of consecutive angles are packed back to back into stages; a
+// window's rows are one contiguous block in the packed (channels, angles,
+// samples, frames) layout and in LDS (row stride nt), so a stage is copied
+// with 1 KiB global_load_lds_dwordx4 instructions regardless of row
+// boundaries, double-buffered across angles and channels, one barrier per
+// stage.  B operands of frames past nt read the next row or zero padding and
+// only feed accumulator columns that are never stored.
+#include "ensemble.hpp"
+#include "rcabeam_mi355x.h"
+#include "tile_staging.hpp"
+
+#include <algorithm>
+
+namespace rcb {
+namespace {
+
+constexpr int kMThreads = 256;
+constexpr int kMWaves = kMThreads / kWave;  // one wave per 16-voxel group
+constexpr int kTileSlots = 64;
+constexpr int kPatchM = 8;
+constexpr int kMaxAnglesM = 16;
+constexpr int kBlockFrames = 32;  // N of one MFMA
+constexpr int kMaxBlocks = 7;     // nt <= 224
+constexpr int kKRows = 8;         // window rows of one K-step
+constexpr int kMaxRowsM = 64;
+constexpr int kMinRowsM = 12;
+constexpr int kLdsBytesM = 80 * 1024;  // two workgroups per CU
+
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+__device__ __forceinline__ f32x16 mfma(u32x4 a, u32x4 b, f32x16 c);
+template <>
+__device__ __forceinline__ f32x16 mfma<__half2>(u32x4 a, u32x4 b, f32x16 c) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8, a), __builtin_bit_cast(f16x8, b), c, 0, 0,
+                                                0);
+}
+template <>
+__device__ __forceinline__ f32x16 mfma<bf16x2>(u32x4 a, u32x4 b, f32x16 c) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0,
+                                                 0, 0);
+}
+
+// Coefficient word of an imaginary output row from that of the real row:
+// (P.re, -P.im) -> (P.im, P.re), i.e. swap the halves, negate the new low half.
+__device__ __forceinline__ uint32_t imag_word(uint32_t w) {
+  return __builtin_amdgcn_alignbit(w, w, 16) ^ 0x00008000u;
+}
+
+// A operand of one lane: coefficient words of its 4 window rows, c0 at row
+// rel (the tap's first row relative to the lane's first row) and c1 at
+// rel + 1; rel outside [-1, 3] gives zeros.
+__device__ __forceinline__ u32x4 place(int rel, uint32_t c0, uint32_t c1) {
+  u32x4 a;
+  a[0] = rel == 0 ? c0 : (rel == -1 ? c1 : 0u);
+  a[1] = rel == 1 ? c0 : (rel == 0 ? c1 : 0u);
+  a[2] = rel == 2 ? c0 : (rel == 1 ? c1 : 0u);
+  a[3] = rel == 3 ? c0 : (rel == 2 ? c1 : 0u);
+  return a;
+}
+
+// A delay term as an ordered key: integer sample, then fraction (a float in
+// [0, 1), whose bits order like the value).  Inactive terms carry the
+// extreme key of the reduction they take part in.
+struct Key {
+  int i;
+  uint32_t f;
+};
+__device__ __forceinline__ bool key_less(Key a, Key b) { return a.i < b.i || (a.i == b.i && a.f < b.f); }
+__device__ __forceinline__ Key key_min(Key a, Key b) { return key_less(b, a) ? b : a; }
+__device__ __forceinline__ Key key_max(Key a, Key b) { return key_less(a, b) ? b : a; }
+constexpr Key kKeyHigh{0x7fffffff, 0u};
+constexpr Key kKeyLow{-0x7fffffff - 1, 0u};
+
+template <int CTRL>
+__device__ __forceinline__ Key dpp_key(Key k) {
+  return Key{__builtin_amdgcn_update_dpp(0, k.i, CTRL, 0xf, 0xf, false),
+             static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(k.f), CTRL, 0xf, 0xf, false))};
+}
+
+// First and last sample rows of the tap of receive key r and transmit key t,
+// computed as combine() computes the tap (integer sum, float32 fraction sum,
+// carry); x = first row, y = last row (first + 1 unless the fraction is 0).
+__device__ __forceinline__ int2 tap_rows(Key r, Key t) {
+  int i = r.i + t.i;
+  float f = __uint_as_float(r.f) + __uint_as_float(t.f);
+  if (f >= 1.0f) {
+    i += 1;
+    f -= 1.0f;
+  }
+  return make_int2(i, f != 0.0f ? i + 1 : i);
+}
+
+// The tap of receive term r and transmit term t exactly as combine() forms
+// it, without branches: first row (-1: no contribution), weight, phasor.
+__device__ __forceinline__ Tap tap_of(const Term& r, const Term& t, int ns) {
+  int i = r.i + t.i;
+  float f = r.f + t.f;
+  const bool carry = f >= 1.0f;
+  i += carry ? 1 : 0;
+  f = carry ? f - 1.0f : f;
+  const bool valid =
+      r.i != kInactive && t.i != kInactive && i >= 0 && (i < ns - 1 || (i == ns - 1 && f == 0.0f));
+  Tap out;
+  out.i0 = valid ? i : -1;
+  out.w = valid ? f : 0.f;
+  out.c = valid ? r.c * t.c - r.s * t.s : 0.f;
+  out.s = valid ? r.c * t.s + r.s * t.c : 0.f;
+  return out;
+}
+
+// Window of the voxels whose receive terms span [rlo, rhi] and transmit terms
+// [tlo, thi]: rows of valid taps (closed window 0 <= s <= ns - 1).  Empty
+// (x > y) if either set is empty.
+__device__ __forceinline__ int2 window(Key rlo, Key rhi, Key tlo, Key thi, int ns) {
+  if (rlo.i == kKeyHigh.i || tlo.i == kKeyHigh.i) return make_int2(kBig, -kBig);
+  const int lo = tap_rows(rlo, tlo).x;
+  const int hi = tap_rows(rhi, thi).y;
+  return make_int2(max(lo, 0), min(hi, ns - 1));
+}
+
+struct MArgs {
+  const void* rc;  // packed (channels, angles, samples, frames) 16-bit RC channels
+  const void* cr;
+  const float* x;
+  const float* z;
+  const float* y;
+  const float* xe;
+  const float* ye;
+  const float* angles;
+  const float* starts;
+  float* pd;
+  void* signal;
+  long long tiles;
+  int nx, nz, ny, nby;
+  int ns, nc, na, nt;
+  int max_rows, out_storage, vec;
+  double c, fs, fd, fn;
+};
+
+struct MLayout {  // byte offsets inside the dynamic LDS allocation
+  int txw, live, psum, win, buffer_words, bytes;
+};
+
+// txt: Term[2][kMaxAnglesM][8] transmit terms (aperture, angle, tile
+// position) at 0; txw: Key[2][kMaxAnglesM][6] their (min, max) over the 8
+// positions, positions 0-3 and positions 4-7; live: one bit per channel
+// (dynamic aperture reaches the tile); psum: float[64]; win: two buffers of
+// max_rows rows x nt words, each followed by zero padding that K-steps at the
+// end of a stage and frame blocks past nt may read.
+__host__ __device__ inline MLayout mlayout(int nc, int nt, int nb, int max_rows) {
+  MLayout l;
+  l.txw = 2 * kMaxAnglesM * 8 * 16;
+  l.live = l.txw + 2 * kMaxAnglesM * 6 * 8;
+  l.psum = l.live + ((2 * nc + 63) / 64) * 8;
+  l.win = (l.psum + kTileSlots * 4 + 15) & ~15;
+  l.buffer_words = ((max_rows + kKRows) * nt + kBlockFrames * nb + 3) & ~3;
+  l.bytes = l.win + 2 * l.buffer_words * 4;
+  return l;
+}
+
+// What a wave knows about one channel: its receive term for the A-operand
+// voxel, the tile window of every angle (lane a) and the group window of
+// every angle (lane a).
+struct Channel {
+  int q, cfg;
+  Term rx;    // A-operand voxel's receive term
+  Windows w;  // tile windows
+  Windows g;  // this wave's group windows
+};
+
+template <typename T, int CP, int NB>
+__global__ __launch_bounds__(kMThreads) __attribute__((amdgpu_waves_per_eu(2))) void das_opw_mfma(MArgs p) {
+  constexpr int CW = CP == kCoefSplit ? 4 : 2;
+  using W = uint32_t;
+  extern __shared__ __align__(16) unsigned char lds[];
+  const MLayout layout = mlayout(p.nc, p.nt, NB, p.max_rows);
+  Term* txt = reinterpret_cast<Term*>(lds);                       // [2][16][8]
+  Key* txw = reinterpret_cast<Key*>(lds + layout.txw);            // [2][16][6]
+  uint64_t* live = reinterpret_cast<uint64_t*>(lds + layout.live);
+  float* psum = reinterpret_cast<float*>(lds + layout.psum);      // [64]
+  W* win = reinterpret_cast<W*>(lds + layout.win);                // [2][buffer_words]
+  const int buffer = layout.buffer_words;
+
+  long long tile;
+  {
+    const long long per_xcd = (p.tiles + 7) / 8;
+    tile = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;  // contiguous tiles per XCD
+  }
+  if (tile >= p.tiles) return;
+  const int tid = threadIdx.x;
+  const int lane = tid & (kWave - 1);
+  const int g = tid / kWave;      // voxel group of this wave
+  const int col = lane & 31;      // frame column of the B operand and the output
+  const int kg = lane >> 5;       // window rows 4 kg .. 4 kg + 3 of a K-step
+  const bool imag_row = (col & 1) != 0;
+  const int nt = p.nt;
+  const int nq = 2 * p.nc;
+  const int max_rows = p.max_rows;
+
+  const int iz = static_cast<int>(tile % p.nz);
+  const long long lateral = tile / p.nz;
+  const int x0 = static_cast<int>(lateral / p.nby) * kPatchM;
+  const int y0 = static_cast<int>(lateral % p.nby) * kPatchM;
+  const double zv = static_cast<double>(p.z[iz]);
+  // A-operand voxel of this lane: group g's (vx, vy) = ((col / 2) / 4, (col / 2) % 4).
+  const int ax = 4 * (g >> 1) + (col >> 3);
+  const int ay = 4 * (g & 1) + ((col >> 1) & 3);
+  // Receive / transmit positions of that voxel for RC (receive along x) and CR.
+  auto in_grid = [&](int cfg_axis_x, int pos) {
+    return cfg_axis_x ? x0 + pos < p.nx : y0 + pos < p.ny;
+  };
+  auto coord = [&](int axis_x, int pos) -> double {
+    return static_cast<double>(axis_x ? p.x[x0 + pos] : p.y[y0 + pos]);
+  };
+
+  // ---- per tile: transmit terms, their min / max, live channels, zeroed buffers
+  {
+    const int cfg = tid >> 7, a = (tid >> 3) & 15, pos = tid & 7;
+    const int axis_x = cfg == 0 ? 0 : 1;  // RC transmits along y, CR along x
+    Term t{kInactive, 0.f, 0.f, 0.f};
+    if (a < p.na && in_grid(axis_x, pos)) {
+      double sn, cs;
+      sincos(static_cast<double>(p.angles[a]), &sn, &cs);
+      t = transmit_term(coord(axis_x, pos), zv, cs, sn, static_cast<double>(p.starts[a]), p.c, p.fs, p.fd);
+    }
+    txt[tid] = t;
+  }
+  for (int q0 = 0; q0 < nq; q0 += kMThreads) {
+    const int q = q0 + tid;
+    bool on = false;
+    if (q < nq) {
+      const int cfg = q >= p.nc ? 1 : 0;
+      const double e = static_cast<double>((cfg == 0 ? p.xe : p.ye)[q - cfg * p.nc]);
+      for (int pos = 0; pos < kPatchM; ++pos) {
+        if (!in_grid(cfg == 0, pos)) continue;
+        const double dv = coord(cfg == 0, pos) - e;
+        on = on || !(p.fn > 0.0) || fabs(dv) <= zv / (2.0 * p.fn);
+      }
+    }
+    const uint64_t mask = __ballot(on);
+    if (lane == 0 && q0 + kWave * g < nq) live[(q0 >> 6) + g] = mask;
+  }
+  {
+    uint4* w4 = reinterpret_cast<uint4*>(win);
+    const int n = 2 * buffer / 4;
+    for (int i = tid; i < n; i += kMThreads) w4[i] = make_uint4(0u, 0u, 0u, 0u);
+  }
+  __syncthreads();
+  if (tid < 2 * kMaxAnglesM) {
+    const Term* t = txt + tid * 8;
+    Key lo[3] = {kKeyHigh, kKeyHigh, kKeyHigh}, hi[3] = {kKeyLow, kKeyLow, kKeyLow};
+    for (int pos = 0; pos < kPatchM; ++pos) {
+      if (t[pos].i == kInactive) continue;
+      const Key k{t[pos].i, __float_as_uint(t[pos].f)};
+      lo[0] = key_min(lo[0], k);
+      hi[0] = key_max(hi[0], k);
+      lo[1 + (pos >> 2)] = key_min(lo[1 + (pos >> 2)], k);
+      hi[1 + (pos >> 2)] = key_max(hi[1 + (pos >> 2)], k);
+    }
+    for (int k = 0; k < 3; ++k) {
+      txw[tid * 6 + 2 * k] = lo[k];
+      txw[tid * 6 + 2 * k + 1] = hi[k];
+    }
+  }
+  __syncthreads();
+
+  // ---- per channel, in every wave (same inputs, same results)
+  auto next_live = [&](int q) {  // first live channel after q, nq if none
+    for (int word = (q + 1) >> 6; word * 64 < nq; ++word) {
+      uint64_t m = live[word];
+      if (word == (q + 1) >> 6) m &= ~0ull << ((q + 1) & 63);
+      m = (static_cast<uint64_t>(uniform(static_cast<int>(m >> 32))) << 32) |
+          static_cast<uint32_t>(uniform(static_cast<int>(m & 0xffffffffu)));
+      if (m) return min(nq, word * 64 + __builtin_ctzll(m));
+    }
+    return nq;
+  };
+  auto setup = [&](int q) {
+    Channel ch;
+    ch.q = q;
+    ch.cfg = q >= p.nc ? 1 : 0;
+    const int axis_x = ch.cfg == 0 ? 1 : 0;  // RC receives along x, CR along y
+    const double e = static_cast<double>((ch.cfg == 0 ? p.xe : p.ye)[q - ch.cfg * p.nc]);
+    // Lane l forms the receive term of tile position l % 8.
+    const int pos = lane & 7;
+    Term r{kInactive, 0.f, 0.f, 0.f};
+    if (in_grid(axis_x, pos)) r = receive_term(coord(axis_x, pos), zv, e, p.c, p.fs, p.fd, p.fn);
+    // This lane's A-operand voxel's receive term, from lane (its position).
+    const int src = ch.cfg == 0 ? ax : ay;
+    ch.rx.i = __shfl(r.i, src, kWave);
+    ch.rx.f = __shfl(r.f, src, kWave);
+    ch.rx.c = __shfl(r.c, src, kWave);
+    ch.rx.s = __shfl(r.s, src, kWave);
+    // Min / max of the receive keys: positions 0-3 and 4-7 (quads), then all 8.
+    const bool on = r.i != kInactive;
+    Key lo = on ? Key{r.i, __float_as_uint(r.f)} : kKeyHigh;
+    Key hi = on ? Key{r.i, __float_as_uint(r.f)} : kKeyLow;
+    lo = key_min(lo, dpp_key<0xB1>(lo));  // quad_perm [1,0,3,2]
+    hi = key_max(hi, dpp_key<0xB1>(hi));
+    lo = key_min(lo, dpp_key<0x4E>(lo));  // quad_perm [2,3,0,1]
+    hi = key_max(hi, dpp_key<0x4E>(hi));
+    const Key hlo[2] = {Key{uniform(lo.i), static_cast<uint32_t>(uniform(static_cast<int>(lo.f)))},
+                        Key{__builtin_amdgcn_readlane(lo.i, 4),
+                            static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(lo.f), 4))}};
+    const Key hhi[2] = {Key{uniform(hi.i), static_cast<uint32_t>(uniform(static_cast<int>(hi.f)))},
+                        Key{__builtin_amdgcn_readlane(hi.i, 4),
+                            static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(hi.f), 4))}};
+    const Key tlo = key_min(hlo[0], hlo[1]), thi = key_max(hhi[0], hhi[1]);
+    // Lane a: windows of angle a for the tile and for this wave's group.
+    const int a = lane < p.na ? lane : 0;
+    const Key* tw = txw + (ch.cfg * kMaxAnglesM + a) * 6;
+    int2 tile_w = window(tlo, thi, tw[0], tw[1], p.ns);
+    // Group g: RC receive half g / 2 (x), transmit half g % 2 (y); CR the other way.
+    const int rh = ch.cfg == 0 ? g >> 1 : g & 1;
+    const int th = ch.cfg == 0 ? g & 1 : g >> 1;
+    int2 group_w = window(hlo[rh], hhi[rh], tw[2 + 2 * th], tw[3 + 2 * th], p.ns);
+    if (lane >= p.na) tile_w = group_w = make_int2(kBig, -kBig);
+    ch.w = Windows{tile_w.x, tile_w.y};
+    ch.g = Windows{group_w.x, group_w.y};
+    return ch;
+  };
+  auto plane_of = [&](const Channel& ch) {
+    const W* data = static_cast<const W*>(ch.cfg == 0 ? p.rc : p.cr);
+    return data + static_cast<size_t>(ch.q - ch.cfg * p.nc) * p.na * p.ns * nt;
+  };
+  const size_t plane_size = static_cast<size_t>(p.ns) * nt;
+
+  // Copy a stage into buf: windows of angles [a0, a1) back to back (or one
+  // chunk of a long window).  A window is contiguous in HBM and in LDS (row
+  // stride nt), so it is copied in 1 KiB global_load_lds_dwordx4
+  // instructions regardless of row boundaries, spread over the waves.
+  auto issue = [&](const Channel& ch, const Stage& s, W* buf) {
+    const W* plane0 = plane_of(ch);
+    int off = 0;
+    for (int a = s.a0; a < s.a1; ++a) {
+      int lo, rows;
+      if (s.r0 >= 0) {
+        lo = s.r0;
+        rows = s.rows;
+      } else {
+        const int2 b = ch.w.at(a);
+        if (b.x > b.y) continue;
+        lo = b.x;
+        rows = b.y - b.x + 1;
+      }
+      const W* src = plane0 + a * plane_size + static_cast<size_t>(lo) * nt;
+      W* dst = buf + off * nt;
+      off += rows;
+      const int words = rows * nt;
+      if (p.vec) {
+        const int chunks = (words + 255) >> 8;  // 256 words = 1 KiB
+        for (int i = g; i < chunks; i += kMWaves) {
+          const int wo = i * 256 + lane * 4;
+          if (wo < words)
+            __builtin_amdgcn_global_load_lds(
+                reinterpret_cast<const void*>(src + wo),
+                reinterpret_cast<__attribute__((address_space(3))) void*>(reinterpret_cast<uintptr_t>(dst + i * 256)),
+                16, 0, 0);
+        }
+      } else {
+        for (int i = tid; i < words; i += kMThreads) dst[i] = src[i];
+      }
+    }
+  };
+
+  f32x16 acc[NB];
+#pragma unroll
+  for (int j = 0; j < NB; ++j)
+#pragma unroll
+    for (int e = 0; e < 16; ++e) acc[j][e] = 0.f;
+
+  // This wave's group, one stage: K-steps of 8 window rows per angle.
+  auto consume = [&](const Channel& ch, const Stage& cur, const W* base) {
+    const int t_lo = cur.r0 >= 0 ? cur.r0 : -kBig;
+    const int t_hi = cur.r0 >= 0 && !cur.last ? cur.r0 + max_rows - 2 : kBig;
+    const int tpos = ch.cfg == 0 ? ay : ax;  // transmit position of the A-operand voxel
+    int off = 0;
+    for (int a = cur.a0; a < cur.a1; ++a) {
+      int first, row_off, lim_lo, lim_hi;
+      if (cur.r0 >= 0) {  // one chunk of a long window
+        first = cur.r0;
+        row_off = 0;
+        lim_lo = cur.r0;
+        lim_hi = cur.r0 + cur.rows - 1;
+      } else {  // whole windows packed back to back
+        const int2 bw = ch.w.at(a);
+        if (bw.x > bw.y) continue;
+        first = bw.x;
+        row_off = off;
+        off += bw.y - bw.x + 1;
+        lim_lo = bw.x;
+        lim_hi = bw.y;
+      }
+      const int2 gw = ch.g.at(a);
+      const int lo = max(gw.x, lim_lo);
+      const int hi = min(gw.y, lim_hi);
+      if (lo > hi) continue;
+      const Tap tp = tap_of(ch.rx, txt[(ch.cfg * kMaxAnglesM + a) * 8 + tpos], p.ns);
+      const bool take = tp.i0 >= 0 && tp.i0 >= t_lo && tp.i0 <= t_hi;
+      const float u = 1.0f - tp.w;
+      const float p0r = u * tp.c, p0i = u * tp.s, p1r = tp.w * tp.c, p1i = tp.w * tp.s;
+      uint32_t c0 = pack2<T>(p0r, -p0i), c1 = pack2<T>(p1r, -p1i), l0 = 0, l1 = 0;
+      if constexpr (CW == 4) {
+        const float2 h0 = widen<T>(c0), h1 = widen<T>(c1);
+        l0 = pack2<T>(p0r - h0.x, -p0i - h0.y);
+        l1 = pack2<T>(p1r - h1.x, -p1i - h1.y);
+        l0 = imag_row ? imag_word(l0) : l0;
+        l1 = imag_row ? imag_word(l1) : l1;
+      }
+      c0 = imag_row ? imag_word(c0) : c0;
+      c1 = imag_row ? imag_word(c1) : c1;
+      for (int r = lo; r <= hi; r += kKRows) {
+        const int rel = take ? tp.i0 - r - 4 * kg : 64;
+        const u32x4 ahi = place(rel, c0, c1);
+        const W* b0 = base + (r - first + row_off + 4 * kg) * nt + col;
+        u32x4 bv[NB];
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+          bv[j][0] = b0[kBlockFrames * j];
+          bv[j][1] = b0[nt + kBlockFrames * j];
+          bv[j][2] = b0[2 * nt + kBlockFrames * j];
+          bv[j][3] = b0[3 * nt + kBlockFrames * j];
+        }
+#pragma unroll
+        for (int j = 0; j < NB; ++j) acc[j] = mfma<T>(ahi, bv[j], acc[j]);
+        if constexpr (CW == 4) {
+          const u32x4 alo = place(rel, l0, l1);
+#pragma unroll
+          for (int j = 0; j < NB; ++j) acc[j] = mfma<T>(alo, bv[j], acc[j]);
+        }
+      }
+    }
+  };
+
+  // ---- stages of all live channels, one barrier per stage
+  Stage cur{0, 0, -1, 0, true, false};
+  Channel ch;
+  ch.q = -1;
+  for (int q = next_live(-1); q < nq; q = next_live(q)) {
+    ch = setup(q);
+    cur = plan_stage(ch.w, 0, p.na, max_rows);
+    if (cur.valid) break;
+  }
+  int b = 0;
+  if (cur.valid) issue(ch, cur, win);
+  __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+  __syncthreads();
+  while (cur.valid) {
+    Stage nxt = next_stage(cur, ch.w, p.na, max_rows);
+    Channel nch = ch;
+    bool crossing = false;
+    if (!nxt.valid) {
+      for (int q = next_live(ch.q); q < nq; q = next_live(q)) {
+        nch = setup(q);
+        nxt = plan_stage(nch.w, 0, p.na, max_rows);
+        if (nxt.valid) break;
+      }
+      crossing = nxt.valid;
+    }
+    if (nxt.valid) issue(crossing ? nch : ch, nxt, win + (b ^ 1) * buffer);
+    consume(ch, cur, win + b * buffer);
+    __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+    __syncthreads();
+    b ^= 1;
+    if (crossing) ch = nch;
+    cur = nxt;
+  }
+
+  // ---- epilogue.  acc[j][e] is output row 8 (e / 4) + 4 kg + e % 4 = (voxel
+  // 4 (e / 4) + 2 kg + (e / 2) % 2 of the group, Re / Im = e % 2) at frame
+  // 32 j + col; group voxel u sits at (4 (g / 2) + u / 4, 4 (g % 2) + u % 4).
+  auto voxel_id = [&](int u) -> long long {
+    const int ix = x0 + 4 * (g >> 1) + (u >> 2), iy = y0 + 4 * (g & 1) + (u & 3);
+    if (ix >= p.nx || iy >= p.ny) return -1;
+    return (static_cast<long long>(ix) * p.nz + iz) * p.ny + iy;
+  };
+  long long vid[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) vid[k] = voxel_id(4 * (k >> 1) + 2 * kg + (k & 1));
+  float pw[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) pw[k] = 0.f;
+#pragma unroll
+  for (int j = 0; j < NB; ++j) {
+    const int t = kBlockFrames * j + col;
+    if (t >= nt) continue;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const float2 s = make_float2(acc[j][2 * k], acc[j][2 * k + 1]);
+      pw[k] = fmaf(s.x, s.x, fmaf(s.y, s.y, pw[k]));
+      if (p.signal && vid[k] >= 0) {
+        const long long index = vid[k] * nt + t;
+        if (p.out_storage == RCABEAM_STORAGE_FP16)
+          store_sample(reinterpret_cast<__half2*>(p.signal) + index, s);
+        else if (p.out_storage == RCABEAM_STORAGE_BF16)
+          store_sample(reinterpret_cast<bf16x2*>(p.signal) + index, s);
+        else
+          store_sample(reinterpret_cast<float2*>(p.signal) + index, s);
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 8; ++k)
+#pragma unroll
+    for (int off = 1; off < 32; off <<= 1) pw[k] += __shfl_xor(pw[k], off, kWave);
+  if (col == 0) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) psum[16 * g + 4 * (k >> 1) + 2 * kg + (k & 1)] = pw[k];
+  }
+  __syncthreads();
+  if (tid < kTileSlots) {
+    const int gg = tid >> 4, u = tid & 15;
+    const int ix = x0 + 4 * (gg >> 1) + (u >> 2), iy = y0 + 4 * (gg & 1) + (u & 3);
+    if (ix < p.nx && iy < p.ny) {
+      const long long v = (static_cast<long long>(ix) * p.nz + iz) * p.ny + iy;
+      p.pd[2 * v] = psum[tid] / static_cast<float>(nt);
+      p.pd[2 * v + 1] = 0.f;
+    }
+  }
+}
+
+template <typename T, int CP, int NB>
+hipError_t launch_shape(MArgs a) {
+  const int fixed = mlayout(a.nc, a.nt, NB, 0).bytes;
+  a.max_rows = std::min(kMaxRowsM, (kLdsBytesM - fixed) / (2 * a.nt * 4));
+  if (a.max_rows < kMinRowsM) return hipErrorNotSupported;
+  const int bytes = mlayout(a.nc, a.nt, NB, a.max_rows).bytes;
+  if (bytes > kLdsBytesM) return hipErrorNotSupported;
+  const long long grid = (a.tiles + 7) / 8 * 8;
+  if (grid > 0x7fffffffLL) return hipErrorInvalidValue;
+  a.vec = reinterpret_cast<uintptr_t>(a.rc) % 16 == 0 && reinterpret_cast<uintptr_t>(a.cr) % 16 == 0 && a.nt % 4 == 0;
+  auto kernel = das_opw_mfma<T, CP, NB>;
+  static bool attribute_set = false;  // one opt-in per instantiation (beyond 64 KiB of dynamic LDS)
+  if (!attribute_set) {
+    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytesM);
+    if (e != hipSuccess) return e;
+    attribute_set = true;
+  }
+  kernel<<<dim3(static_cast<unsigned>(grid)), kMThreads, bytes>>>(a);
+  return hipGetLastError();
+}
+
+template <typename T, int CP>
+hipError_t launch_blocks(const MArgs& a) {
+  const int blocks = (a.nt + kBlockFrames - 1) / kBlockFrames;
+  if (blocks <= 1) return launch_shape<T, CP, 1>(a);
+  if (blocks <= 2) return launch_shape<T, CP, 2>(a);
+  if (blocks <= 4) return launch_shape<T, CP, 4>(a);
+  return launch_shape<T, CP, kMaxBlocks>(a);
+}
+
+}  // namespace
+
+hipError_t launch_opw_grid_mfma(const EnsembleArgs& p, const float* x, const float* z, const float* y, int nx, int nz,
+                                int ny, int channel_storage, int coefficients, int signal_storage) {
+  if (p.na > kMaxAnglesM || p.nt > kBlockFrames * kMaxBlocks) return hipErrorNotSupported;
+  if (static_cast<long long>(p.ns) * p.nt > 0x7fffffffLL) return hipErrorNotSupported;
+  if (signal_storage < RCABEAM_STORAGE_C64 || signal_storage > RCABEAM_STORAGE_BF16) return hipErrorInvalidValue;
+  const int nbx = (nx + kPatchM - 1) / kPatchM, nby = (ny + kPatchM - 1) / kPatchM;
+  MArgs a;
+  a.rc = p.rc;
+  a.cr = p.cr;
+  a.x = x;
+  a.z = z;
+  a.y = y;
+  a.xe = p.xe;
+  a.ye = p.ye;
+  a.angles = p.angles;
+  a.starts = p.starts;
+  a.pd = p.pd;
+  a.signal = p.signal;
+  a.tiles = static_cast<long long>(nbx) * nby * nz;
+  a.nx = nx;
+  a.nz = nz;
+  a.ny = ny;
+  a.nby = nby;
+  a.ns = p.ns;
+  a.nc = p.nc;
+  a.na = p.na;
+  a.nt = p.nt;
+  a.max_rows = 0;
+  a.out_storage = signal_storage;
+  a.vec = 0;
+  a.c = p.c;
+  a.fs = p.fs;
+  a.fd = p.fd;
+  a.fn = p.fn;
+  switch (channel_storage) {
+    case RCABEAM_STORAGE_FP16:
+      if (coefficients == kCoefSingle) return launch_blocks<__half2, kCoefSingle>(a);
+      if (coefficients == kCoefSplit) return launch_blocks<__half2, kCoefSplit>(a);
+      return hipErrorNotSupported;
+    case RCABEAM_STORAGE_BF16:
+      if (coefficients == kCoefSingle) return launch_blocks<bf16x2, kCoefSingle>(a);
+      return hipErrorNotSupported;
+    default:
+      return hipErrorNotSupported;
+  }
+}
+
+}  // namespace rcb

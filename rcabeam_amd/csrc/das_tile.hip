@@ -34,6 +34,7 @@
 // multiplies two 16-bit pairs exactly and accumulates in float32.
 #include "ensemble.hpp"
 #include "rcabeam_mi355x.h"
+#include "tile_staging.hpp"
 
 #include <algorithm>
 
@@ -52,55 +53,6 @@ constexpr int kMinRows = 4;
 constexpr int kMaxFramesPerLane = 26;
 // Dynamic LDS per workgroup: two workgroups share a CU's 160 KiB.
 constexpr int kLdsBytes = 80 * 1024;
-constexpr int kBig = 0x7fffffff;
-
-typedef _Float16 half2v __attribute__((ext_vector_type(2)));
-typedef __bf16 bf162v __attribute__((ext_vector_type(2)));
-
-template <typename T>
-struct Word;  // one complex sample as stored in LDS
-template <>
-struct Word<float2> {
-  using type = float2;
-};
-template <>
-struct Word<__half2> {
-  using type = uint32_t;
-};
-template <>
-struct Word<bf16x2> {
-  using type = uint32_t;
-};
-
-template <typename T>
-__device__ __forceinline__ float2 widen(typename Word<T>::type w);
-template <>
-__device__ __forceinline__ float2 widen<float2>(float2 w) {
-  return w;
-}
-template <>
-__device__ __forceinline__ float2 widen<__half2>(uint32_t w) {
-  const half2v h = __builtin_bit_cast(half2v, w);
-  return make_float2(static_cast<float>(h.x), static_cast<float>(h.y));
-}
-template <>
-__device__ __forceinline__ float2 widen<bf16x2>(uint32_t w) {
-  return make_float2(__uint_as_float(w << 16), __uint_as_float(w & 0xffff0000u));
-}
-
-template <typename T>
-__device__ __forceinline__ uint32_t pack2(float lo, float hi);
-template <>
-__device__ __forceinline__ uint32_t pack2<__half2>(float lo, float hi) {
-  half2v h;
-  h.x = static_cast<_Float16>(lo);
-  h.y = static_cast<_Float16>(hi);
-  return __builtin_bit_cast(uint32_t, h);
-}
-template <>
-__device__ __forceinline__ uint32_t pack2<bf16x2>(float lo, float hi) {
-  return bf16_bits_rne(lo) | (bf16_bits_rne(hi) << 16);
-}
 
 template <typename T>
 __device__ __forceinline__ float dot2(uint32_t a, uint32_t b, float c);
@@ -178,17 +130,6 @@ __device__ __forceinline__ void accumulate(const typename Word<T>::type* pa, con
   }
 }
 
-__device__ __forceinline__ int wave_min(int x) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) x = min(x, __shfl_xor(x, off, kWave));
-  return x;
-}
-__device__ __forceinline__ int wave_max(int x) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) x = max(x, __shfl_xor(x, off, kWave));
-  return x;
-}
-
 // Row stride of an LDS window (in samples): at least fb and such that the 8
 // rows a wave reads in one instruction fall on distinct LDS bank groups.
 constexpr int window_stride(int fb, int word_bytes) {
@@ -248,111 +189,6 @@ __device__ __forceinline__ long long slot_voxel(const EnsembleArgs& p, const Til
   return v;
 }
 
-// Row windows of one channel held in registers: lane a (< na) holds angle
-// a's [first row, last row] (empty: first > last); any lane reads them with
-// v_readlane, so stage planning costs no LDS round trips.
-struct Windows {
-  int lo, hi;
-  __device__ __forceinline__ int2 at(int a) const {
-    return make_int2(__builtin_amdgcn_readlane(lo, a), __builtin_amdgcn_readlane(hi, a));
-  }
-};
-
-__device__ __forceinline__ Windows load_windows(const int2* bounds, int na) {
-  const int lane = static_cast<int>(threadIdx.x % kWave);
-  const int2 b = lane < na ? bounds[lane] : make_int2(kBig, -kBig);
-  return Windows{b.x, b.y};
-}
-
-// One stage: angles [a0, a1) whose windows are packed back to back, or one
-// chunk (rows [r0, r0 + rows)) of a window longer than max_rows (r0 >= 0).
-struct Stage {
-  int a0, a1, r0, rows;
-  bool last, valid;
-};
-
-// The stage that starts at the first live angle at or after a.
-__device__ __forceinline__ Stage plan_stage(const Windows& w, int a, int na, int max_rows) {
-  for (; a < na; ++a) {
-    const int2 b = w.at(a);
-    if (b.x > b.y) continue;
-    const int rows = b.y - b.x + 1;
-    if (rows > max_rows) return Stage{a, a + 1, b.x, max_rows, false, true};
-    int total = rows, a1 = a + 1;
-    for (; a1 < na; ++a1) {
-      const int2 b1 = w.at(a1);
-      if (b1.x > b1.y) continue;
-      const int r1 = b1.y - b1.x + 1;
-      if (total + r1 > max_rows) break;
-      total += r1;
-    }
-    return Stage{a, a1, -1, total, true, true};
-  }
-  return Stage{0, 0, -1, 0, true, false};
-}
-
-__device__ __forceinline__ Stage next_stage(const Stage& s, const Windows& w, int na, int max_rows) {
-  if (s.r0 >= 0 && !s.last) {
-    const int hi = w.at(s.a0).y;
-    const int r0 = s.r0 + max_rows - 1;
-    const bool last = r0 + max_rows - 1 >= hi;
-    return Stage{s.a0, s.a1, r0, last ? hi + 1 - r0 : max_rows, last, true};
-  }
-  return plan_stage(w, s.a1, na, max_rows);
-}
-
-// Copy rows [r_first, r_first + rows) x frames [0, nt) of one plane into LDS
-// rows starting at dst.  vec: wave w copies rows w, w + 8, ...; each lane
-// moves one 16-byte unit of the row straight from HBM into LDS
-// (global_load_lds_dwordx4, asynchronous: the caller waits on vmcnt before
-// the barrier), so one instruction moves 1 KiB contiguous in both memories.
-// Otherwise a synchronous element copy.
-template <typename W, int STRIDE>
-__device__ __forceinline__ void copy_rows(const W* __restrict__ plane, int nt, int r_first, int rows, W* dst,
-                                          bool vec) {
-  const W* __restrict__ src = plane + static_cast<size_t>(r_first) * nt;
-  if (vec) {
-    constexpr int U = 16 / static_cast<int>(sizeof(W));
-    const int units = nt / U;  // 16-byte units of one row (nt is a multiple of U)
-    const int lane = static_cast<int>(threadIdx.x % kWave);
-    for (int row = static_cast<int>(threadIdx.x / kWave); row < rows; row += kWaves) {
-      const W* s_row = src + row * nt;
-      W* d_row = dst + row * STRIDE;
-      for (int c0 = 0; c0 < units; c0 += kWave)
-        if (c0 + lane < units)
-          __builtin_amdgcn_global_load_lds(
-              reinterpret_cast<const void*>(s_row + (c0 + lane) * U),
-              reinterpret_cast<__attribute__((address_space(3))) void*>(reinterpret_cast<uintptr_t>(d_row + c0 * U)),
-              16, 0, 0);
-    }
-  } else {
-    const int n = rows * STRIDE;
-    for (int q = threadIdx.x; q < n; q += kThreads) {
-      const int row = q / STRIDE;
-      const int col = q - row * STRIDE;
-      if (col < nt) dst[q] = src[row * nt + col];
-    }
-  }
-}
-
-template <typename W, int STRIDE>
-__device__ __forceinline__ void issue_stage(const W* plane0, size_t plane_size, int nt, const Stage& s,
-                                            const Windows& w, W* buf, bool vec) {
-  if (s.r0 >= 0) {
-    copy_rows<W, STRIDE>(plane0 + s.a0 * plane_size, nt, s.r0, s.rows, buf, vec);
-    return;
-  }
-  int off = 0;
-  for (int a = s.a0; a < s.a1; ++a) {
-    const int2 b = w.at(a);
-    if (b.x > b.y) continue;
-    copy_rows<W, STRIDE>(plane0 + a * plane_size, nt, b.x, b.y - b.x + 1, buf + off * STRIDE, vec);
-    off += b.y - b.x + 1;
-  }
-}
-
-// s_waitcnt vmcnt(0) (expcnt and lgkmcnt unconstrained) in the gfx9 encoding.
-constexpr int kWaitVmcnt0 = 0x0F70;
 
 template <typename O>
 __device__ __forceinline__ void store_out(void* signal, long long index, float2 v) {
@@ -455,7 +291,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
     Stage cur = pending;
     if (!cur.valid) {
       cur = plan_stage(w, 0, p.na, max_rows);
-      if (cur.valid) issue_stage<W, STRIDE>(plane0, plane_size, p.nt, cur, w, win + b * buffer, vec);
+      if (cur.valid) issue_stage<W, STRIDE, kThreads>(plane0, plane_size, p.nt, cur, w, win + b * buffer, vec);
     }
     __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
     __syncthreads();  // stage landed; channel q+1's taps and windows visible
@@ -465,11 +301,11 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
       Stage nxt = next_stage(cur, w, p.na, max_rows);
       bool crossing = false;
       if (nxt.valid) {
-        issue_stage<W, STRIDE>(plane0, plane_size, p.nt, nxt, w, win + (b ^ 1) * buffer, vec);
+        issue_stage<W, STRIDE, kThreads>(plane0, plane_size, p.nt, nxt, w, win + (b ^ 1) * buffer, vec);
       } else if (q + 1 < nq) {
         const Windows w1 = load_windows(bounds + ((q + 1) & 1) * kMaxAngles, p.na);
         nxt = plan_stage(w1, 0, p.na, max_rows);
-        if (nxt.valid) issue_stage<W, STRIDE>(plane_of(q + 1), plane_size, p.nt, nxt, w1, win + (b ^ 1) * buffer, vec);
+        if (nxt.valid) issue_stage<W, STRIDE, kThreads>(plane_of(q + 1), plane_size, p.nt, nxt, w1, win + (b ^ 1) * buffer, vec);
         crossing = true;
       }
       const W* base = win + b * buffer + fl;

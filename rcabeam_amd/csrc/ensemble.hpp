@@ -52,6 +52,13 @@ hipError_t launch_opw_tile(const EnsembleArgs& p, int channel_storage, int coeff
 hipError_t launch_opw_grid(const EnsembleArgs& p, const float* x, const float* z, const float* y, int nx, int nz,
                            int ny, int channel_storage, int coefficients, int signal_storage, bool apertures);
 
+// OPW on a Cartesian grid on MFMA for FP16 / BF16 channels (das_mfma.hip):
+// kCoefSingle (FP16 or BF16) or kCoefSplit (FP16).  Same outputs as
+// launch_opw_grid up to float32 summation order.  hipErrorNotSupported for
+// shapes or storages it does not handle (na > 16, nt > 224, complex64).
+hipError_t launch_opw_grid_mfma(const EnsembleArgs& p, const float* x, const float* z, const float* y, int nx, int nz,
+                                int ny, int channel_storage, int coefficients, int signal_storage);
+
 // Adjoint of the grid OPW for complex64 channels (das_tile.hip): accumulates
 // the packed channel gradients grc / gcr and dL/dc (*gc) for the IQ gradient
 // giq (nv, nt); p.rc / p.cr hold the forward's packed channels.
