@@ -11,9 +11,14 @@ tube segments, -60 dB noise.
 One step = pack RC and CR -> frame-batched OPW delay-and-sum (complex64 IQ in
 HBM) -> scaled Gram on MFMA -> float64 eigendecomposition -> filtered power,
 starting from channel data resident in HBM.  `--precision float16` (or
-`bfloat16`) is the mixed-precision form: channels and beamformed IQ stored as
-16-bit, delay-and-sum with one 16-bit coefficient per sample (exact products,
-float32 accumulation), Gram formed exactly from the 16-bit IQ.
+`bfloat16`) rounds channels and beamformed IQ to 16 bits, delay-and-sum on
+MFMA with one 16-bit coefficient per sample (exact products, float32
+accumulation), Gram formed exactly from the 16-bit IQ.  `float16-lowrank`
+(or `bfloat16-lowrank`) is the mixed-precision form reported beside the
+float32 one: the channel Gram and a float64 eigendecomposition give the 4
+strongest temporal modes of the channels, which are coded as 16-bit hi + lo
+coefficients while only the residual is rounded to 16 bits (rcabeam_amd.lowrank),
+MFMA delay-and-sum with 16-bit coefficients, complex64 IQ, exact float64 Gram.
 
 Prints one JSON line.  `value` is voxel-frames/s of the whole step.
 """
@@ -47,11 +52,14 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=3)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--precision", choices=("float32", "float16", "bfloat16"), default="float32",
-                   help="float32: complex64 channels and IQ; float16 / bfloat16: 16-bit channels and IQ (mixed precision)")
+    p.add_argument("--precision", choices=("float32", "float16", "bfloat16", "float16-lowrank", "bfloat16-lowrank"),
+                   default="float32",
+                   help="float32: complex64 channels and IQ; float16 / bfloat16: 16-bit channels and IQ; "
+                        "*-lowrank: low-rank + 16-bit residual channels, complex64 IQ (the mixed-precision form)")
+    p.add_argument("--lowrank-modes", type=int, default=4, help="temporal modes k of the low-rank channel coding")
     p.add_argument("--grid", type=int, nargs=3, default=(128, 256, 128))
     p.add_argument("--no-cpu-baseline", action="store_true")
-    p.add_argument("--no-mixed", action="store_true", help="skip the float16 comparison run")
+    p.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision comparison runs")
     p.add_argument("--cpu-voxels", type=int, default=2400)
     p.add_argument("--copy-calibration", action="store_true",
                    help="only run a plain 1 GiB device-to-device copy three times (HBM counter calibration)")
@@ -75,7 +83,7 @@ def copy_calibration() -> int:
 class Pipeline:
     """Preallocated buffers and the one-step enqueue of the filtered-OPW pipeline."""
 
-    def __init__(self, scene, rc, cr, storage: str, rank_start: int):
+    def __init__(self, scene, rc, cr, storage: str, rank_start: int, lowrank_modes: int = 4):
         import torch
 
         from rcabeam_amd import _device as dev
@@ -86,20 +94,36 @@ class Pipeline:
         self.dev = dev
         self.lib = _lib.load()
         self.rc, self.cr = rc, cr
+        self.lowrank = storage.endswith("-lowrank")
+        self.channel_storage = storage.replace("-lowrank", "")
         self.storage = storage
-        self.code = _lib.STORAGE[storage]
+        # beamformed IQ storage: complex64 for float32 and the low-rank forms
+        iq_storage = "float32" if (self.lowrank or storage == "float32") else storage
+        self.code = _lib.STORAGE[iq_storage]
         self.rank_start = rank_start
         self.g = _Geometry(scene.angles, scene.t_start, scene.grid, scene.geom)
         self.ns, self.nc, self.na, self.nt = rc.shape
         nv = self.g.nv
+        self.k = lowrank_modes if self.lowrank else 0
+        self.ntc = self.nt + 2 * self.k  # frames of the packed channels
         words = torch.complex64 if storage == "float32" else torch.int32
-        self.packed = [dev.empty((self.nc, self.na, self.ns, self.nt), words) for _ in range(2)]
+        self.packed = [dev.empty((self.nc, self.na, self.ns, self.ntc), words) for _ in range(2)]
         self.range_error = dev.empty((1,), torch.int32)
         self.pd = dev.empty((nv, 2), torch.float32)
-        self.signal = dev.empty((nv, self.nt), torch.complex64 if storage == "float32" else torch.int32)
+        self.signal = dev.empty((nv, self.nt), torch.complex64 if self.code == 0 else torch.int32)
+        if self.lowrank:
+            rows = self.ns * self.nc * self.na
+            self.c_rows = rows
+            self.c_work_bytes = int(self.lib.rcabeam_gram_f64_workspace_bytes(rows, self.nt))
+            self.c_work = dev.empty((self.c_work_bytes // 8,), torch.float64)
+            self.c_real = dev.empty((int(self.lib.rcabeam_gram_real_elements(self.nt)),), torch.float64)
+            self.c_gram = dev.empty((self.nt, self.nt), torch.complex128)
+            self.c_values = dev.empty((self.nt,), torch.float64)
+            self.basis = dev.empty((self.k, self.nt), torch.complex128)
+            self.coded = dev.empty((nv, self.ntc), torch.complex64)
         self.stats = dev.empty((3,), torch.int32)
         self.real = dev.empty((int(self.lib.rcabeam_gram_real_elements(self.nt)),), torch.float64)
-        if storage == "float32":
+        if self.code == 0:
             self.work_bytes = int(self.lib.rcabeam_gram_f64_workspace_bytes(nv, self.nt))
             self.work = dev.empty((self.work_bytes // 8,), torch.float64)
         else:
@@ -119,29 +143,53 @@ class Pipeline:
 
     def step(self, das_events=None):
         """Enqueue one full step on the null stream; optional events bracket the DAS launch."""
+        from rcabeam_amd import _lib
+
         dev, nv = self.dev, self.g.nv
         c, fs, fd, fn = self.g.constants
         shape = (self.ns, self.nc, self.na, self.nt)
-        for raw, packed in zip((self.rc, self.cr), self.packed):
-            if self.storage == "float32":
-                self._call("rcabeam_pack_ensemble_device", dev.ptr(raw), dev.ptr(packed), *shape)
-            else:
-                pack = "rcabeam_pack_ensemble_fp16_device" if self.storage == "float16" else "rcabeam_pack_ensemble_bf16_device"
-                self._call(pack, dev.ptr(raw), dev.ptr(packed), dev.ptr(self.range_error), *shape)
+        channel_code = _lib.STORAGE[self.channel_storage]
+        if self.lowrank:
+            # Channel basis: exact float64 Gram of both apertures' Casorati matrices, float64 EVD, the k
+            # strongest modes (descending; rows of the row-major view of the column-major eigenvectors).
+            for i, raw in enumerate((self.rc, self.cr)):
+                self._call("rcabeam_temporal_gram_f64_device", dev.ptr(raw), self.c_rows, self.nt,
+                           dev.ptr(self.c_real), dev.ptr(self.c_work), self.c_work_bytes, i)
+            self._call("rcabeam_gram_hermitian_device", dev.ptr(self.c_real), self.nt, 1.0 / (2 * self.c_rows),
+                       dev.ptr(self.c_gram))
+            self._call("rcabeam_eigh_device", dev.ptr(self.c_gram), dev.ptr(self.c_values), dev.ptr(self.e_work),
+                       dev.ptr(self.info), self.nt)
+            self.basis.copy_(self.c_gram[self.nt - self.k:].flip(0))
+            for raw, packed in zip((self.rc, self.cr), self.packed):
+                self._call("rcabeam_pack_ensemble_lowrank_device", dev.ptr(raw), dev.ptr(packed), channel_code,
+                           dev.ptr(self.basis), self.k, dev.ptr(self.range_error), *shape)
+        else:
+            for raw, packed in zip((self.rc, self.cr), self.packed):
+                if self.storage == "float32":
+                    self._call("rcabeam_pack_ensemble_device", dev.ptr(raw), dev.ptr(packed), *shape)
+                else:
+                    pack = ("rcabeam_pack_ensemble_fp16_device" if self.storage == "float16"
+                            else "rcabeam_pack_ensemble_bf16_device")
+                    self._call(pack, dev.ptr(raw), dev.ptr(packed), dev.ptr(self.range_error), *shape)
         if das_events is not None:
             das_events[0].record()
-        # OPW on the Cartesian grid (8 x 8 lateral tiles).  float32: complex64 channels and IQ;
-        # 16-bit: 16-bit channels and IQ, one 16-bit coefficient per sample (dot2, float32 accumulation).
+        # OPW on the Cartesian grid (8 x 8 lateral tiles).  float32: complex64 channels and IQ, VALU;
+        # 16-bit: MFMA with one 16-bit coefficient per sample (exact products, float32 accumulation).
         nx, nz, ny = self.g.shape
         coef = 0 if self.storage == "float32" else _lib_coef_16bit()
-        self._call("rcabeam_opw_grid_device", dev.ptr(self.packed[0]), dev.ptr(self.packed[1]), self.code, coef,
+        out = self.coded if self.lowrank else self.signal
+        self._call("rcabeam_opw_grid_device", dev.ptr(self.packed[0]), dev.ptr(self.packed[1]), channel_code, coef,
                    dev.ptr(self.g.axes[0]), dev.ptr(self.g.axes[1]), dev.ptr(self.g.axes[2]), nx, nz, ny,
                    dev.ptr(self.g.xe), dev.ptr(self.g.ye), dev.ptr(self.g.angles), dev.ptr(self.g.starts),
-                   dev.ptr(self.pd), dev.ptr(self.signal), self.code, 0, *shape, c, fs, fd, fn)
+                   dev.ptr(self.pd), dev.ptr(out), 0 if self.lowrank else self.code, 0, self.ns, self.nc, self.na,
+                   self.ntc, c, fs, fd, fn)
         if das_events is not None:
             das_events[1].record()
+        if self.lowrank:
+            self._call("rcabeam_lowrank_expand_device", dev.ptr(self.coded), dev.ptr(self.basis), self.k,
+                       dev.ptr(self.signal), 0, dev.ptr(self.pd), nv, self.nt)
         self._call("rcabeam_iq_absmax_device", dev.ptr(self.signal), self.code, nv * self.nt, dev.ptr(self.stats), 0)
-        if self.storage == "float32":
+        if self.code == 0:
             self._call("rcabeam_temporal_gram_f64_device", dev.ptr(self.signal), nv, self.nt, dev.ptr(self.real),
                        dev.ptr(self.work), self.work_bytes, 0)
         else:
@@ -306,7 +354,7 @@ def main() -> int:
     log(f"bench: generating scene grid={tuple(args.grid)}")
     scene, rc, cr = mixture_scene(grid_shape=tuple(args.grid))
     storage = args.precision
-    pipe = Pipeline(scene, rc, cr, storage, rank_start)
+    pipe = Pipeline(scene, rc, cr, storage, rank_start, args.lowrank_modes)
     nv, nt = pipe.g.nv, pipe.nt
     counter = dev.empty((1,), torch.int64)
     counter.zero_()
@@ -331,7 +379,9 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"float32": "f32", "float16": "f16-storage/f32-accumulate", "bfloat16": "bf16-storage/f32-accumulate"}[storage],
+        "dtype": {"float32": "f32", "float16": "f16-storage/f32-accumulate", "bfloat16": "bf16-storage/f32-accumulate",
+                  "float16-lowrank": "f16-residual-storage/f32-accumulate",
+                  "bfloat16-lowrank": "bf16-residual-storage/f32-accumulate"}[storage],
         "data": "synthetic (GPU-simulated tissue + blood + noise scene, seed 42)",
         "config": {
             "workload": "filtered OPW PD: 128+128 el, IQ 1024x128x11x200 per aperture, grid "
@@ -352,21 +402,40 @@ def main() -> int:
         },
     }
     if storage == "float32" and not args.no_mixed:
-        # The mixed-precision form on the same scene: FP16 channels and IQ, 16-bit coefficients.
-        del pipe
-        torch.cuda.empty_cache()
-        mixed = Pipeline(scene, rc, cr, "float16", rank_start)
-        m_ms, m_das = time_pipeline(mixed, args.warmup, args.steps)
-        record["mixed"] = {
-            "precision": "float16 channels and IQ, one 16-bit coefficient per sample, float32 accumulation",
-            "value": nv * nt / (m_ms / 1e3),
-            "ms_per_step": m_ms,
-            "das_ms": m_das,
-            "speedup_vs_float32": ms_per_step / m_ms,
-            "traffic": traffic_from_profiles("float16"),
-        }
-        del mixed
-        torch.cuda.empty_cache()
+        # Mixed-precision forms on the same scene, with their full-size difference from the float32 pipeline
+        # (whose own error against the float64 oracle is ~1e-5 on filtered power; tests/test_mixed_precision_gpu.py).
+        reference = {"iq": pipe.signal, "pd": pipe.pd[:, 0].clone(), "power": pipe.power.clone()}
+
+        def rel(a, b):
+            a = a.to(torch.complex128) if a.is_complex() else a.to(torch.float64)
+            b = b.to(torch.complex128) if b.is_complex() else b.to(torch.float64)
+            return float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
+
+        for key, form, label in (
+            ("mixed", "float16-lowrank",
+             f"float16 channels coded as {args.lowrank_modes} temporal modes (16-bit hi + lo) + 16-bit residual, "
+             "MFMA delay-and-sum with one 16-bit coefficient per sample, float32 accumulation, complex64 IQ"),
+            ("mixed_direct", "float16",
+             "float16 channels and IQ rounded directly, MFMA delay-and-sum with one 16-bit coefficient per sample"),
+        ):
+            log(f"bench: mixed form {form}")
+            mixed = Pipeline(scene, rc, cr, form, rank_start, args.lowrank_modes)
+            m_ms, m_das = time_pipeline(mixed, args.warmup, args.steps)
+            iq = mixed.signal if mixed.code == 0 else mixed.signal.view(torch.float16).to(torch.float32).view(
+                torch.complex64)
+            record[key] = {
+                "precision": label,
+                "value": nv * nt / (m_ms / 1e3),
+                "ms_per_step": m_ms,
+                "das_ms": m_das,
+                "das_tflops_algorithmic": das_flops / (m_das / 1e3) / 1e12,
+                "speedup_vs_float32": ms_per_step / m_ms,
+                "rel_l2_vs_float32": {"iq": rel(iq, reference["iq"]), "power": rel(mixed.pd[:, 0], reference["pd"]),
+                                      "filtered_power": rel(mixed.power, reference["power"])},
+            }
+            del mixed, iq
+            torch.cuda.empty_cache()
+        del reference
     log("bench: adjoint / forward on the differentiable-OPW workload")
     record["adjoint"] = adjoint_timing()
     if not args.no_cpu_baseline:

@@ -104,6 +104,29 @@ rcabeam_status rcabeam_count_active_taps_device(const float* scan, const float* 
                                                 size_t na, size_t nv, float c, float fs, float fd, float fn,
                                                 unsigned long long* out);
 
+/* ---- low-rank + residual coding of 16-bit channels (mixed precision; no
+ * reference counterpart) -------------------------------------------------
+ * Pack complex64 (samples, channels, angles, frames) channels to the packed
+ * (channels, angles, samples, frames + 2k) layout in `storage` (FP16 or
+ * BF16): with basis V (complex128, k rows of nt = the columns of an nt x k
+ * matrix with orthonormal columns) each row c is written as its residual
+ * e = c - (c V) V^H in frames [0, nt) and the coefficients w = c V as 16-bit
+ * hi and lo parts in frames nt + 2j, nt + 2j + 1.  Delay-and-sum of the
+ * coded channels (nt + 2k frames, any rcabeam_opw_grid_device form) followed
+ * by rcabeam_lowrank_expand_device gives the delay-and-sum of c with only
+ * the residual rounded to 16 bits.  k <= 12, nt <= 256; range_error as
+ * rcabeam_pack_ensemble_fp16_device. */
+rcabeam_status rcabeam_pack_ensemble_lowrank_device(const void* source, void* target, int storage,
+                                                    const void* basis, size_t k, int* range_error, size_t ns,
+                                                    size_t nc, size_t na, size_t nt);
+
+/* Expand coded delay-and-sum output coded (nv, nt + 2k) complex64 into
+ * x (nv, nt) of out_storage: x_t = d_t + sum_j (d_{nt+2j} + d_{nt+2j+1})
+ * conj(V[t, j]); pd (optional, 2 nv floats): mean |x_t|^2 in the even
+ * entries, 0 in the odd ones.  out may be null when pd is given. */
+rcabeam_status rcabeam_lowrank_expand_device(const void* coded, const void* basis, size_t k, void* out,
+                                             int out_storage, float* pd, size_t nv, size_t nt);
+
 /* ---- temporal clutter filter (replaces the CuPy/cuBLAS/cuSOLVER path of
  * reference src/rcabeam/gpu_clutter.py:112-273) ----------------------------
  * iq is a row-major Casorati matrix (rows, frames) of `storage` samples. */

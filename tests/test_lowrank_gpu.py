@@ -1,0 +1,111 @@
+"""Low-rank + residual channel coding (rcabeam_amd.lowrank) kernel by kernel, against NumPy float64.
+
+* pack: residual e = c - (c V) V^H rounded to FP16 and the coefficients w = c V as FP16 hi + lo in 2k
+  extra frame columns, in the packed (channels, angles, samples, frames + 2k) layout;
+* expand: x_t = d_t + sum_j (d_{nt+2j} + d_{nt+2j+1}) conj(V[t, j]) and its mean power;
+* round trip: decoding the packed rows themselves (the identity map in place of the delay-and-sum)
+  returns the channels to FP16 rounding of the residual only.
+"""
+
+import numpy as np
+import pytest
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _basis(rng, nt, k):
+    q, _ = np.linalg.qr(rng.normal(size=(nt, k)) + 1j * rng.normal(size=(nt, k)))
+    return q  # (nt, k), orthonormal columns
+
+
+def _device_basis(v):
+    import torch
+
+    from rcabeam_amd import _device as dev
+
+    return dev.to_device(np.ascontiguousarray(v.T), torch.complex128)  # (k, nt): row j = column j
+
+
+def test_pack_matches_numpy_coding():
+    import torch
+
+    from rcabeam_amd import _device as dev
+    from rcabeam_amd.lowrank import pack_channels_lowrank
+
+    rng = np.random.default_rng(3)
+    ns, nc, na, nt, k = 9, 5, 3, 40, 3
+    v = _basis(rng, nt, k)
+    # rows dominated by the basis modes (clutter) plus a small residual
+    coef = (rng.normal(size=(ns, nc, na, k)) + 1j * rng.normal(size=(ns, nc, na, k))) * 30
+    small = (rng.normal(size=(ns, nc, na, nt)) + 1j * rng.normal(size=(ns, nc, na, nt))) * 0.01
+    c = (coef @ v.conj().T + small).astype(np.complex64)
+    packed = pack_channels_lowrank(dev.to_device(c, torch.complex64), "float16", _device_basis(v))
+    words = packed.cpu().numpy().view(np.float16).reshape(nc, na, ns, nt + 2 * k, 2)
+    got = (words[..., 0].astype(np.float64) + 1j * words[..., 1].astype(np.float64)).transpose(2, 0, 1, 3)
+    c64 = c.astype(np.complex128)
+    w = c64 @ v
+    e = c64 - w @ v.conj().T
+    e16 = e.real.astype(np.float32).astype(np.float16) + 1j * e.imag.astype(np.float32).astype(np.float16)
+    # residual: FP16 rounding of the float64 residual (1 ulp allowed for summation order)
+    np.testing.assert_allclose(got[..., :nt], e16, rtol=2 ** -10, atol=2 ** -24)
+    whi, wlo = got[..., nt::2], got[..., nt + 1::2]
+    assert rel_l2(whi + wlo, w) < 2 ** -20
+    assert rel_l2(whi, w) < 2 ** -10
+
+
+@pytest.mark.parametrize("storage", ["float32", "float16"])
+def test_expand_matches_numpy(storage):
+    import torch
+
+    from rcabeam_amd import _device as dev
+    from rcabeam_amd import _lib
+
+    rng = np.random.default_rng(4)
+    nv, nt, k = 333, 72, 4
+    v = _basis(rng, nt, k)
+    coded = (rng.normal(size=(nv, nt + 2 * k)) + 1j * rng.normal(size=(nv, nt + 2 * k))).astype(np.complex64)
+    d = coded.astype(np.complex128)
+    expected = d[:, :nt] + (d[:, nt::2] + d[:, nt + 1::2]) @ v.conj().T
+    code = _lib.STORAGE[storage]
+    out = dev.empty((nv, nt), torch.complex64 if code == 0 else torch.int32)
+    pd = dev.empty((nv, 2), torch.float32)
+    coded_d, basis_d = dev.to_device(coded, torch.complex64), _device_basis(v)  # kept alive across the call
+    _lib.call("rcabeam_lowrank_expand_device", dev.ptr(coded_d), dev.ptr(basis_d), k, dev.ptr(out), code,
+              dev.ptr(pd), nv, nt)
+    x = out if code == 0 else out.view(torch.float16).to(torch.float32).view(torch.complex64)
+    x = x.cpu().numpy()
+    assert rel_l2(x, expected) < (1e-6 if storage == "float32" else 1e-3)
+    assert rel_l2(pd[:, 0].cpu().numpy(), np.mean(np.abs(expected) ** 2, axis=1)) < 1e-6
+    np.testing.assert_array_equal(pd[:, 1].cpu().numpy(), 0.0)
+
+
+def test_round_trip_through_identity_map():
+    """Decode the packed rows directly: the identity stands in for the (linear, frame-wise) delay-and-sum."""
+    import torch
+
+    from rcabeam_amd import _device as dev
+    from rcabeam_amd import _lib
+    from rcabeam_amd.lowrank import pack_channels_lowrank
+
+    rng = np.random.default_rng(5)
+    ns, nc, na, nt, k = 16, 4, 2, 64, 2
+    v = _basis(rng, nt, k)
+    coef = (rng.normal(size=(ns, nc, na, k)) + 1j * rng.normal(size=(ns, nc, na, k))) * 100
+    blood = (rng.normal(size=(ns, nc, na, nt)) + 1j * rng.normal(size=(ns, nc, na, nt))) * 0.01
+    c = (coef @ v.conj().T + blood).astype(np.complex64)
+    basis = _device_basis(v)
+    packed = pack_channels_lowrank(dev.to_device(c, torch.complex64), "float16", basis)
+    words = packed.cpu().numpy().view(np.float16).reshape(-1, nt + 2 * k, 2)
+    rows = (words[..., 0].astype(np.float32) + 1j * words[..., 1].astype(np.float32)).astype(np.complex64)
+    out = dev.empty((rows.shape[0], nt), torch.complex64)
+    rows_d = dev.to_device(rows, torch.complex64)
+    _lib.call("rcabeam_lowrank_expand_device", dev.ptr(rows_d), dev.ptr(basis), k, dev.ptr(out), 0, 0,
+              rows.shape[0], nt)
+    back = out.cpu().numpy().reshape(nc, na, ns, nt).transpose(2, 0, 1, 3)
+    direct = (c.real.astype(np.float16).astype(np.float32) + 1j * c.imag.astype(np.float16).astype(np.float32))
+    weak = np.linalg.norm(blood)
+    err_coded = np.linalg.norm(back - c) / weak  # error relative to the weak part
+    err_direct = np.linalg.norm(direct - c) / weak
+    assert err_coded < 2e-3, err_coded      # FP16 rounding of the residual (~2^-12 of it) and float32 of the rest
+    assert err_direct > 50 * err_coded      # rounding the channels directly swamps the weak part

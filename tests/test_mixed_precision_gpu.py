@@ -15,7 +15,10 @@ and bounded (bounds sit just above the measured errors, as regression guards).
 16-bit channel storage quantizes the 0 dB tissue echoes with a relative error of ~2^-12 (FP16)
 or ~2^-9 (BF16).  That error is white across frames, so the clutter filter cannot remove it and it
 lands on the -40 dB blood signal: filtered power errors of a few 1e-3 (FP16) and several 1e-2
-(BF16), while IQ and unfiltered power stay below 1e-3 with FP16.
+(BF16), while IQ and unfiltered power stay below 1e-3 with FP16.  Low-rank + residual coding
+(rcabeam_amd.lowrank, k = 4 temporal modes of the channels kept as 16-bit hi + lo coefficients)
+rounds only the residual: filtered power 4e-5 (FP16) and 3e-4 (BF16) with complex64 IQ.  The
+beamformed IQ stored in FP16 rounds the tissue again, so that form stays at a few 1e-3.
 """
 
 import numpy as np
@@ -49,26 +52,33 @@ def _scene():
     return _CACHE["scene"]
 
 
-def _run(channel_storage, iq_storage, coefficients, gram_precision):
+def _run(channel_storage, iq_storage, coefficients, gram_precision, lowrank_k=0):
     from rcabeam_amd import _device as dev
     from rcabeam_amd import _lib
     from rcabeam_amd import clutter as gc
     from rcabeam_amd.ensemble import _Geometry, pack_channels
+    from rcabeam_amd.lowrank import beamform_lowrank
 
     scene, rc, cr, *_ = _scene()
     lib = _lib.load()
     g = _Geometry(scene.angles, scene.t_start, scene.grid, scene.geom)
     ns, nc, na, nt = rc.shape
-    packed = [pack_channels(x, channel_storage) for x in (rc, cr)]
     code = _lib.STORAGE[iq_storage]
-    pd = dev.empty((g.nv, 2), torch.float32)
-    signal = dev.empty((g.nv, nt), torch.complex64 if code == 0 else torch.int32)
-    nx, nz, ny = g.shape
-    c, fs, fd, fn = g.constants
-    _lib.check(lib.rcabeam_opw_grid_device(
-        dev.ptr(packed[0]), dev.ptr(packed[1]), _lib.STORAGE[channel_storage], coefficients, dev.ptr(g.axes[0]),
-        dev.ptr(g.axes[1]), dev.ptr(g.axes[2]), nx, nz, ny, dev.ptr(g.xe), dev.ptr(g.ye), dev.ptr(g.angles),
-        dev.ptr(g.starts), dev.ptr(pd), dev.ptr(signal), code, 0, ns, nc, na, nt, c, fs, fd, fn), "das")
+    if lowrank_k:
+        signal, pd = beamform_lowrank(rc, cr, scene.angles, scene.t_start, scene.grid, scene.geom,
+                                      storage=channel_storage, k=lowrank_k, coefficients=coefficients,
+                                      iq_storage=iq_storage)
+    else:
+        packed = [pack_channels(x, channel_storage) for x in (rc, cr)]
+        pd = dev.empty((g.nv, 2), torch.float32)
+        signal = dev.empty((g.nv, nt), torch.complex64 if code == 0 else torch.int32)
+        nx, nz, ny = g.shape
+        c, fs, fd, fn = g.constants
+        _lib.check(lib.rcabeam_opw_grid_device(
+            dev.ptr(packed[0]), dev.ptr(packed[1]), _lib.STORAGE[channel_storage], coefficients,
+            dev.ptr(g.axes[0]), dev.ptr(g.axes[1]), dev.ptr(g.axes[2]), nx, nz, ny, dev.ptr(g.xe), dev.ptr(g.ye),
+            dev.ptr(g.angles), dev.ptr(g.starts), dev.ptr(pd), dev.ptr(signal), code, 0, ns, nc, na, nt,
+            c, fs, fd, fn), "das")
     storage = None if code == 0 else iq_storage
     values, basis = gc.temporal_basis(signal, precision=gram_precision, storage=storage)
     power = gc.filtered_power(signal, basis, rank_start=RANK_START, storage=storage)
@@ -82,13 +92,19 @@ def _run(channel_storage, iq_storage, coefficients, gram_precision):
 
 
 # channels, beamformed IQ, coefficients (0: float32-accurate, 1: one 16-bit per sample), Gram precision,
-# bounds on (IQ, unfiltered power, filtered power, clutter singular values)
+# bounds on (IQ, unfiltered power, filtered power, clutter singular values), low-rank coding modes k
+# (0: channels rounded directly)
 CASES = {
-    "float32": ("float32", "float32", 0, "complex128", (1e-5, 1e-5, 2e-5, 1e-6)),
-    "fp16 channels + fp16 IQ": ("float16", "float16", 1, "complex64", (5e-4, 1e-3, 6e-3, 1e-5)),
-    "fp16 channels + complex64 IQ": ("float16", "float32", 1, "complex64", (5e-4, 1e-3, 4e-3, 1e-5)),
-    "bf16 channels + bf16 IQ": ("bfloat16", "bfloat16", 1, "complex64", (4e-3, 8e-3, 2e-1, 1e-4)),
-    "bf16 channels + complex64 IQ": ("bfloat16", "float32", 1, "complex64", (3e-3, 6e-3, 5e-2, 1e-4)),
+    "float32": ("float32", "float32", 0, "complex128", (1e-5, 1e-5, 2e-5, 1e-6), 0),
+    "fp16 channels + fp16 IQ": ("float16", "float16", 1, "complex64", (5e-4, 1e-3, 6e-3, 1e-5), 0),
+    "fp16 channels + complex64 IQ": ("float16", "float32", 1, "complex64", (5e-4, 1e-3, 4e-3, 1e-5), 0),
+    "bf16 channels + bf16 IQ": ("bfloat16", "bfloat16", 1, "complex64", (4e-3, 8e-3, 2e-1, 1e-4), 0),
+    "bf16 channels + complex64 IQ": ("bfloat16", "float32", 1, "complex64", (3e-3, 6e-3, 5e-2, 1e-4), 0),
+    "fp16 low-rank channels + complex64 IQ": ("float16", "float32", 1, "complex128", (4e-4, 3e-4, 1.5e-4, 5e-6), 4),
+    "fp16 low-rank channels + fp16 IQ": ("float16", "float16", 1, "complex64", (6e-4, 3e-4, 8e-3, 1e-5), 4),
+    "bf16 low-rank channels + complex64 IQ": ("bfloat16", "float32", 1, "complex128", (3e-3, 2e-3, 1e-3, 2e-4), 4),
+    "fp16 low-rank channels, float32-accurate coefficients": ("float16", "float32", 0, "complex128",
+                                                              (1e-5, 1e-5, 5e-5, 5e-7), 4),
 }
 
 
@@ -96,9 +112,9 @@ CASES = {
 def test_pipeline_error_against_float64_oracle(name):
     from oracle import combiners
 
-    channel_storage, iq_storage, coefficients, precision, bounds = CASES[name]
+    channel_storage, iq_storage, coefficients, precision, bounds, lowrank_k = CASES[name]
     _, _, _, iq64, values64, power64 = _scene()
-    iq, values, power, pd = _run(channel_storage, iq_storage, coefficients, precision)
+    iq, values, power, pd = _run(channel_storage, iq_storage, coefficients, precision, lowrank_k)
     e_iq = rel_l2(iq, iq64)
     e_pd = rel_l2(pd, combiners.power_doppler(iq64))
     e_fp = rel_l2(power, power64)
