@@ -736,11 +736,15 @@ extern "C" rcabeam_status rcabeam_opw_grid_device(const void* rc, const void* cr
                        ? kCoefSingle
                        : accurate_coefficients(channel_storage);
   hipError_t e = hipErrorNotSupported;
-  // 16-bit channels: MFMA interpolation; complex64 channels, BF16 channels with
-  // float32-accurate coefficients and separate apertures: the VALU tile kernel.
+  // OPW: 16-bit channels on MFMA, complex64 channels on the VALU grid kernel;
+  // BF16 channels with float32-accurate coefficients, separate apertures and
+  // shapes those kernels do not take: the VALU tile kernel.
   if (method == kOpw && channel_storage != RCABEAM_STORAGE_C64)
     e = launch_opw_grid_mfma(p, x, z, y, static_cast<int>(nx), static_cast<int>(nz), static_cast<int>(ny),
                              channel_storage, coef, signal_storage);
+  if (method == kOpw && channel_storage == RCABEAM_STORAGE_C64)
+    e = launch_opw_grid_valu(p, x, z, y, static_cast<int>(nx), static_cast<int>(nz), static_cast<int>(ny),
+                             signal_storage);
   if (e == hipErrorNotSupported)
     e = launch_opw_grid(p, x, z, y, static_cast<int>(nx), static_cast<int>(nz), static_cast<int>(ny),
                         channel_storage, coef, signal_storage, method == kApertures);

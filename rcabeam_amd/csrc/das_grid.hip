@@ -1,0 +1,276 @@
+// OPW delay-and-sum of complex64 channels on a Cartesian grid, float32 VALU
+// arithmetic: the complex64 OPW path of rcabeam_opw_grid_device (reference
+// src/rcabeam/ensemble.cu:341-454 computes the same sums with per-tap
+// float32 gathers).
+//
+// Work decomposition (one 512-thread workgroup = 8 wave64s, two per CU): a
+// workgroup owns an 8 x 8 lateral patch of voxels at one depth and all
+// frames (tiles ordered depth-fastest, XCD-aware).  Wave w owns voxel row
+// x = w; the 8 lanes l = 8 y + f of voxel (w, y) own the frame pairs
+// (2 f + 16 j, 2 f + 16 j + 1), j < FPP, so one ds_read_b128 of a wave reads
+// 128 contiguous bytes of one sample row per voxel (no LDS bank conflicts
+// whatever the rows).  Tile tables, analytic windows, live-channel skipping
+// and stage planning are those of grid_tile.hpp; windows of consecutive
+// angles are packed into stages copied from the packed (channels, angles,
+// samples, frames) layout into LDS rows of nt samples with 1 KiB
+// global_load_lds_dwordx4 instructions, double-buffered across angles and
+// channels, one barrier per stage.
+//
+// Per tap and frame: v = a + w (b - a), acc += v e^{i phi} (8 FMA), with the
+// tap (row, weight, phasor) of the lane's own voxel combined from its
+// receive and transmit terms exactly as combine() does.
+#include "grid_tile.hpp"
+#include "rcabeam_mi355x.h"
+
+#include <algorithm>
+
+namespace rcb {
+namespace {
+
+constexpr int kVThreads = 512;
+constexpr int kVWaves = kVThreads / kWave;
+constexpr int kVTileSlots = 64;
+constexpr int kVMaxRows = 48;
+constexpr int kVMinRows = 8;
+constexpr int kVLdsBytes = 80 * 1024;  // two workgroups per CU
+constexpr int kMaxPairs = 13;          // nt <= 208
+
+struct VLayout {  // byte offsets inside the dynamic LDS allocation
+  int psum, win, buffer_words, bytes;
+};
+
+// Tile tables at 0; psum: float[64]; win: two buffers of max_rows rows x nt
+// complex64 samples plus the frames the last pair block of the last row
+// reads past nt (32-bit words).
+__host__ __device__ inline VLayout vlayout(int nc, int nt, int fpp, int max_rows) {
+  VLayout l;
+  l.psum = tile_table_bytes(nc);
+  l.win = (l.psum + kVTileSlots * 4 + 15) & ~15;
+  l.buffer_words = (2 * (max_rows * nt + 16 * fpp) + 3) & ~3;
+  l.bytes = l.win + 2 * l.buffer_words * 4;
+  return l;
+}
+
+template <int FPP>
+__global__ __launch_bounds__(kVThreads) __attribute__((amdgpu_waves_per_eu(4))) void das_opw_grid_valu(GridArgs p) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  const VLayout layout = vlayout(p.nc, p.nt, FPP, p.max_rows);
+  float* psum = reinterpret_cast<float*>(lds + layout.psum);      // [64]
+  float2* win = reinterpret_cast<float2*>(lds + layout.win);      // [2][buffer_words / 2]
+  const int buffer = layout.buffer_words / 2;                     // complex samples per buffer
+
+  GridTile<kVThreads> tile(p, lds);
+  if (!tile.valid()) return;
+  const int tid = threadIdx.x;
+  const int lane = tid & (kWave - 1);
+  const int wave = tid / kWave;  // voxel row x = wave
+  const int vy = lane >> 3;      // voxel column y
+  const int fl = lane & 7;       // frame pairs 2 fl + 16 j
+  const int nt = p.nt;
+  const int nq = 2 * p.nc;
+  const int max_rows = p.max_rows;
+
+  {  // both window buffers zeroed once (pairs past nt then read zeros or finite samples)
+    uint4* w4 = reinterpret_cast<uint4*>(win);
+    const int n = layout.buffer_words * 2 / 4;
+    for (int i = tid; i < n; i += kVThreads) w4[i] = make_uint4(0u, 0u, 0u, 0u);
+  }
+  tile.build_tables();  // ends with a barrier
+
+  const size_t plane_size = static_cast<size_t>(p.ns) * nt;
+  auto plane_of = [&](const Channel& ch) {
+    const float2* data = static_cast<const float2*>(ch.cfg == 0 ? p.rc : p.cr);
+    return data + static_cast<size_t>(ch.q - ch.cfg * p.nc) * p.na * plane_size;
+  };
+  // Receive positions of this lane's voxel: x for RC, y for CR.
+  auto setup = [&](int q) { return tile.setup(q, wave, vy, 0); };
+
+  // Copy a stage into buf: windows of angles [a0, a1) back to back (or one
+  // chunk of a long window), contiguous in HBM and in LDS (rows of nt
+  // samples), 1 KiB per wave instruction spread over the waves.
+  auto issue = [&](const Channel& ch, const Stage& s, float2* buf) {
+    const float2* plane0 = plane_of(ch);
+    int off = 0;
+    for (int a = s.a0; a < s.a1; ++a) {
+      int lo, rows;
+      if (s.r0 >= 0) {
+        lo = s.r0;
+        rows = s.rows;
+      } else {
+        const int2 b = ch.w.at(a);
+        if (b.x > b.y) continue;
+        lo = b.x;
+        rows = b.y - b.x + 1;
+      }
+      const float2* src = plane0 + a * plane_size + static_cast<size_t>(lo) * nt;
+      float2* dst = buf + off * nt;
+      off += rows;
+      const int samples = rows * nt;
+      if (p.vec) {
+        const int chunks = (samples + 127) >> 7;  // 128 samples = 1 KiB
+        for (int i = wave; i < chunks; i += kVWaves) {
+          const int so = i * 128 + lane * 2;
+          if (so < samples)
+            __builtin_amdgcn_global_load_lds(
+                reinterpret_cast<const void*>(src + so),
+                reinterpret_cast<__attribute__((address_space(3))) void*>(reinterpret_cast<uintptr_t>(dst + i * 128)),
+                16, 0, 0);
+        }
+      } else {
+        for (int i = tid; i < samples; i += kVThreads) dst[i] = src[i];
+      }
+    }
+  };
+
+  float2 acc[2 * FPP];
+#pragma unroll
+  for (int j = 0; j < 2 * FPP; ++j) acc[j] = make_float2(0.f, 0.f);
+
+  auto consume = [&](const Channel& ch, const Stage& cur, const float2* base) {
+    const int t_lo = cur.r0 >= 0 ? cur.r0 : -kBig;
+    const int t_hi = cur.r0 >= 0 && !cur.last ? cur.r0 + max_rows - 2 : kBig;
+    const int tpos = ch.cfg == 0 ? vy : wave;  // transmit position of this lane's voxel
+    int off = 0;
+    for (int a = cur.a0; a < cur.a1; ++a) {
+      int first, row_off;
+      if (cur.r0 >= 0) {
+        first = cur.r0;
+        row_off = 0;
+      } else {
+        const int2 bw = ch.w.at(a);
+        if (bw.x > bw.y) continue;
+        first = bw.x;
+        row_off = off;
+        off += bw.y - bw.x + 1;
+      }
+      const Tap tp = tap_of(ch.rx, tile.transmit(ch.cfg, a, tpos), p.ns);
+      if (!(tp.i0 >= 0 && tp.i0 >= t_lo && tp.i0 <= t_hi)) continue;
+      const int r0 = tp.i0 - first + row_off;
+      const int r1 = tp.w != 0.f ? r0 + 1 : r0;
+      const float4* s0 = reinterpret_cast<const float4*>(base + r0 * nt + 2 * fl);
+      const float4* s1 = reinterpret_cast<const float4*>(base + r1 * nt + 2 * fl);
+#pragma unroll
+      for (int j = 0; j < FPP; ++j) {
+        const float4 va = s0[8 * j], vb = s1[8 * j];  // frames 2 fl + 16 j, + 1
+        const float ax = fmaf(tp.w, vb.x - va.x, va.x), ay = fmaf(tp.w, vb.y - va.y, va.y);
+        const float bx = fmaf(tp.w, vb.z - va.z, va.z), by = fmaf(tp.w, vb.w - va.w, va.w);
+        acc[2 * j].x = fmaf(tp.c, ax, fmaf(-tp.s, ay, acc[2 * j].x));
+        acc[2 * j].y = fmaf(tp.c, ay, fmaf(tp.s, ax, acc[2 * j].y));
+        acc[2 * j + 1].x = fmaf(tp.c, bx, fmaf(-tp.s, by, acc[2 * j + 1].x));
+        acc[2 * j + 1].y = fmaf(tp.c, by, fmaf(tp.s, bx, acc[2 * j + 1].y));
+      }
+    }
+  };
+
+  // ---- stages of all live channels, one barrier per stage
+  Stage cur{0, 0, -1, 0, true, false};
+  Channel ch;
+  ch.q = -1;
+  for (int q = tile.next_live(-1); q < nq; q = tile.next_live(q)) {
+    ch = setup(q);
+    cur = plan_stage(ch.w, 0, p.na, max_rows);
+    if (cur.valid) break;
+  }
+  int b = 0;
+  if (cur.valid) issue(ch, cur, win);
+  __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+  __syncthreads();
+  while (cur.valid) {
+    Stage nxt = next_stage(cur, ch.w, p.na, max_rows);
+    Channel nch = ch;
+    bool crossing = false;
+    if (!nxt.valid) {
+      for (int q = tile.next_live(ch.q); q < nq; q = tile.next_live(q)) {
+        nch = setup(q);
+        nxt = plan_stage(nch.w, 0, p.na, max_rows);
+        if (nxt.valid) break;
+      }
+      crossing = nxt.valid;
+    }
+    if (nxt.valid) issue(crossing ? nch : ch, nxt, win + (b ^ 1) * buffer);
+    consume(ch, cur, win + b * buffer);
+    __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+    __syncthreads();
+    b ^= 1;
+    if (crossing) ch = nch;
+    cur = nxt;
+  }
+
+  // ---- epilogue: IQ (frame pairs as float4 where both frames exist) and power
+  const long long v = tile.voxel(wave, vy);
+  float power = 0.f;
+#pragma unroll
+  for (int j = 0; j < FPP; ++j) {
+    const int t = 2 * fl + 16 * j;
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      if (t + e < nt) {
+        const float2 s = acc[2 * j + e];
+        power = fmaf(s.x, s.x, fmaf(s.y, s.y, power));
+        if (p.signal && v >= 0) {
+          const long long index = v * nt + t + e;
+          if (p.out_storage == RCABEAM_STORAGE_FP16)
+            store_sample(reinterpret_cast<__half2*>(p.signal) + index, s);
+          else if (p.out_storage == RCABEAM_STORAGE_BF16)
+            store_sample(reinterpret_cast<bf16x2*>(p.signal) + index, s);
+          else
+            store_sample(reinterpret_cast<float2*>(p.signal) + index, s);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int off = 1; off < 8; off <<= 1) power += __shfl_xor(power, off, kWave);
+  if (fl == 0) psum[wave * 8 + vy] = power;
+  __syncthreads();
+  if (tid < kVTileSlots) {
+    const long long vv = tile.voxel(tid >> 3, tid & 7);
+    if (vv >= 0) {
+      p.pd[2 * vv] = psum[tid] / static_cast<float>(nt);
+      p.pd[2 * vv + 1] = 0.f;
+    }
+  }
+}
+
+template <int FPP>
+hipError_t launch_pairs(GridArgs a) {
+  const int fixed = vlayout(a.nc, a.nt, FPP, 0).bytes;
+  a.max_rows = std::min(kVMaxRows, (kVLdsBytes - fixed) / (2 * a.nt * 8));
+  if (a.max_rows < kVMinRows) return hipErrorNotSupported;
+  const int bytes = vlayout(a.nc, a.nt, FPP, a.max_rows).bytes;
+  if (bytes > kVLdsBytes) return hipErrorNotSupported;
+  const long long grid = (a.tiles + 7) / 8 * 8;
+  if (grid > 0x7fffffffLL) return hipErrorInvalidValue;
+  // 16-byte rows and pair loads need an even frame count and 16-byte aligned planes.
+  if (a.nt % 2 != 0 || reinterpret_cast<uintptr_t>(a.rc) % 16 || reinterpret_cast<uintptr_t>(a.cr) % 16)
+    return hipErrorNotSupported;
+  a.vec = 1;
+  auto kernel = das_opw_grid_valu<FPP>;
+  static bool attribute_set = false;  // one opt-in per instantiation (beyond 64 KiB of dynamic LDS)
+  if (!attribute_set) {
+    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, kVLdsBytes);
+    if (e != hipSuccess) return e;
+    attribute_set = true;
+  }
+  kernel<<<dim3(static_cast<unsigned>(grid)), kVThreads, bytes>>>(a);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_opw_grid_valu(const EnsembleArgs& p, const float* x, const float* z, const float* y, int nx,
+                                int nz, int ny, int signal_storage) {
+  if (p.na > kGridMaxAngles || p.nt > 16 * kMaxPairs) return hipErrorNotSupported;
+  if (static_cast<long long>(p.ns) * p.nt > 0x7fffffffLL) return hipErrorNotSupported;
+  if (signal_storage < RCABEAM_STORAGE_C64 || signal_storage > RCABEAM_STORAGE_BF16) return hipErrorInvalidValue;
+  const GridArgs a = make_grid_args(p, x, z, y, nx, nz, ny, signal_storage);
+  const int pairs = (p.nt + 15) / 16;
+  if (pairs <= 1) return launch_pairs<1>(a);
+  if (pairs <= 2) return launch_pairs<2>(a);
+  if (pairs <= 4) return launch_pairs<4>(a);
+  if (pairs <= 7) return launch_pairs<7>(a);
+  return launch_pairs<kMaxPairs>(a);
+}
+
+}  // namespace rcb

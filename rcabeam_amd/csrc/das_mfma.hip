@@ -43,9 +43,8 @@
 // boundaries, double-buffered across angles and channels, one barrier per
 // stage.  B operands of frames past nt read the next row or zero padding and
 // only feed accumulator columns that are never stored.
-#include "ensemble.hpp"
+#include "grid_tile.hpp"
 #include "rcabeam_mi355x.h"
-#include "tile_staging.hpp"
 
 #include <algorithm>
 
@@ -55,8 +54,6 @@ namespace {
 constexpr int kMThreads = 256;
 constexpr int kMWaves = kMThreads / kWave;  // one wave per 16-voxel group
 constexpr int kTileSlots = 64;
-constexpr int kPatchM = 8;
-constexpr int kMaxAnglesM = 16;
 constexpr int kBlockFrames = 32;  // N of one MFMA
 constexpr int kMaxBlocks = 7;     // nt <= 224
 constexpr int kKRows = 8;         // window rows of one K-step
@@ -100,267 +97,55 @@ __device__ __forceinline__ u32x4 place(int rel, uint32_t c0, uint32_t c1) {
   return a;
 }
 
-// A delay term as an ordered key: integer sample, then fraction (a float in
-// [0, 1), whose bits order like the value).  Inactive terms carry the
-// extreme key of the reduction they take part in.
-struct Key {
-  int i;
-  uint32_t f;
-};
-__device__ __forceinline__ bool key_less(Key a, Key b) { return a.i < b.i || (a.i == b.i && a.f < b.f); }
-__device__ __forceinline__ Key key_min(Key a, Key b) { return key_less(b, a) ? b : a; }
-__device__ __forceinline__ Key key_max(Key a, Key b) { return key_less(a, b) ? b : a; }
-constexpr Key kKeyHigh{0x7fffffff, 0u};
-constexpr Key kKeyLow{-0x7fffffff - 1, 0u};
-
-template <int CTRL>
-__device__ __forceinline__ Key dpp_key(Key k) {
-  return Key{__builtin_amdgcn_update_dpp(0, k.i, CTRL, 0xf, 0xf, false),
-             static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(k.f), CTRL, 0xf, 0xf, false))};
-}
-
-// First and last sample rows of the tap of receive key r and transmit key t,
-// computed as combine() computes the tap (integer sum, float32 fraction sum,
-// carry); x = first row, y = last row (first + 1 unless the fraction is 0).
-__device__ __forceinline__ int2 tap_rows(Key r, Key t) {
-  int i = r.i + t.i;
-  float f = __uint_as_float(r.f) + __uint_as_float(t.f);
-  if (f >= 1.0f) {
-    i += 1;
-    f -= 1.0f;
-  }
-  return make_int2(i, f != 0.0f ? i + 1 : i);
-}
-
-// The tap of receive term r and transmit term t exactly as combine() forms
-// it, without branches: first row (-1: no contribution), weight, phasor.
-__device__ __forceinline__ Tap tap_of(const Term& r, const Term& t, int ns) {
-  int i = r.i + t.i;
-  float f = r.f + t.f;
-  const bool carry = f >= 1.0f;
-  i += carry ? 1 : 0;
-  f = carry ? f - 1.0f : f;
-  const bool valid =
-      r.i != kInactive && t.i != kInactive && i >= 0 && (i < ns - 1 || (i == ns - 1 && f == 0.0f));
-  Tap out;
-  out.i0 = valid ? i : -1;
-  out.w = valid ? f : 0.f;
-  out.c = valid ? r.c * t.c - r.s * t.s : 0.f;
-  out.s = valid ? r.c * t.s + r.s * t.c : 0.f;
-  return out;
-}
-
-// Window of the voxels whose receive terms span [rlo, rhi] and transmit terms
-// [tlo, thi]: rows of valid taps (closed window 0 <= s <= ns - 1).  Empty
-// (x > y) if either set is empty.
-__device__ __forceinline__ int2 window(Key rlo, Key rhi, Key tlo, Key thi, int ns) {
-  if (rlo.i == kKeyHigh.i || tlo.i == kKeyHigh.i) return make_int2(kBig, -kBig);
-  const int lo = tap_rows(rlo, tlo).x;
-  const int hi = tap_rows(rhi, thi).y;
-  return make_int2(max(lo, 0), min(hi, ns - 1));
-}
-
-struct MArgs {
-  const void* rc;  // packed (channels, angles, samples, frames) 16-bit RC channels
-  const void* cr;
-  const float* x;
-  const float* z;
-  const float* y;
-  const float* xe;
-  const float* ye;
-  const float* angles;
-  const float* starts;
-  float* pd;
-  void* signal;
-  long long tiles;
-  int nx, nz, ny, nby;
-  int ns, nc, na, nt;
-  int max_rows, out_storage, vec;
-  double c, fs, fd, fn;
-};
-
 struct MLayout {  // byte offsets inside the dynamic LDS allocation
-  int txw, live, psum, win, buffer_words, bytes;
+  int psum, win, buffer_words, bytes;
 };
 
-// txt: Term[2][kMaxAnglesM][8] transmit terms (aperture, angle, tile
-// position) at 0; txw: Key[2][kMaxAnglesM][6] their (min, max) over the 8
-// positions, positions 0-3 and positions 4-7; live: one bit per channel
-// (dynamic aperture reaches the tile); psum: float[64]; win: two buffers of
+// Tile tables (grid_tile.hpp) at 0; psum: float[64]; win: two buffers of
 // max_rows rows x nt words, each followed by zero padding that K-steps at the
 // end of a stage and frame blocks past nt may read.
 __host__ __device__ inline MLayout mlayout(int nc, int nt, int nb, int max_rows) {
   MLayout l;
-  l.txw = 2 * kMaxAnglesM * 8 * 16;
-  l.live = l.txw + 2 * kMaxAnglesM * 6 * 8;
-  l.psum = l.live + ((2 * nc + 63) / 64) * 8;
+  l.psum = tile_table_bytes(nc);
   l.win = (l.psum + kTileSlots * 4 + 15) & ~15;
   l.buffer_words = ((max_rows + kKRows) * nt + kBlockFrames * nb + 3) & ~3;
   l.bytes = l.win + 2 * l.buffer_words * 4;
   return l;
 }
 
-// What a wave knows about one channel: its receive term for the A-operand
-// voxel, the tile window of every angle (lane a) and the group window of
-// every angle (lane a).
-struct Channel {
-  int q, cfg;
-  Term rx;    // A-operand voxel's receive term
-  Windows w;  // tile windows
-  Windows g;  // this wave's group windows
-};
-
 template <typename T, int CP, int NB>
-__global__ __launch_bounds__(kMThreads) __attribute__((amdgpu_waves_per_eu(2))) void das_opw_mfma(MArgs p) {
+__global__ __launch_bounds__(kMThreads) __attribute__((amdgpu_waves_per_eu(2))) void das_opw_mfma(GridArgs p) {
   constexpr int CW = CP == kCoefSplit ? 4 : 2;
   using W = uint32_t;
   extern __shared__ __align__(16) unsigned char lds[];
   const MLayout layout = mlayout(p.nc, p.nt, NB, p.max_rows);
-  Term* txt = reinterpret_cast<Term*>(lds);                       // [2][16][8]
-  Key* txw = reinterpret_cast<Key*>(lds + layout.txw);            // [2][16][6]
-  uint64_t* live = reinterpret_cast<uint64_t*>(lds + layout.live);
-  float* psum = reinterpret_cast<float*>(lds + layout.psum);      // [64]
-  W* win = reinterpret_cast<W*>(lds + layout.win);                // [2][buffer_words]
+  float* psum = reinterpret_cast<float*>(lds + layout.psum);  // [64]
+  W* win = reinterpret_cast<W*>(lds + layout.win);            // [2][buffer_words]
   const int buffer = layout.buffer_words;
 
-  long long tile;
-  {
-    const long long per_xcd = (p.tiles + 7) / 8;
-    tile = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;  // contiguous tiles per XCD
-  }
-  if (tile >= p.tiles) return;
+  GridTile<kMThreads> tile(p, lds);
+  if (!tile.valid()) return;
   const int tid = threadIdx.x;
   const int lane = tid & (kWave - 1);
-  const int g = tid / kWave;      // voxel group of this wave
+  const int g = tid / kWave;      // voxel group (tile quarter) of this wave
   const int col = lane & 31;      // frame column of the B operand and the output
   const int kg = lane >> 5;       // window rows 4 kg .. 4 kg + 3 of a K-step
   const bool imag_row = (col & 1) != 0;
   const int nt = p.nt;
   const int nq = 2 * p.nc;
   const int max_rows = p.max_rows;
-
-  const int iz = static_cast<int>(tile % p.nz);
-  const long long lateral = tile / p.nz;
-  const int x0 = static_cast<int>(lateral / p.nby) * kPatchM;
-  const int y0 = static_cast<int>(lateral % p.nby) * kPatchM;
-  const double zv = static_cast<double>(p.z[iz]);
   // A-operand voxel of this lane: group g's (vx, vy) = ((col / 2) / 4, (col / 2) % 4).
   const int ax = 4 * (g >> 1) + (col >> 3);
   const int ay = 4 * (g & 1) + ((col >> 1) & 3);
-  // Receive / transmit positions of that voxel for RC (receive along x) and CR.
-  auto in_grid = [&](int cfg_axis_x, int pos) {
-    return cfg_axis_x ? x0 + pos < p.nx : y0 + pos < p.ny;
-  };
-  auto coord = [&](int axis_x, int pos) -> double {
-    return static_cast<double>(axis_x ? p.x[x0 + pos] : p.y[y0 + pos]);
-  };
 
-  // ---- per tile: transmit terms, their min / max, live channels, zeroed buffers
-  {
-    const int cfg = tid >> 7, a = (tid >> 3) & 15, pos = tid & 7;
-    const int axis_x = cfg == 0 ? 0 : 1;  // RC transmits along y, CR along x
-    Term t{kInactive, 0.f, 0.f, 0.f};
-    if (a < p.na && in_grid(axis_x, pos)) {
-      double sn, cs;
-      sincos(static_cast<double>(p.angles[a]), &sn, &cs);
-      t = transmit_term(coord(axis_x, pos), zv, cs, sn, static_cast<double>(p.starts[a]), p.c, p.fs, p.fd);
-    }
-    txt[tid] = t;
-  }
-  for (int q0 = 0; q0 < nq; q0 += kMThreads) {
-    const int q = q0 + tid;
-    bool on = false;
-    if (q < nq) {
-      const int cfg = q >= p.nc ? 1 : 0;
-      const double e = static_cast<double>((cfg == 0 ? p.xe : p.ye)[q - cfg * p.nc]);
-      for (int pos = 0; pos < kPatchM; ++pos) {
-        if (!in_grid(cfg == 0, pos)) continue;
-        const double dv = coord(cfg == 0, pos) - e;
-        on = on || !(p.fn > 0.0) || fabs(dv) <= zv / (2.0 * p.fn);
-      }
-    }
-    const uint64_t mask = __ballot(on);
-    if (lane == 0 && q0 + kWave * g < nq) live[(q0 >> 6) + g] = mask;
-  }
-  {
+  {  // both window buffers and their padding zeroed once
     uint4* w4 = reinterpret_cast<uint4*>(win);
     const int n = 2 * buffer / 4;
     for (int i = tid; i < n; i += kMThreads) w4[i] = make_uint4(0u, 0u, 0u, 0u);
   }
-  __syncthreads();
-  if (tid < 2 * kMaxAnglesM) {
-    const Term* t = txt + tid * 8;
-    Key lo[3] = {kKeyHigh, kKeyHigh, kKeyHigh}, hi[3] = {kKeyLow, kKeyLow, kKeyLow};
-    for (int pos = 0; pos < kPatchM; ++pos) {
-      if (t[pos].i == kInactive) continue;
-      const Key k{t[pos].i, __float_as_uint(t[pos].f)};
-      lo[0] = key_min(lo[0], k);
-      hi[0] = key_max(hi[0], k);
-      lo[1 + (pos >> 2)] = key_min(lo[1 + (pos >> 2)], k);
-      hi[1 + (pos >> 2)] = key_max(hi[1 + (pos >> 2)], k);
-    }
-    for (int k = 0; k < 3; ++k) {
-      txw[tid * 6 + 2 * k] = lo[k];
-      txw[tid * 6 + 2 * k + 1] = hi[k];
-    }
-  }
-  __syncthreads();
-
-  // ---- per channel, in every wave (same inputs, same results)
-  auto next_live = [&](int q) {  // first live channel after q, nq if none
-    for (int word = (q + 1) >> 6; word * 64 < nq; ++word) {
-      uint64_t m = live[word];
-      if (word == (q + 1) >> 6) m &= ~0ull << ((q + 1) & 63);
-      m = (static_cast<uint64_t>(uniform(static_cast<int>(m >> 32))) << 32) |
-          static_cast<uint32_t>(uniform(static_cast<int>(m & 0xffffffffu)));
-      if (m) return min(nq, word * 64 + __builtin_ctzll(m));
-    }
-    return nq;
-  };
-  auto setup = [&](int q) {
-    Channel ch;
-    ch.q = q;
-    ch.cfg = q >= p.nc ? 1 : 0;
-    const int axis_x = ch.cfg == 0 ? 1 : 0;  // RC receives along x, CR along y
-    const double e = static_cast<double>((ch.cfg == 0 ? p.xe : p.ye)[q - ch.cfg * p.nc]);
-    // Lane l forms the receive term of tile position l % 8.
-    const int pos = lane & 7;
-    Term r{kInactive, 0.f, 0.f, 0.f};
-    if (in_grid(axis_x, pos)) r = receive_term(coord(axis_x, pos), zv, e, p.c, p.fs, p.fd, p.fn);
-    // This lane's A-operand voxel's receive term, from lane (its position).
-    const int src = ch.cfg == 0 ? ax : ay;
-    ch.rx.i = __shfl(r.i, src, kWave);
-    ch.rx.f = __shfl(r.f, src, kWave);
-    ch.rx.c = __shfl(r.c, src, kWave);
-    ch.rx.s = __shfl(r.s, src, kWave);
-    // Min / max of the receive keys: positions 0-3 and 4-7 (quads), then all 8.
-    const bool on = r.i != kInactive;
-    Key lo = on ? Key{r.i, __float_as_uint(r.f)} : kKeyHigh;
-    Key hi = on ? Key{r.i, __float_as_uint(r.f)} : kKeyLow;
-    lo = key_min(lo, dpp_key<0xB1>(lo));  // quad_perm [1,0,3,2]
-    hi = key_max(hi, dpp_key<0xB1>(hi));
-    lo = key_min(lo, dpp_key<0x4E>(lo));  // quad_perm [2,3,0,1]
-    hi = key_max(hi, dpp_key<0x4E>(hi));
-    const Key hlo[2] = {Key{uniform(lo.i), static_cast<uint32_t>(uniform(static_cast<int>(lo.f)))},
-                        Key{__builtin_amdgcn_readlane(lo.i, 4),
-                            static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(lo.f), 4))}};
-    const Key hhi[2] = {Key{uniform(hi.i), static_cast<uint32_t>(uniform(static_cast<int>(hi.f)))},
-                        Key{__builtin_amdgcn_readlane(hi.i, 4),
-                            static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(hi.f), 4))}};
-    const Key tlo = key_min(hlo[0], hlo[1]), thi = key_max(hhi[0], hhi[1]);
-    // Lane a: windows of angle a for the tile and for this wave's group.
-    const int a = lane < p.na ? lane : 0;
-    const Key* tw = txw + (ch.cfg * kMaxAnglesM + a) * 6;
-    int2 tile_w = window(tlo, thi, tw[0], tw[1], p.ns);
-    // Group g: RC receive half g / 2 (x), transmit half g % 2 (y); CR the other way.
-    const int rh = ch.cfg == 0 ? g >> 1 : g & 1;
-    const int th = ch.cfg == 0 ? g & 1 : g >> 1;
-    int2 group_w = window(hlo[rh], hhi[rh], tw[2 + 2 * th], tw[3 + 2 * th], p.ns);
-    if (lane >= p.na) tile_w = group_w = make_int2(kBig, -kBig);
-    ch.w = Windows{tile_w.x, tile_w.y};
-    ch.g = Windows{group_w.x, group_w.y};
-    return ch;
-  };
+  tile.build_tables();  // ends with a barrier
+  auto setup = [&](int q) { return tile.setup(q, ax, ay, g); };
+  auto next_live = [&](int q) { return tile.next_live(q); };
   auto plane_of = [&](const Channel& ch) {
     const W* data = static_cast<const W*>(ch.cfg == 0 ? p.rc : p.cr);
     return data + static_cast<size_t>(ch.q - ch.cfg * p.nc) * p.na * p.ns * nt;
@@ -437,7 +222,7 @@ __global__ __launch_bounds__(kMThreads) __attribute__((amdgpu_waves_per_eu(2))) 
       const int lo = max(gw.x, lim_lo);
       const int hi = min(gw.y, lim_hi);
       if (lo > hi) continue;
-      const Tap tp = tap_of(ch.rx, txt[(ch.cfg * kMaxAnglesM + a) * 8 + tpos], p.ns);
+      const Tap tp = tap_of(ch.rx, tile.transmit(ch.cfg, a, tpos), p.ns);
       const bool take = tp.i0 >= 0 && tp.i0 >= t_lo && tp.i0 <= t_hi;
       const float u = 1.0f - tp.w;
       const float p0r = u * tp.c, p0i = u * tp.s, p1r = tp.w * tp.c, p1i = tp.w * tp.s;
@@ -511,11 +296,7 @@ __global__ __launch_bounds__(kMThreads) __attribute__((amdgpu_waves_per_eu(2))) 
   // ---- epilogue.  acc[j][e] is output row 8 (e / 4) + 4 kg + e % 4 = (voxel
   // 4 (e / 4) + 2 kg + (e / 2) % 2 of the group, Re / Im = e % 2) at frame
   // 32 j + col; group voxel u sits at (4 (g / 2) + u / 4, 4 (g % 2) + u % 4).
-  auto voxel_id = [&](int u) -> long long {
-    const int ix = x0 + 4 * (g >> 1) + (u >> 2), iy = y0 + 4 * (g & 1) + (u & 3);
-    if (ix >= p.nx || iy >= p.ny) return -1;
-    return (static_cast<long long>(ix) * p.nz + iz) * p.ny + iy;
-  };
+  auto voxel_id = [&](int u) { return tile.voxel(4 * (g >> 1) + (u >> 2), 4 * (g & 1) + (u & 3)); };
   long long vid[8];
 #pragma unroll
   for (int k = 0; k < 8; ++k) vid[k] = voxel_id(4 * (k >> 1) + 2 * kg + (k & 1));
@@ -552,9 +333,8 @@ __global__ __launch_bounds__(kMThreads) __attribute__((amdgpu_waves_per_eu(2))) 
   __syncthreads();
   if (tid < kTileSlots) {
     const int gg = tid >> 4, u = tid & 15;
-    const int ix = x0 + 4 * (gg >> 1) + (u >> 2), iy = y0 + 4 * (gg & 1) + (u & 3);
-    if (ix < p.nx && iy < p.ny) {
-      const long long v = (static_cast<long long>(ix) * p.nz + iz) * p.ny + iy;
+    const long long v = tile.voxel(4 * (gg >> 1) + (u >> 2), 4 * (gg & 1) + (u & 3));
+    if (v >= 0) {
       p.pd[2 * v] = psum[tid] / static_cast<float>(nt);
       p.pd[2 * v + 1] = 0.f;
     }
@@ -562,7 +342,7 @@ __global__ __launch_bounds__(kMThreads) __attribute__((amdgpu_waves_per_eu(2))) 
 }
 
 template <typename T, int CP, int NB>
-hipError_t launch_shape(MArgs a) {
+hipError_t launch_shape(GridArgs a) {
   const int fixed = mlayout(a.nc, a.nt, NB, 0).bytes;
   a.max_rows = std::min(kMaxRowsM, (kLdsBytesM - fixed) / (2 * a.nt * 4));
   if (a.max_rows < kMinRowsM) return hipErrorNotSupported;
@@ -584,7 +364,7 @@ hipError_t launch_shape(MArgs a) {
 }
 
 template <typename T, int CP>
-hipError_t launch_blocks(const MArgs& a) {
+hipError_t launch_blocks(const GridArgs& a) {
   const int blocks = (a.nt + kBlockFrames - 1) / kBlockFrames;
   if (blocks <= 1) return launch_shape<T, CP, 1>(a);
   if (blocks <= 2) return launch_shape<T, CP, 2>(a);
@@ -596,38 +376,10 @@ hipError_t launch_blocks(const MArgs& a) {
 
 hipError_t launch_opw_grid_mfma(const EnsembleArgs& p, const float* x, const float* z, const float* y, int nx, int nz,
                                 int ny, int channel_storage, int coefficients, int signal_storage) {
-  if (p.na > kMaxAnglesM || p.nt > kBlockFrames * kMaxBlocks) return hipErrorNotSupported;
+  if (p.na > kGridMaxAngles || p.nt > kBlockFrames * kMaxBlocks) return hipErrorNotSupported;
   if (static_cast<long long>(p.ns) * p.nt > 0x7fffffffLL) return hipErrorNotSupported;
   if (signal_storage < RCABEAM_STORAGE_C64 || signal_storage > RCABEAM_STORAGE_BF16) return hipErrorInvalidValue;
-  const int nbx = (nx + kPatchM - 1) / kPatchM, nby = (ny + kPatchM - 1) / kPatchM;
-  MArgs a;
-  a.rc = p.rc;
-  a.cr = p.cr;
-  a.x = x;
-  a.z = z;
-  a.y = y;
-  a.xe = p.xe;
-  a.ye = p.ye;
-  a.angles = p.angles;
-  a.starts = p.starts;
-  a.pd = p.pd;
-  a.signal = p.signal;
-  a.tiles = static_cast<long long>(nbx) * nby * nz;
-  a.nx = nx;
-  a.nz = nz;
-  a.ny = ny;
-  a.nby = nby;
-  a.ns = p.ns;
-  a.nc = p.nc;
-  a.na = p.na;
-  a.nt = p.nt;
-  a.max_rows = 0;
-  a.out_storage = signal_storage;
-  a.vec = 0;
-  a.c = p.c;
-  a.fs = p.fs;
-  a.fd = p.fd;
-  a.fn = p.fn;
+  const GridArgs a = make_grid_args(p, x, z, y, nx, nz, ny, signal_storage);
   switch (channel_storage) {
     case RCABEAM_STORAGE_FP16:
       if (coefficients == kCoefSingle) return launch_blocks<__half2, kCoefSingle>(a);

@@ -59,6 +59,13 @@ hipError_t launch_opw_grid(const EnsembleArgs& p, const float* x, const float* z
 hipError_t launch_opw_grid_mfma(const EnsembleArgs& p, const float* x, const float* z, const float* y, int nx, int nz,
                                 int ny, int channel_storage, int coefficients, int signal_storage);
 
+// OPW on a Cartesian grid for complex64 channels, float32 VALU
+// interpolation (das_grid.hip): analytic windows, live-channel skipping.
+// Same results as launch_opw_grid up to float32 summation order;
+// hipErrorNotSupported for na > 16, nt > 208, odd nt or unaligned channels.
+hipError_t launch_opw_grid_valu(const EnsembleArgs& p, const float* x, const float* z, const float* y, int nx,
+                                int nz, int ny, int signal_storage);
+
 // Adjoint of the grid OPW for complex64 channels (das_tile.hip): accumulates
 // the packed channel gradients grc / gcr and dL/dc (*gc) for the IQ gradient
 // giq (nv, nt); p.rc / p.cr hold the forward's packed channels.

@@ -268,12 +268,12 @@ def test_sixteen_bit_coefficients_error(storage):
 @pytest.mark.parametrize("storage,coefficients", [("float32", 0), ("float16", 0), ("float16", 1), ("bfloat16", 1)])
 def test_grid_tiles_equal_voxel_list(storage, coefficients):
     """rcabeam_opw_grid_device against the voxel-list entry point on the same packed channels.  complex64
-    channels: the grid runs the VALU tile kernel with, per voxel, the same operations in the same order,
-    so IQ and power are bit-identical.  16-bit channels: the grid runs the MFMA kernel, whose products are
-    the same exact 16-bit products summed in another order; float32 rounding of ~770-term sums in two
-    orders differs by ~2^-24 sqrt(770) ~ 2e-6 relative (bound 5e-6; both stay within 1e-5 of the float64
-    oracle, test_fp16_opw_on_quantized_point_scene).  Grid sizes that are not multiples of the patch
-    leave partial tiles."""
+    channels: the grid kernel (das_grid.hip) accumulates taps in the voxel-list kernel's order, with its
+    phasor products rounded independently (rel L2 < 1e-6).  16-bit channels: the grid runs the MFMA
+    kernel, whose products are the same exact 16-bit products summed in another order; float32 rounding
+    of ~770-term sums in two orders differs by ~2^-24 sqrt(770) ~ 2e-6 relative (bound 5e-6; both stay
+    within 1e-5 of the float64 oracle, test_fp16_opw_on_quantized_point_scene).  Grid sizes that are not
+    multiples of the patch leave partial tiles."""
     import torch
 
     from rcabeam_amd import _device as dev
@@ -305,13 +305,10 @@ def test_grid_tiles_equal_voxel_list(storage, coefficients):
         dev.synchronize()
         out.append((sig.cpu().numpy(), pd.cpu().numpy()))
     assert np.isfinite(out[1][0]).all() and np.isfinite(out[1][1]).all()
-    if storage == "float32":
-        np.testing.assert_array_equal(out[0][0], out[1][0])
-        np.testing.assert_array_equal(out[0][1], out[1][1])
-    else:
-        assert rel_l2(out[1][0], out[0][0]) < 5e-6
-        assert rel_l2(out[1][1], out[0][1]) < 5e-6
-        np.testing.assert_array_equal(out[1][1][:, 1], 0.0)
+    bound = 1e-6 if storage == "float32" else 5e-6
+    assert rel_l2(out[1][0], out[0][0]) < bound
+    assert rel_l2(out[1][1], out[0][1]) < bound
+    np.testing.assert_array_equal(out[1][1][:, 1], 0.0)
 
 
 def test_grid_entry_rejects_unsupported_shapes():
