@@ -63,7 +63,7 @@ __global__ __launch_bounds__(kVThreads) __attribute__((amdgpu_waves_per_eu(4))) 
   if (!tile.valid()) return;
   const int tid = threadIdx.x;
   const int lane = tid & (kWave - 1);
-  const int wave = tid / kWave;  // voxel row x = wave
+  const int wave = uniform(tid / kWave);  // voxel row x = wave (wave-uniform: loops over it stay scalar)
   const int vy = lane >> 3;      // voxel column y
   const int fl = lane & 7;       // frame pairs 2 fl + 16 j
   const int nt = p.nt;
@@ -90,11 +90,12 @@ __global__ __launch_bounds__(kVThreads) __attribute__((amdgpu_waves_per_eu(4))) 
   // samples), 1 KiB per wave instruction spread over the waves.
   auto issue = [&](const Channel& ch, const Stage& s, float2* buf) {
     const float2* plane0 = plane_of(ch);
+    const int a0 = uniform(s.a0), a1 = uniform(s.a1), sr0 = uniform(s.r0);
     int off = 0;
-    for (int a = s.a0; a < s.a1; ++a) {
+    for (int a = a0; a < a1; ++a) {
       int lo, rows;
-      if (s.r0 >= 0) {
-        lo = s.r0;
+      if (sr0 >= 0) {
+        lo = sr0;
         rows = s.rows;
       } else {
         const int2 b = ch.w.at(a);
@@ -106,18 +107,14 @@ __global__ __launch_bounds__(kVThreads) __attribute__((amdgpu_waves_per_eu(4))) 
       float2* dst = buf + off * nt;
       off += rows;
       const int samples = rows * nt;
-      if (p.vec) {
-        const int chunks = (samples + 127) >> 7;  // 128 samples = 1 KiB
-        for (int i = wave; i < chunks; i += kVWaves) {
-          const int so = i * 128 + lane * 2;
-          if (so < samples)
-            __builtin_amdgcn_global_load_lds(
-                reinterpret_cast<const void*>(src + so),
-                reinterpret_cast<__attribute__((address_space(3))) void*>(reinterpret_cast<uintptr_t>(dst + i * 128)),
-                16, 0, 0);
-        }
-      } else {
-        for (int i = tid; i < samples; i += kVThreads) dst[i] = src[i];
+      const int chunks = (samples + 127) >> 7;  // 128 samples = 1 KiB
+      for (int i = wave; i < chunks; i += kVWaves) {
+        const int so = i * 128 + lane * 2;
+        if (so < samples)
+          __builtin_amdgcn_global_load_lds(
+              reinterpret_cast<const void*>(src + so),
+              reinterpret_cast<__attribute__((address_space(3))) void*>(reinterpret_cast<uintptr_t>(dst + i * 128)),
+              16, 0, 0);
       }
     }
   };
@@ -127,14 +124,16 @@ __global__ __launch_bounds__(kVThreads) __attribute__((amdgpu_waves_per_eu(4))) 
   for (int j = 0; j < 2 * FPP; ++j) acc[j] = make_float2(0.f, 0.f);
 
   auto consume = [&](const Channel& ch, const Stage& cur, const float2* base) {
-    const int t_lo = cur.r0 >= 0 ? cur.r0 : -kBig;
-    const int t_hi = cur.r0 >= 0 && !cur.last ? cur.r0 + max_rows - 2 : kBig;
+    // Stage fields are wave-uniform; saying so keeps the angle loop scalar.
+    const int a0 = uniform(cur.a0), a1 = uniform(cur.a1), cr0 = uniform(cur.r0);
+    const int t_lo = cr0 >= 0 ? cr0 : -kBig;
+    const int t_hi = cr0 >= 0 && !cur.last ? cr0 + max_rows - 2 : kBig;
     const int tpos = ch.cfg == 0 ? vy : wave;  // transmit position of this lane's voxel
     int off = 0;
-    for (int a = cur.a0; a < cur.a1; ++a) {
+    for (int a = a0; a < a1; ++a) {
       int first, row_off;
-      if (cur.r0 >= 0) {
-        first = cur.r0;
+      if (cr0 >= 0) {
+        first = cr0;
         row_off = 0;
       } else {
         const int2 bw = ch.w.at(a);
@@ -144,20 +143,23 @@ __global__ __launch_bounds__(kVThreads) __attribute__((amdgpu_waves_per_eu(4))) 
         off += bw.y - bw.x + 1;
       }
       const Tap tp = tap_of(ch.rx, tile.transmit(ch.cfg, a, tpos), p.ns);
-      if (!(tp.i0 >= 0 && tp.i0 >= t_lo && tp.i0 <= t_hi)) continue;
-      const int r0 = tp.i0 - first + row_off;
-      const int r1 = tp.w != 0.f ? r0 + 1 : r0;
-      const float4* s0 = reinterpret_cast<const float4*>(base + r0 * nt + 2 * fl);
-      const float4* s1 = reinterpret_cast<const float4*>(base + r1 * nt + 2 * fl);
+      // A lane-dependent branch around the body (not a `continue`) keeps the
+      // angle loop itself wave-uniform.
+      if (tp.i0 >= 0 && tp.i0 >= t_lo && tp.i0 <= t_hi) {
+        const int r0 = tp.i0 - first + row_off;
+        const int r1 = tp.w != 0.f ? r0 + 1 : r0;
+        const float4* s0 = reinterpret_cast<const float4*>(base + r0 * nt + 2 * fl);
+        const float4* s1 = reinterpret_cast<const float4*>(base + r1 * nt + 2 * fl);
 #pragma unroll
-      for (int j = 0; j < FPP; ++j) {
-        const float4 va = s0[8 * j], vb = s1[8 * j];  // frames 2 fl + 16 j, + 1
-        const float ax = fmaf(tp.w, vb.x - va.x, va.x), ay = fmaf(tp.w, vb.y - va.y, va.y);
-        const float bx = fmaf(tp.w, vb.z - va.z, va.z), by = fmaf(tp.w, vb.w - va.w, va.w);
-        acc[2 * j].x = fmaf(tp.c, ax, fmaf(-tp.s, ay, acc[2 * j].x));
-        acc[2 * j].y = fmaf(tp.c, ay, fmaf(tp.s, ax, acc[2 * j].y));
-        acc[2 * j + 1].x = fmaf(tp.c, bx, fmaf(-tp.s, by, acc[2 * j + 1].x));
-        acc[2 * j + 1].y = fmaf(tp.c, by, fmaf(tp.s, bx, acc[2 * j + 1].y));
+        for (int j = 0; j < FPP; ++j) {
+          const float4 va = s0[8 * j], vb = s1[8 * j];  // frames 2 fl + 16 j, + 1
+          const float ax = fmaf(tp.w, vb.x - va.x, va.x), ay = fmaf(tp.w, vb.y - va.y, va.y);
+          const float bx = fmaf(tp.w, vb.z - va.z, va.z), by = fmaf(tp.w, vb.w - va.w, va.w);
+          acc[2 * j].x = fmaf(tp.c, ax, fmaf(-tp.s, ay, acc[2 * j].x));
+          acc[2 * j].y = fmaf(tp.c, ay, fmaf(tp.s, ax, acc[2 * j].y));
+          acc[2 * j + 1].x = fmaf(tp.c, bx, fmaf(-tp.s, by, acc[2 * j + 1].x));
+          acc[2 * j + 1].y = fmaf(tp.c, by, fmaf(tp.s, bx, acc[2 * j + 1].y));
+        }
       }
     }
   };

@@ -127,7 +127,7 @@ __global__ __launch_bounds__(kMThreads) __attribute__((amdgpu_waves_per_eu(2))) 
   if (!tile.valid()) return;
   const int tid = threadIdx.x;
   const int lane = tid & (kWave - 1);
-  const int g = tid / kWave;      // voxel group (tile quarter) of this wave
+  const int g = uniform(tid / kWave);  // voxel group (tile quarter) of this wave, wave-uniform
   const int col = lane & 31;      // frame column of the B operand and the output
   const int kg = lane >> 5;       // window rows 4 kg .. 4 kg + 3 of a K-step
   const bool imag_row = (col & 1) != 0;

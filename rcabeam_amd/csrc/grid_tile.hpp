@@ -194,7 +194,7 @@ struct GridTile {
   // Transmit terms, their min / max and the live-channel bits; ends with a
   // barrier, after which the tables are visible to every wave.
   __device__ void build_tables() {
-    const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = uniform(tid / kWave);
     if (tid < 2 * kGridMaxAngles * 8) {
       const int cfg = tid >> 7, a = (tid >> 3) & 15, pos = tid & 7;
       const bool axis_x = cfg != 0;  // RC transmits along y, CR along x
