@@ -15,10 +15,12 @@ starting from channel data resident in HBM.  `--precision float16` (or
 MFMA with one 16-bit coefficient per sample (exact products, float32
 accumulation), Gram formed exactly from the 16-bit IQ.  `float16-lowrank`
 (or `bfloat16-lowrank`) is the mixed-precision form reported beside the
-float32 one: the channel Gram and a float64 eigendecomposition give the 4
-strongest temporal modes of the channels, which are coded as 16-bit hi + lo
+float32 one: a channel Gram (split bfloat16 hi + lo products on MFMA over every
+4th channel row) and a float64 eigendecomposition give the 4 strongest
+temporal modes of the channels, which are coded as 16-bit hi + lo
 coefficients while only the residual is rounded to 16 bits (rcabeam_amd.lowrank),
-MFMA delay-and-sum with 16-bit coefficients, complex64 IQ, exact float64 Gram.
+MFMA delay-and-sum with 16-bit coefficients, complex64 IQ, IQ Gram with split
+bfloat16 products on MFMA.
 
 Prints one JSON line.  `value` is voxel-frames/s of the whole step.
 """
@@ -41,6 +43,14 @@ HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 FLOPS_PER_TAP = 14  # 2 sub + 2 fma (interpolation) + 4 fma (rotate-accumulate), as counted in the reference kernel
 PROFILES = ROOT / "profiles"
+# The delay-and-sum kernel each precision's rcabeam_opw_grid_device launch runs on this workload.
+DAS_KERNEL = {
+    "float32": "das_opw_grid_valu<13> (complex64 channels, float32 VALU)",
+    "float16": "das_opw_mfma<__half2> (float16 channels, MFMA)",
+    "bfloat16": "das_opw_mfma<bf16x2> (bfloat16 channels, MFMA)",
+    "float16-lowrank": "das_opw_mfma<__half2> (low-rank coded float16 channels, MFMA)",
+    "bfloat16-lowrank": "das_opw_mfma<bf16x2> (low-rank coded bfloat16 channels, MFMA)",
+}
 
 
 def log(msg: str) -> None:
@@ -112,10 +122,14 @@ class Pipeline:
         self.pd = dev.empty((nv, 2), torch.float32)
         self.signal = dev.empty((nv, self.nt), torch.complex64 if self.code == 0 else torch.int32)
         if self.lowrank:
-            rows = self.ns * self.nc * self.na
+            from rcabeam_amd.lowrank import ROW_STEP
+
+            self.row_step = ROW_STEP
+            rows = (self.ns * self.nc * self.na + ROW_STEP - 1) // ROW_STEP
             self.c_rows = rows
-            self.c_work_bytes = int(self.lib.rcabeam_gram_f64_workspace_bytes(rows, self.nt))
-            self.c_work = dev.empty((self.c_work_bytes // 8,), torch.float64)
+            self.c_work_bytes = int(self.lib.rcabeam_gram_workspace_bytes(rows, self.nt))
+            self.c_work = dev.empty((self.c_work_bytes // 4,), torch.float32)
+            self.c_stats = dev.empty((3,), torch.int32)
             self.c_real = dev.empty((int(self.lib.rcabeam_gram_real_elements(self.nt)),), torch.float64)
             self.c_gram = dev.empty((self.nt, self.nt), torch.complex128)
             self.c_values = dev.empty((self.nt,), torch.float64)
@@ -123,7 +137,10 @@ class Pipeline:
             self.coded = dev.empty((nv, self.ntc), torch.complex64)
         self.stats = dev.empty((3,), torch.int32)
         self.real = dev.empty((int(self.lib.rcabeam_gram_real_elements(self.nt)),), torch.float64)
-        if self.code == 0:
+        # IQ Gram: exact float64 for the float32 form; split bfloat16 (hi + lo) products on MFMA for the
+        # low-rank mixed form; one exact 16-bit product for 16-bit IQ.
+        self.exact_gram = storage == "float32"
+        if self.exact_gram:
             self.work_bytes = int(self.lib.rcabeam_gram_f64_workspace_bytes(nv, self.nt))
             self.work = dev.empty((self.work_bytes // 8,), torch.float64)
         else:
@@ -150,11 +167,15 @@ class Pipeline:
         shape = (self.ns, self.nc, self.na, self.nt)
         channel_code = _lib.STORAGE[self.channel_storage]
         if self.lowrank:
-            # Channel basis: exact float64 Gram of both apertures' Casorati matrices, float64 EVD, the k
-            # strongest modes (descending; rows of the row-major view of the column-major eigenvectors).
+            # Channel basis (rcabeam_amd.lowrank.channel_basis): split-bfloat16 Gram of every row_step-th row
+            # of both apertures' Casorati matrices, float64 EVD, the k strongest modes (descending; rows of the
+            # row-major view of the column-major eigenvectors).
             for i, raw in enumerate((self.rc, self.cr)):
-                self._call("rcabeam_temporal_gram_f64_device", dev.ptr(raw), self.c_rows, self.nt,
-                           dev.ptr(self.c_real), dev.ptr(self.c_work), self.c_work_bytes, i)
+                self._call("rcabeam_iq_absmax_device", dev.ptr(raw), 0, raw.numel(), dev.ptr(self.c_stats), i)
+            for i, raw in enumerate((self.rc, self.cr)):
+                self._call("rcabeam_temporal_gram_sampled_device", dev.ptr(raw), 0, self.c_rows, self.row_step,
+                           self.nt, dev.ptr(self.c_stats), dev.ptr(self.c_real), dev.ptr(self.c_work),
+                           self.c_work_bytes, i)
             self._call("rcabeam_gram_hermitian_device", dev.ptr(self.c_real), self.nt, 1.0 / (2 * self.c_rows),
                        dev.ptr(self.c_gram))
             self._call("rcabeam_eigh_device", dev.ptr(self.c_gram), dev.ptr(self.c_values), dev.ptr(self.e_work),
@@ -189,7 +210,7 @@ class Pipeline:
             self._call("rcabeam_lowrank_expand_device", dev.ptr(self.coded), dev.ptr(self.basis), self.k,
                        dev.ptr(self.signal), 0, dev.ptr(self.pd), nv, self.nt)
         self._call("rcabeam_iq_absmax_device", dev.ptr(self.signal), self.code, nv * self.nt, dev.ptr(self.stats), 0)
-        if self.code == 0:
+        if self.exact_gram:
             self._call("rcabeam_temporal_gram_f64_device", dev.ptr(self.signal), nv, self.nt, dev.ptr(self.real),
                        dev.ptr(self.work), self.work_bytes, 0)
         else:
@@ -393,7 +414,7 @@ def main() -> int:
         "roofline": {
             "bound": "mfma",
             "engine": "fp32 VALU (delay-and-sum has no GEMM form at this sparsity; fp32 peak used as the roof)",
-            "kernel": "das_opw_tile (OPW delay-and-sum)",
+            "kernel": DAS_KERNEL[storage],
             "achieved": achieved,
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",

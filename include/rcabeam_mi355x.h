@@ -152,6 +152,15 @@ rcabeam_status rcabeam_temporal_gram_device(const void* iq, int storage, size_t 
                                             const float* scale, double* real_gram, float* workspace,
                                             size_t workspace_bytes, int accumulate);
 
+/* The same Gram over every row_step-th row of iq: rows 0, row_step, ...,
+ * (rows - 1) * row_step (rows = number of rows used; workspace as for
+ * rcabeam_gram_workspace_bytes(rows, frames)).  No reference counterpart:
+ * it forms the temporal basis of the low-rank channel coding from a row
+ * sample of the channel Casorati matrix. */
+rcabeam_status rcabeam_temporal_gram_sampled_device(const void* iq, int storage, size_t rows, size_t row_step,
+                                                    size_t frames, const float* scale, double* real_gram,
+                                                    float* workspace, size_t workspace_bytes, int accumulate);
+
 /* Exact real Gram of complex64 input in float64 on v_mfma_f64_16x16x4_f64
  * (products exact, float64 accumulation): the complex128 Gram of reference
  * gpu_clutter.py:140-161 without prescaling (float64 cannot overflow on

@@ -57,6 +57,8 @@ _SIGNATURES: dict[str, list] = {
     "rcabeam_gram_workspace_bytes": [c_size_t, c_size_t],
     "rcabeam_gram_real_elements": [c_size_t],
     "rcabeam_temporal_gram_device": [c_void_p, c_int, c_size_t, c_size_t, c_void_p, c_void_p, c_void_p, c_size_t, c_int],
+    "rcabeam_temporal_gram_sampled_device": [c_void_p, c_int, c_size_t, c_size_t, c_size_t, c_void_p, c_void_p,
+                                             c_void_p, c_size_t, c_int],
     "rcabeam_gram_f64_workspace_bytes": [c_size_t, c_size_t],
     "rcabeam_temporal_gram_f64_device": [c_void_p, c_size_t, c_size_t, c_void_p, c_void_p, c_size_t, c_int],
     "rcabeam_simulate_lds_bytes": [c_size_t, c_int],

@@ -60,9 +60,10 @@ __device__ __forceinline__ float bf16_bits_to_float(unsigned short b) { return _
 // MODE kSplitBf16: T = float2 (complex64), hi/lo bfloat16.
 // MODE kExactF16:  T = __half2, one fp16 product.
 // MODE kExactBf16: T = bf16x2, one bf16 product.
+// Row r of the Gram is row r * row_step of x (row_step 1: every row).
 template <int MODE, typename T>
-__global__ __launch_bounds__(kGramThreads) void k_gram(const T* __restrict__ x, long long rows, int frames,
-                                                      const float* __restrict__ scale_ptr, GramPlan plan,
+__global__ __launch_bounds__(kGramThreads) void k_gram(const T* __restrict__ x, long long rows, long long row_step,
+                                                      int frames, const float* __restrict__ scale_ptr, GramPlan plan,
                                                       float* __restrict__ partials) {
   constexpr bool kSplit = MODE == kSplitBf16;
   extern __shared__ __align__(16) unsigned short opnd[];  // [2 (hi, lo)][cols][kGramKPad]
@@ -107,7 +108,7 @@ __global__ __launch_bounds__(kGramThreads) void k_gram(const T* __restrict__ x, 
           const int k = idx / frames, t = idx - k * frames;
           const long long r = rbase + k;
           float2 v = make_float2(0.f, 0.f);
-          if (r < row_end) v = load_sample(x + r * frames + t);
+          if (r < row_end) v = load_sample(x + r * row_step * frames + t);
           unsigned short hr, hi_i, lr = 0, li = 0;
           if constexpr (MODE == kExactF16) {
             hr = __half_as_ushort(__float2half_rn(v.x));
@@ -454,8 +455,8 @@ static void make_plan(int nb, int pass, GramPlan& plan) {
 static size_t lds_bytes(int nb) { return static_cast<size_t>(2) * nb * 32 * kGramKPad * sizeof(unsigned short); }
 
 template <int MODE, typename T>
-static hipError_t gram_launch(const void* x, long long rows, int frames, const float* scale, double* c,
-                              float* workspace, int accumulate) {
+static hipError_t gram_launch(const void* x, long long rows, long long row_step, int frames, const float* scale,
+                              double* c, float* workspace, int accumulate) {
   const int nb = column_blocks(frames);
   const long long groups = (rows + kGramRowsPerGroup - 1) / kGramRowsPerGroup;
   const int passes = static_cast<int>((triangle_blocks(nb) + kGramBlocksPerPass - 1) / kGramBlocksPerPass);
@@ -468,7 +469,7 @@ static hipError_t gram_launch(const void* x, long long rows, int frames, const f
     GramPlan plan;
     make_plan(nb, pass, plan);
     k_gram<MODE, T><<<dim3(static_cast<unsigned>(groups)), kGramThreads, lds>>>(
-        static_cast<const T*>(x), rows, frames, scale, plan, workspace);
+        static_cast<const T*>(x), rows, row_step, frames, scale, plan, workspace);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     const long long n = static_cast<long long>(plan.pass_blocks) * 1024;
     k_gram_reduce<<<dim3(static_cast<unsigned>((n + 255) / 256)), 256>>>(workspace, static_cast<int>(groups), plan, c,
@@ -549,28 +550,38 @@ extern "C" rcabeam_status rcabeam_iq_absmax_device(const void* iq, int storage, 
   return launch_status();
 }
 
-extern "C" rcabeam_status rcabeam_temporal_gram_device(const void* iq, int storage, size_t rows, size_t frames,
-                                                       const float* scale, double* real_gram, float* workspace,
-                                                       size_t workspace_bytes, int accumulate) {
-  if (!iq || !real_gram || !workspace || !rows || !frames || storage < 0 || storage > 2) return RCABEAM_ERROR_ARGUMENT;
+extern "C" rcabeam_status rcabeam_temporal_gram_sampled_device(const void* iq, int storage, size_t rows,
+                                                               size_t row_step, size_t frames, const float* scale,
+                                                               double* real_gram, float* workspace,
+                                                               size_t workspace_bytes, int accumulate) {
+  if (!iq || !real_gram || !workspace || !rows || !row_step || !frames || storage < 0 || storage > 2)
+    return RCABEAM_ERROR_ARGUMENT;
   if (storage == RCABEAM_STORAGE_C64 && !scale) return RCABEAM_ERROR_ARGUMENT;
+  if (rows > (1ull << 40) || row_step > (1ull << 20)) return RCABEAM_ERROR_ARGUMENT;
   const size_t need = rcabeam_gram_workspace_bytes(rows, frames);
   if (!need || workspace_bytes < need) return RCABEAM_ERROR_ARGUMENT;
   hipError_t e;
-  const long long n = static_cast<long long>(rows);
+  const long long n = static_cast<long long>(rows), step = static_cast<long long>(row_step);
   const int t = static_cast<int>(frames);
   switch (storage) {
     case RCABEAM_STORAGE_C64:
-      e = gram_launch<kSplitBf16, float2>(iq, n, t, scale, real_gram, workspace, accumulate);
+      e = gram_launch<kSplitBf16, float2>(iq, n, step, t, scale, real_gram, workspace, accumulate);
       break;
     case RCABEAM_STORAGE_FP16:
-      e = gram_launch<kExactF16, __half2>(iq, n, t, scale, real_gram, workspace, accumulate);
+      e = gram_launch<kExactF16, __half2>(iq, n, step, t, scale, real_gram, workspace, accumulate);
       break;
     default:
-      e = gram_launch<kExactBf16, bf16x2>(iq, n, t, scale, real_gram, workspace, accumulate);
+      e = gram_launch<kExactBf16, bf16x2>(iq, n, step, t, scale, real_gram, workspace, accumulate);
   }
   if (e == hipErrorInvalidValue) return RCABEAM_ERROR_ARGUMENT;
   return e == hipSuccess ? RCABEAM_SUCCESS : RCABEAM_ERROR_CUDA;
+}
+
+extern "C" rcabeam_status rcabeam_temporal_gram_device(const void* iq, int storage, size_t rows, size_t frames,
+                                                       const float* scale, double* real_gram, float* workspace,
+                                                       size_t workspace_bytes, int accumulate) {
+  return rcabeam_temporal_gram_sampled_device(iq, storage, rows, 1, frames, scale, real_gram, workspace,
+                                              workspace_bytes, accumulate);
 }
 
 extern "C" rcabeam_status rcabeam_gram_hermitian_device(const double* real_gram, size_t frames, double scale2,

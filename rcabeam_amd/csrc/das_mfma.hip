@@ -158,11 +158,12 @@ __global__ __launch_bounds__(kMThreads) __attribute__((amdgpu_waves_per_eu(2))) 
   // instructions regardless of row boundaries, spread over the waves.
   auto issue = [&](const Channel& ch, const Stage& s, W* buf) {
     const W* plane0 = plane_of(ch);
+    const int a0 = uniform(s.a0), a1 = uniform(s.a1), sr0 = uniform(s.r0);
     int off = 0;
-    for (int a = s.a0; a < s.a1; ++a) {
+    for (int a = a0; a < a1; ++a) {
       int lo, rows;
-      if (s.r0 >= 0) {
-        lo = s.r0;
+      if (sr0 >= 0) {
+        lo = sr0;
         rows = s.rows;
       } else {
         const int2 b = ch.w.at(a);
@@ -198,17 +199,19 @@ __global__ __launch_bounds__(kMThreads) __attribute__((amdgpu_waves_per_eu(2))) 
 
   // This wave's group, one stage: K-steps of 8 window rows per angle.
   auto consume = [&](const Channel& ch, const Stage& cur, const W* base) {
-    const int t_lo = cur.r0 >= 0 ? cur.r0 : -kBig;
-    const int t_hi = cur.r0 >= 0 && !cur.last ? cur.r0 + max_rows - 2 : kBig;
+    // Stage fields are wave-uniform; saying so keeps the angle and K-step loops scalar.
+    const int a0 = uniform(cur.a0), a1 = uniform(cur.a1), cr0 = uniform(cur.r0), crows = uniform(cur.rows);
+    const int t_lo = cr0 >= 0 ? cr0 : -kBig;
+    const int t_hi = cr0 >= 0 && !cur.last ? cr0 + max_rows - 2 : kBig;
     const int tpos = ch.cfg == 0 ? ay : ax;  // transmit position of the A-operand voxel
     int off = 0;
-    for (int a = cur.a0; a < cur.a1; ++a) {
+    for (int a = a0; a < a1; ++a) {
       int first, row_off, lim_lo, lim_hi;
-      if (cur.r0 >= 0) {  // one chunk of a long window
-        first = cur.r0;
+      if (cr0 >= 0) {  // one chunk of a long window
+        first = cr0;
         row_off = 0;
-        lim_lo = cur.r0;
-        lim_hi = cur.r0 + cur.rows - 1;
+        lim_lo = cr0;
+        lim_hi = cr0 + crows - 1;
       } else {  // whole windows packed back to back
         const int2 bw = ch.w.at(a);
         if (bw.x > bw.y) continue;

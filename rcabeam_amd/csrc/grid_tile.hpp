@@ -295,7 +295,8 @@ struct GridTile {
     // Quarter g: RC receive half g / 2 (x), transmit half g % 2 (y); CR the other way.
     const int rh = ch.cfg == 0 ? g >> 1 : g & 1;
     const int th = ch.cfg == 0 ? g & 1 : g >> 1;
-    int2 group_w = window(hlo[rh], hhi[rh], tw[2 + 2 * th], tw[3 + 2 * th], p.ns);
+    // (selects, not indexing: a runtime index would place the key arrays in scratch memory)
+    int2 group_w = window(rh ? hlo[1] : hlo[0], rh ? hhi[1] : hhi[0], tw[2 + 2 * th], tw[3 + 2 * th], p.ns);
     if (lane >= p.na) tile_w = group_w = make_int2(kBig, -kBig);
     ch.w = Windows{tile_w.x, tile_w.y};
     ch.g = Windows{group_w.x, group_w.y};

@@ -20,20 +20,49 @@ from rcabeam_amd.ensemble import _RANGE_MESSAGE, _Geometry
 from rcabeam_amd.sim import RCAGeometry
 
 MAX_BASIS = 12  # rcabeam_pack_ensemble_lowrank_device: k <= 12
+ROW_STEP = 4  # channel rows sampled for the basis: every 4th (sample, channel, angle) row
 
 
-def channel_basis(rc, cr, k: int):
+def channel_basis(rc, cr, k: int, row_step: int = ROW_STEP):
     """The k strongest temporal modes of the RC and CR channels together: complex128 (k, nt), row j = column j
-    of V (orthonormal), from the exact float64 Gram and the float64 eigendecomposition on the GPU."""
-    from rcabeam_amd import clutter
+    of V (orthonormal).
 
+    The Gram is formed from every `row_step`-th row of both channel Casorati matrices (rows = (sample,
+    channel, angle), complex64) with split bfloat16 (hi + lo) products on MFMA, then decomposed in float64
+    on the GPU.  Any orthonormal V codes the channels exactly up to the 16-bit rounding of the residual;
+    the sample only has to catch the strong tissue modes, which every depth band carries."""
+    t = dev.torch()
     if not 1 <= k <= MAX_BASIS:
         raise ValueError(f"k must be in [1, {MAX_BASIS}]")
+    if row_step < 1:
+        raise ValueError("row_step must be positive")
     nt = rc.shape[-1]
     if k >= nt:
         raise ValueError("k must be smaller than the frame count")
-    _, basis = clutter.temporal_basis((rc.reshape(-1, nt), cr.reshape(-1, nt)), precision="complex128")
-    return basis[:, :k].transpose(0, 1).contiguous()
+    lib = _lib.load()
+    stats = dev.empty((3,), t.int32)
+    for i, a in enumerate((rc, cr)):
+        _lib.call("rcabeam_iq_absmax_device", dev.ptr(a), 0, a.numel(), dev.ptr(stats), int(i > 0))
+    total = rc.numel() // nt
+    rows = (total + row_step - 1) // row_step
+    nbytes = int(lib.rcabeam_gram_workspace_bytes(rows, nt))
+    if nbytes == 0:
+        raise ValueError("Frame count out of range for the GPU Gram")
+    work = dev.empty((nbytes // 4,), t.float32)
+    real = dev.empty((int(lib.rcabeam_gram_real_elements(nt)),), t.float64)
+    for i, a in enumerate((rc, cr)):
+        _lib.call("rcabeam_temporal_gram_sampled_device", dev.ptr(a), 0, rows, row_step, nt, dev.ptr(stats),
+                  dev.ptr(real), dev.ptr(work), nbytes, int(i > 0))
+    gram = dev.empty((nt, nt), t.complex128)
+    _lib.call("rcabeam_gram_hermitian_device", dev.ptr(real), nt, 1.0 / (2 * rows), dev.ptr(gram))
+    values = dev.empty((nt,), t.float64)
+    e_work = dev.empty((nt,), t.float64)
+    info = dev.empty((1,), t.int32)
+    _lib.call("rcabeam_eigh_device", dev.ptr(gram), dev.ptr(values), dev.ptr(e_work), dev.ptr(info), nt)
+    if int(info.item()) != 0:
+        raise ValueError("Channel Gram eigendecomposition failed (non-finite channels?)")
+    # Eigenvectors (ascending) in column-major order: row j of the row-major view is vector j.
+    return gram[nt - k:].flip(0).contiguous()
 
 
 def pack_channels_lowrank(raw, storage: str, basis, packed=None, range_error=None):
@@ -89,4 +118,4 @@ def beamform_lowrank(rc, cr, angles, t_start, grid, geom: RCAGeometry, *, storag
     return out, pd
 
 
-__all__ = ["MAX_BASIS", "beamform_lowrank", "channel_basis", "pack_channels_lowrank"]
+__all__ = ["MAX_BASIS", "ROW_STEP", "beamform_lowrank", "channel_basis", "pack_channels_lowrank"]

@@ -109,3 +109,73 @@ def test_round_trip_through_identity_map():
     err_direct = np.linalg.norm(direct - c) / weak
     assert err_coded < 2e-3, err_coded      # FP16 rounding of the residual (~2^-12 of it) and float32 of the rest
     assert err_direct > 50 * err_coded      # rounding the channels directly swamps the weak part
+
+
+def test_sampled_gram_matches_numpy():
+    """rcabeam_temporal_gram_sampled_device on complex64: the real Gram of rows 0, s, 2s, ... (split bfloat16
+    products), against float64 NumPy on the same rows; row step 1 equals rcabeam_temporal_gram_device."""
+    import torch
+
+    from rcabeam_amd import _device as dev
+    from rcabeam_amd import _lib
+
+    lib = _lib.load()
+    rng = np.random.default_rng(11)
+    total, nt, step = 5003, 24, 3
+    x = (rng.normal(size=(total, nt)) + 1j * rng.normal(size=(total, nt))).astype(np.complex64) * 7.0
+    xd = dev.to_device(x, torch.complex64)
+    stats = dev.empty((3,), torch.int32)
+    _lib.call("rcabeam_iq_absmax_device", dev.ptr(xd), 0, x.size, dev.ptr(stats), 0)
+    scale = float(stats.cpu().numpy().view(np.float32)[0])
+    ld = int(np.sqrt(int(lib.rcabeam_gram_real_elements(nt))))
+
+    def real_gram(rows, row_step, entry):
+        nbytes = int(lib.rcabeam_gram_workspace_bytes(rows, nt))
+        work = dev.empty((nbytes // 4,), torch.float32)
+        real = dev.empty((ld * ld,), torch.float64)
+        if entry == "sampled":
+            _lib.call("rcabeam_temporal_gram_sampled_device", dev.ptr(xd), 0, rows, row_step, nt, dev.ptr(stats),
+                      dev.ptr(real), dev.ptr(work), nbytes, 0)
+        else:
+            _lib.call("rcabeam_temporal_gram_device", dev.ptr(xd), 0, rows, nt, dev.ptr(stats), dev.ptr(real),
+                      dev.ptr(work), nbytes, 0)
+        c = real.cpu().numpy().reshape(ld, ld)[: 2 * nt, : 2 * nt]
+        return np.triu(c) + np.triu(c, 1).T  # upper blocks stored
+
+    rows = (total + step - 1) // step
+    sample = x[::step].astype(np.complex128) / scale
+    a = np.stack([sample.real, sample.imag], axis=-1).reshape(rows, 2 * nt)
+    expect = a.T @ a
+    got = real_gram(rows, step, "sampled")
+    assert rel_l2(got, expect) < 1e-5  # bfloat16 hi + lo keep 16 significant bits (2^-17 = 7.6e-6 per value)
+    # a wrong row map (every row, or rows offset by one) is far off
+    wrong = np.stack([x[1::step].real, x[1::step].imag], axis=-1).reshape(-1, 2 * nt).astype(np.float64) / scale
+    assert rel_l2(got, wrong.T @ wrong) > 1e-2
+    np.testing.assert_array_equal(real_gram(total, 1, "sampled"), real_gram(total, 1, "plain"))
+
+
+def test_channel_basis_spans_planted_clutter():
+    """channel_basis recovers a planted 3-mode temporal subspace of RC + CR channels from a row sample."""
+    import torch
+
+    from rcabeam_amd import _device as dev
+    from rcabeam_amd.lowrank import channel_basis
+
+    rng = np.random.default_rng(13)
+    ns, nc, na, nt, k = 64, 8, 3, 48, 3
+    v = _basis(rng, nt, k)
+    amp = np.array([300.0, 100.0, 30.0])
+
+    def channels():
+        coef = (rng.normal(size=(ns, nc, na, k)) + 1j * rng.normal(size=(ns, nc, na, k))) * amp
+        noise = (rng.normal(size=(ns, nc, na, nt)) + 1j * rng.normal(size=(ns, nc, na, nt))) * 0.1
+        return dev.to_device((coef @ v.conj().T + noise).astype(np.complex64), torch.complex64)
+
+    rc, cr = channels(), channels()
+    basis = channel_basis(rc, cr, k).cpu().numpy()  # (k, nt), row j = column j
+    assert np.allclose(basis @ basis.conj().T, np.eye(k), atol=1e-10)
+    # principal angles: the projection of the planted columns onto the found span keeps their norm
+    kept = np.linalg.norm(basis.conj() @ v, axis=0)
+    assert np.all(kept > 1 - 1e-5), kept
+    # modes in descending strength: the strongest planted mode is (up to phase) the first row
+    assert abs(np.vdot(basis[0], v[:, 0])) > 0.999

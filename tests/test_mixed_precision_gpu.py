@@ -101,6 +101,8 @@ CASES = {
     "bf16 channels + bf16 IQ": ("bfloat16", "bfloat16", 1, "complex64", (4e-3, 8e-3, 2e-1, 1e-4), 0),
     "bf16 channels + complex64 IQ": ("bfloat16", "float32", 1, "complex64", (3e-3, 6e-3, 5e-2, 1e-4), 0),
     "fp16 low-rank channels + complex64 IQ": ("float16", "float32", 1, "complex128", (4e-4, 3e-4, 1.5e-4, 5e-6), 4),
+    "fp16 low-rank channels + complex64 IQ, split Gram": ("float16", "float32", 1, "complex64",
+                                                          (4e-4, 3e-4, 1.5e-4, 1e-5), 4),
     "fp16 low-rank channels + fp16 IQ": ("float16", "float16", 1, "complex64", (6e-4, 3e-4, 8e-3, 1e-5), 4),
     "bf16 low-rank channels + complex64 IQ": ("bfloat16", "float32", 1, "complex128", (3e-3, 2e-3, 1e-3, 2e-4), 4),
     "fp16 low-rank channels, float32-accurate coefficients": ("float16", "float32", 0, "complex128",
