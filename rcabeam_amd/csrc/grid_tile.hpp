@@ -7,8 +7,9 @@
 // its transmit term on (y, z, angle); CR swaps x and y.  A tile therefore
 // needs 8 receive terms per channel and 8 transmit terms per angle (float64,
 // split into integer sample, float32 fraction and unit phasor, common.hpp).
-// Transmit terms are formed once per tile into LDS; every wave forms the 8
-// receive terms of a channel itself (lanes 0-7, duplicated over the wave).
+// Transmit terms are formed once per tile into LDS; every wave forms the
+// receive terms of 8 consecutive channels at once (lane l: channel l / 8,
+// position l % 8) and hands them out channel by channel.
 // The first tap row is monotonic in both terms, so the row window of any set
 // of voxels is exactly [first row of (min receive, min transmit), last row of
 // (max receive, max transmit)] clipped to the closed sample window: the
@@ -255,25 +256,28 @@ struct GridTile {
     return nq;
   }
 
-  // Channel q as this wave sees it.  rx_pos: receive position (tile-local,
-  // along x for RC, along y for CR) of this lane's voxel, given for RC and
-  // CR; quarter g of the tile (x half g / 2, y half g % 2) for ch.g.
-  __device__ Channel setup(int q, int rx_pos_rc, int rx_pos_cr, int g) const {
+  // Receive terms of the 8 channels [rx_base, rx_base + 8), formed together:
+  // lane l holds channel rx_base + l / 8 at tile position l % 8 (RC receives
+  // along x, CR along y; a run may cross from RC to CR), and the min / max
+  // keys over its quad of positions (0-3 or 4-7).  Live channels of a tile
+  // are consecutive per aperture, so one fill serves up to 8 setups.
+  int rx_base = -kBig;
+  Term rx_term;
+  Key rx_lo, rx_hi;
+
+  __device__ void fill_receive(int q0) {
     const int lane = threadIdx.x & (kWave - 1);
-    Channel ch;
-    ch.q = q;
-    ch.cfg = q >= p.nc ? 1 : 0;
-    const bool axis_x = ch.cfg == 0;  // RC receives along x, CR along y
-    const double e = static_cast<double>((ch.cfg == 0 ? p.xe : p.ye)[q - ch.cfg * p.nc]);
-    const int pos = lane & 7;  // lane l forms the receive term of tile position l % 8
+    const int q = q0 + (lane >> 3);
+    const int pos = lane & 7;
     Term r{kInactive, 0.f, 0.f, 0.f};
-    if (in_grid(axis_x, pos)) r = receive_term(coord(axis_x, pos), zv, e, p.c, p.fs, p.fd, p.fn);
-    const int src = ch.cfg == 0 ? rx_pos_rc : rx_pos_cr;
-    ch.rx.i = __shfl(r.i, src, kWave);
-    ch.rx.f = __shfl(r.f, src, kWave);
-    ch.rx.c = __shfl(r.c, src, kWave);
-    ch.rx.s = __shfl(r.s, src, kWave);
-    // Min / max of the receive keys: positions 0-3 and 4-7 (quads), then all 8.
+    if (q < 2 * p.nc) {
+      const int cfg = q >= p.nc ? 1 : 0;
+      const bool axis_x = cfg == 0;
+      if (in_grid(axis_x, pos)) {
+        const double e = static_cast<double>((cfg == 0 ? p.xe : p.ye)[q - cfg * p.nc]);
+        r = receive_term(coord(axis_x, pos), zv, e, p.c, p.fs, p.fd, p.fn);
+      }
+    }
     const bool on = r.i != kInactive;
     Key lo = on ? Key{r.i, __float_as_uint(r.f)} : kKeyHigh;
     Key hi = on ? Key{r.i, __float_as_uint(r.f)} : kKeyLow;
@@ -281,12 +285,37 @@ struct GridTile {
     hi = key_max(hi, dpp_key<0xB1>(hi));
     lo = key_min(lo, dpp_key<0x4E>(lo));  // quad_perm [2,3,0,1]
     hi = key_max(hi, dpp_key<0x4E>(hi));
-    const Key hlo[2] = {Key{uniform(lo.i), static_cast<uint32_t>(uniform(static_cast<int>(lo.f)))},
-                        Key{__builtin_amdgcn_readlane(lo.i, 4),
-                            static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(lo.f), 4))}};
-    const Key hhi[2] = {Key{uniform(hi.i), static_cast<uint32_t>(uniform(static_cast<int>(hi.f)))},
-                        Key{__builtin_amdgcn_readlane(hi.i, 4),
-                            static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(hi.f), 4))}};
+    rx_term = r;
+    rx_lo = lo;
+    rx_hi = hi;
+    rx_base = q0;
+  }
+
+  // Channel q as this wave sees it.  rx_pos: receive position (tile-local,
+  // along x for RC, along y for CR) of this lane's voxel, given for RC and
+  // CR; quarter g of the tile (x half g / 2, y half g % 2) for ch.g.  Called
+  // by the whole wave with q uniform.
+  __device__ Channel setup(int q, int rx_pos_rc, int rx_pos_cr, int g) {
+    const int lane = threadIdx.x & (kWave - 1);
+    Channel ch;
+    ch.q = q;
+    ch.cfg = q >= p.nc ? 1 : 0;
+    if (q < rx_base || q >= rx_base + 8) fill_receive(q);
+    const int slot = uniform(8 * (q - rx_base));  // first lane of channel q in the fill
+    const int src = slot + (ch.cfg == 0 ? rx_pos_rc : rx_pos_cr);
+    ch.rx.i = __shfl(rx_term.i, src, kWave);
+    ch.rx.f = __shfl(rx_term.f, src, kWave);
+    ch.rx.c = __shfl(rx_term.c, src, kWave);
+    ch.rx.s = __shfl(rx_term.s, src, kWave);
+    // Min / max of the receive keys: positions 0-3 and 4-7 (quads), then all 8.
+    const Key hlo[2] = {Key{__builtin_amdgcn_readlane(rx_lo.i, slot),
+                            static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(rx_lo.f), slot))},
+                        Key{__builtin_amdgcn_readlane(rx_lo.i, slot + 4),
+                            static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(rx_lo.f), slot + 4))}};
+    const Key hhi[2] = {Key{__builtin_amdgcn_readlane(rx_hi.i, slot),
+                            static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(rx_hi.f), slot))},
+                        Key{__builtin_amdgcn_readlane(rx_hi.i, slot + 4),
+                            static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(rx_hi.f), slot + 4))}};
     const Key tlo = key_min(hlo[0], hlo[1]), thi = key_max(hhi[0], hhi[1]);
     // Lane a: windows of angle a for the tile and for quarter g.
     const int a = lane < p.na ? lane : 0;
