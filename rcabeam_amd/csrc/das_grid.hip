@@ -5,7 +5,7 @@
 //
 // Work decomposition (one 512-thread workgroup = 8 wave64s, two per CU): a
 // workgroup owns an 8 x 8 lateral patch of voxels at one depth and all
-// frames (tiles ordered depth-fastest, XCD-aware).  Wave w owns voxel row
+// frames (tiles in XCD-aware 4 x 4 x 4 blocks).  Wave w owns voxel row
 // x = w; the 8 lanes l = 8 y + f of voxel (w, y) own the frame pairs
 // (2 f + 16 j, 2 f + 16 j + 1), j < FPP, so one ds_read_b128 of a wave reads
 // 128 contiguous bytes of one sample row per voxel (no LDS bank conflicts
@@ -34,30 +34,25 @@ constexpr int kVMaxRows = 48;
 constexpr int kVMinRows = 8;
 constexpr int kVLdsBytes = 80 * 1024;  // two workgroups per CU
 constexpr int kMaxPairs = 13;          // nt <= 208
+// Each stage buffer is its own LDS variable (36 KiB).  The compiler can then
+// tell that LDS reads of one buffer do not touch the copies in flight into
+// the other, and does not wait for those copies before the reads.
+constexpr int kVStageWords = 9216;
 
-struct VLayout {  // byte offsets inside the dynamic LDS allocation
-  int psum, win, buffer_words, bytes;
-};
+// Dynamic LDS: tile tables at 0, then psum: float[64].
+__host__ __device__ inline int psum_offset(int nc) { return tile_table_bytes(nc); }
+__host__ __device__ inline int dynamic_bytes(int nc) { return psum_offset(nc) + kVTileSlots * 4; }
 
-// Tile tables at 0; psum: float[64]; win: two buffers of max_rows rows x nt
-// complex64 samples plus the frames the last pair block of the last row
-// reads past nt (32-bit words).
-__host__ __device__ inline VLayout vlayout(int nc, int nt, int fpp, int max_rows) {
-  VLayout l;
-  l.psum = tile_table_bytes(nc);
-  l.win = (l.psum + kVTileSlots * 4 + 15) & ~15;
-  l.buffer_words = (2 * (max_rows * nt + 16 * fpp) + 3) & ~3;
-  l.bytes = l.win + 2 * l.buffer_words * 4;
-  return l;
-}
+// Rows of nt complex64 samples a stage buffer holds, with room for the frames
+// the last pair block of the last row reads past nt.
+__host__ __device__ inline int stage_rows(int nt, int fpp) { return (kVStageWords / 2 - 16 * fpp) / nt; }
 
 template <int FPP>
 __global__ __launch_bounds__(kVThreads) __attribute__((amdgpu_waves_per_eu(4))) void das_opw_grid_valu(GridArgs p) {
   extern __shared__ __align__(16) unsigned char lds[];
-  const VLayout layout = vlayout(p.nc, p.nt, FPP, p.max_rows);
-  float* psum = reinterpret_cast<float*>(lds + layout.psum);      // [64]
-  float2* win = reinterpret_cast<float2*>(lds + layout.win);      // [2][buffer_words / 2]
-  const int buffer = layout.buffer_words / 2;                     // complex samples per buffer
+  __shared__ __align__(16) float2 stage_a[kVStageWords / 2];
+  __shared__ __align__(16) float2 stage_b[kVStageWords / 2];
+  float* psum = reinterpret_cast<float*>(lds + psum_offset(p.nc));  // [64]
 
   GridTile<kVThreads> tile(p, lds);
   if (!tile.valid()) return;
@@ -70,10 +65,10 @@ __global__ __launch_bounds__(kVThreads) __attribute__((amdgpu_waves_per_eu(4))) 
   const int nq = 2 * p.nc;
   const int max_rows = p.max_rows;
 
-  {  // both window buffers zeroed once (pairs past nt then read zeros or finite samples)
-    uint4* w4 = reinterpret_cast<uint4*>(win);
-    const int n = layout.buffer_words * 2 / 4;
-    for (int i = tid; i < n; i += kVThreads) w4[i] = make_uint4(0u, 0u, 0u, 0u);
+  // Both stage buffers zeroed once (pairs past nt then read zeros or finite samples).
+  for (int i = tid; i < kVStageWords / 4; i += kVThreads) {
+    reinterpret_cast<uint4*>(stage_a)[i] = make_uint4(0u, 0u, 0u, 0u);
+    reinterpret_cast<uint4*>(stage_b)[i] = make_uint4(0u, 0u, 0u, 0u);
   }
   tile.build_tables();  // ends with a barrier
 
@@ -90,13 +85,13 @@ __global__ __launch_bounds__(kVThreads) __attribute__((amdgpu_waves_per_eu(4))) 
   // samples), 1 KiB per wave instruction spread over the waves.
   auto issue = [&](const Channel& ch, const Stage& s, float2* buf) {
     const float2* plane0 = plane_of(ch);
-    const int a0 = uniform(s.a0), a1 = uniform(s.a1), sr0 = uniform(s.r0);
+    const int a0 = uniform(s.a0), a1 = uniform(s.a1), sr0 = uniform(s.r0), srows = uniform(s.rows);
     int off = 0;
     for (int a = a0; a < a1; ++a) {
       int lo, rows;
       if (sr0 >= 0) {
         lo = sr0;
-        rows = s.rows;
+        rows = srows;
       } else {
         const int2 b = ch.w.at(a);
         if (b.x > b.y) continue;
@@ -111,10 +106,8 @@ __global__ __launch_bounds__(kVThreads) __attribute__((amdgpu_waves_per_eu(4))) 
       for (int i = wave; i < chunks; i += kVWaves) {
         const int so = i * 128 + lane * 2;
         if (so < samples)
-          __builtin_amdgcn_global_load_lds(
-              reinterpret_cast<const void*>(src + so),
-              reinterpret_cast<__attribute__((address_space(3))) void*>(reinterpret_cast<uintptr_t>(dst + i * 128)),
-              16, 0, 0);
+          __builtin_amdgcn_global_load_lds(reinterpret_cast<const void*>(src + so),
+                                           (__attribute__((address_space(3))) void*)(dst + i * 128), 16, 0, 0);
       }
     }
   };
@@ -129,6 +122,7 @@ __global__ __launch_bounds__(kVThreads) __attribute__((amdgpu_waves_per_eu(4))) 
     const int t_lo = cr0 >= 0 ? cr0 : -kBig;
     const int t_hi = cr0 >= 0 && !cur.last ? cr0 + max_rows - 2 : kBig;
     const int tpos = ch.cfg == 0 ? vy : wave;  // transmit position of this lane's voxel
+    const uint32_t row_bytes = static_cast<uint32_t>(nt) * 8u;
     int off = 0;
     for (int a = a0; a < a1; ++a) {
       int first, row_off;
@@ -146,10 +140,12 @@ __global__ __launch_bounds__(kVThreads) __attribute__((amdgpu_waves_per_eu(4))) 
       // A lane-dependent branch around the body (not a `continue`) keeps the
       // angle loop itself wave-uniform.
       if (tp.i0 >= 0 && tp.i0 >= t_lo && tp.i0 <= t_hi) {
-        const int r0 = tp.i0 - first + row_off;
-        const int r1 = tp.w != 0.f ? r0 + 1 : r0;
-        const float4* s0 = reinterpret_cast<const float4*>(base + r0 * nt + 2 * fl);
-        const float4* s1 = reinterpret_cast<const float4*>(base + r1 * nt + 2 * fl);
+        // Row r0 of the staged window (0 <= r0 < 2^24 here, so a 24-bit multiply
+        // gives its byte offset); row r0 + 1 only where the weight is non-zero.
+        const uint32_t r0 = static_cast<uint32_t>(tp.i0 - first + row_off);
+        const unsigned char* row = reinterpret_cast<const unsigned char*>(base) + __umul24(r0, row_bytes) + 16 * fl;
+        const float4* s0 = reinterpret_cast<const float4*>(row);
+        const float4* s1 = reinterpret_cast<const float4*>(row + (tp.w != 0.f ? row_bytes : 0u));
 #pragma unroll
         for (int j = 0; j < FPP; ++j) {
           const float4 va = s0[8 * j], vb = s1[8 * j];  // frames 2 fl + 16 j, + 1
@@ -164,7 +160,8 @@ __global__ __launch_bounds__(kVThreads) __attribute__((amdgpu_waves_per_eu(4))) 
     }
   };
 
-  // ---- stages of all live channels, one barrier per stage
+  // ---- stages of all live channels, one barrier per stage.  Two stages per
+  // loop trip, so each consume reads a buffer known at compile time.
   Stage cur{0, 0, -1, 0, true, false};
   Channel ch;
   ch.q = -1;
@@ -173,11 +170,12 @@ __global__ __launch_bounds__(kVThreads) __attribute__((amdgpu_waves_per_eu(4))) 
     cur = plan_stage(ch.w, 0, p.na, max_rows);
     if (cur.valid) break;
   }
-  int b = 0;
-  if (cur.valid) issue(ch, cur, win);
+  if (cur.valid) issue(ch, cur, stage_a);
   __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
   __syncthreads();
-  while (cur.valid) {
+  // One stage: copy the next stage into `next` while the current one is read
+  // from `now`.
+  auto step = [&](const float2* now, float2* next) {
     Stage nxt = next_stage(cur, ch.w, p.na, max_rows);
     Channel nch = ch;
     bool crossing = false;
@@ -189,13 +187,17 @@ __global__ __launch_bounds__(kVThreads) __attribute__((amdgpu_waves_per_eu(4))) 
       }
       crossing = nxt.valid;
     }
-    if (nxt.valid) issue(crossing ? nch : ch, nxt, win + (b ^ 1) * buffer);
-    consume(ch, cur, win + b * buffer);
+    if (nxt.valid) issue(crossing ? nch : ch, nxt, next);
+    consume(ch, cur, now);
     __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
     __syncthreads();
-    b ^= 1;
     if (crossing) ch = nch;
     cur = nxt;
+  };
+  while (cur.valid) {
+    step(stage_a, stage_b);
+    if (!cur.valid) break;
+    step(stage_b, stage_a);
   }
 
   // ---- epilogue: IQ (frame pairs as float4 where both frames exist) and power
@@ -236,11 +238,10 @@ __global__ __launch_bounds__(kVThreads) __attribute__((amdgpu_waves_per_eu(4))) 
 
 template <int FPP>
 hipError_t launch_pairs(GridArgs a) {
-  const int fixed = vlayout(a.nc, a.nt, FPP, 0).bytes;
-  a.max_rows = std::min(kVMaxRows, (kVLdsBytes - fixed) / (2 * a.nt * 8));
+  a.max_rows = std::min(kVMaxRows, stage_rows(a.nt, FPP));
   if (a.max_rows < kVMinRows) return hipErrorNotSupported;
-  const int bytes = vlayout(a.nc, a.nt, FPP, a.max_rows).bytes;
-  if (bytes > kVLdsBytes) return hipErrorNotSupported;
+  const int bytes = dynamic_bytes(a.nc);
+  if (bytes + 2 * kVStageWords * 4 > kVLdsBytes) return hipErrorNotSupported;
   const long long grid = (a.tiles + 7) / 8 * 8;
   if (grid > 0x7fffffffLL) return hipErrorInvalidValue;
   // 16-byte rows and pair loads need an even frame count and 16-byte aligned planes.
@@ -248,13 +249,6 @@ hipError_t launch_pairs(GridArgs a) {
     return hipErrorNotSupported;
   a.vec = 1;
   auto kernel = das_opw_grid_valu<FPP>;
-  static bool attribute_set = false;  // one opt-in per instantiation (beyond 64 KiB of dynamic LDS)
-  if (!attribute_set) {
-    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
-                                             hipFuncAttributeMaxDynamicSharedMemorySize, kVLdsBytes);
-    if (e != hipSuccess) return e;
-    attribute_set = true;
-  }
   kernel<<<dim3(static_cast<unsigned>(grid)), kVThreads, bytes>>>(a);
   return hipGetLastError();
 }

@@ -18,8 +18,8 @@
 //   kCoefSplit  : hi + lo FP16 coefficients, float32-accurate (2 MFMAs)
 //
 // Work decomposition: one 256-thread workgroup (4 wave64s, two per CU) owns
-// an 8 x 8 lateral patch at one depth and all frames (tiles ordered
-// depth-fastest, XCD-aware); wave g owns the 16 voxels of group g and every
+// an 8 x 8 lateral patch at one depth and all frames (tiles in XCD-aware
+// 4 x 4 x 4 blocks); wave g owns the 16 voxels of group g and every
 // 32-frame block (NB accumulator blocks of 16 floats per lane).
 //
 // On a grid the delay separates: the receive term of RC depends on (x, z,
@@ -158,13 +158,13 @@ __global__ __launch_bounds__(kMThreads) __attribute__((amdgpu_waves_per_eu(2))) 
   // instructions regardless of row boundaries, spread over the waves.
   auto issue = [&](const Channel& ch, const Stage& s, W* buf) {
     const W* plane0 = plane_of(ch);
-    const int a0 = uniform(s.a0), a1 = uniform(s.a1), sr0 = uniform(s.r0);
+    const int a0 = uniform(s.a0), a1 = uniform(s.a1), sr0 = uniform(s.r0), srows = uniform(s.rows);
     int off = 0;
     for (int a = a0; a < a1; ++a) {
       int lo, rows;
       if (sr0 >= 0) {
         lo = sr0;
-        rows = s.rows;
+        rows = srows;
       } else {
         const int2 b = ch.w.at(a);
         if (b.x > b.y) continue;
@@ -242,7 +242,8 @@ __global__ __launch_bounds__(kMThreads) __attribute__((amdgpu_waves_per_eu(2))) 
       for (int r = lo; r <= hi; r += kKRows) {
         const int rel = take ? tp.i0 - r - 4 * kg : 64;
         const u32x4 ahi = place(rel, c0, c1);
-        const W* b0 = base + (r - first + row_off + 4 * kg) * nt + col;
+        // 0 <= row < 2^24: a 24-bit multiply gives the word offset.
+        const W* b0 = base + __umul24(static_cast<uint32_t>(r - first + row_off + 4 * kg), static_cast<uint32_t>(nt)) + col;
         u32x4 bv[NB];
 #pragma unroll
         for (int j = 0; j < NB; ++j) {
