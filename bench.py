@@ -70,6 +70,7 @@ def parse():
     p.add_argument("--grid", type=int, nargs=3, default=(128, 256, 128))
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision comparison runs")
+    p.add_argument("--no-combiners", action="store_true", help="skip the full-size combiner timings")
     p.add_argument("--cpu-voxels", type=int, default=2400)
     p.add_argument("--copy-calibration", action="store_true",
                    help="only run a plain 1 GiB device-to-device copy three times (HBM counter calibration)")
@@ -257,6 +258,28 @@ def cpu_baseline(scene, rc, cr, rank_start, voxels: int):
         "sample": f"{voxels} random voxels of the same 128x256x128 workload, all 200 frames, float64 NumPy oracle "
                   f"(DAS single-threaded; host has {cores} cores), {seconds:.1f} s",
     }
+
+
+def combiner_timing(scene, rc, cr) -> dict:
+    """The combiners on the same full-size acquisition and grid: one `ensemble_pd_from_channels` call per
+    method (pack + delay-and-sum + combiner epilogues; St-SW in voxel tiles bounded by its workspace), channels
+    resident in HBM, timed between synchronizations (one call each: the long kernels dominate)."""
+    import torch
+
+    from rcabeam_amd.ensemble import ensemble_pd_from_channels
+
+    nx, nz, ny = (len(a) for a in scene.grid)
+    frames = rc.shape[-1]
+    out = {}
+    for method, k in (("xdoppler", 2), ("rc_fmas", 2), ("dmas", 2), ("st_sw", 2), ("st_sw", 4)):
+        key = method if method != "st_sw" else f"st_sw_k{k}"
+        torch.cuda.synchronize()
+        begin = time.perf_counter()
+        ensemble_pd_from_channels(rc, cr, scene.angles, scene.t_start, scene.grid, scene.geom, method=method, k=k)
+        torch.cuda.synchronize()
+        seconds = time.perf_counter() - begin
+        out[key] = {"ms": seconds * 1e3, "voxel_frames_per_s": nx * nz * ny * frames / seconds}
+    return out
 
 
 def adjoint_timing(reps: int = 3) -> dict:
@@ -459,6 +482,9 @@ def main() -> int:
         del reference
     log("bench: adjoint / forward on the differentiable-OPW workload")
     record["adjoint"] = adjoint_timing()
+    if not args.no_combiners:
+        log("bench: combiners at full size")
+        record["combiners"] = combiner_timing(scene, rc, cr)
     if not args.no_cpu_baseline:
         log("bench: CPU baseline (oracle) on a voxel sample")
         record["cpu_baseline"] = cpu_baseline(scene, rc, cr, rank_start, args.cpu_voxels)

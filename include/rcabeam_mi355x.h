@@ -80,6 +80,15 @@ rcabeam_status rcabeam_opw_grid_device(const void* rc, const void* cr, int chann
 rcabeam_status rcabeam_unpack_ensemble_device(const void* source, void* target, size_t ns, size_t nc, size_t na,
                                               size_t nt);
 
+/* XDoppler power from separate-aperture IQ: apertures (nv, 2, nt) complex64
+ * (RC then CR per voxel, as rcabeam_opw_grid_device method 5 writes them);
+ * pd[2v] = |mean_t RC_t conj(CR_t)|, pd[2v + 1] = its imaginary part;
+ * signal (optional, (nv, nt) complex64) = RC_t conj(CR_t).  With method 5 on
+ * a grid it gives rcabeam_ensemble_device method 1 (reference
+ * src/rcabeam/ensemble.cu:236-315, 778-780) up to float32 summation order. */
+rcabeam_status rcabeam_xdoppler_from_apertures_device(const void* apertures, size_t nv, size_t nt, float* pd,
+                                                      void* signal);
+
 /* Adjoint (transpose) of rcabeam_opw_grid_device for complex64 channels, the
  * backward pass of a differentiable OPW (no reference counterpart).  rc, cr:
  * the forward's packed complex64 channels; grad_iq: (nv, nt) complex64

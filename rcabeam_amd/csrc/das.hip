@@ -736,20 +736,61 @@ extern "C" rcabeam_status rcabeam_opw_grid_device(const void* rc, const void* cr
                        ? kCoefSingle
                        : accurate_coefficients(channel_storage);
   hipError_t e = hipErrorNotSupported;
-  // OPW: 16-bit channels on MFMA, complex64 channels on the VALU grid kernel;
-  // BF16 channels with float32-accurate coefficients, separate apertures and
-  // shapes those kernels do not take: the VALU tile kernel.
+  // OPW: 16-bit channels on MFMA; OPW and separate apertures of complex64
+  // channels on the VALU grid kernel; BF16 channels with float32-accurate
+  // coefficients and shapes those kernels do not take: the VALU tile kernel.
   if (method == kOpw && channel_storage != RCABEAM_STORAGE_C64)
     e = launch_opw_grid_mfma(p, x, z, y, static_cast<int>(nx), static_cast<int>(nz), static_cast<int>(ny),
                              channel_storage, coef, signal_storage);
-  if (method == kOpw && channel_storage == RCABEAM_STORAGE_C64)
+  if (channel_storage == RCABEAM_STORAGE_C64)
     e = launch_opw_grid_valu(p, x, z, y, static_cast<int>(nx), static_cast<int>(nz), static_cast<int>(ny),
-                             signal_storage);
+                             signal_storage, method == kApertures);
   if (e == hipErrorNotSupported)
     e = launch_opw_grid(p, x, z, y, static_cast<int>(nx), static_cast<int>(nz), static_cast<int>(ny),
                         channel_storage, coef, signal_storage, method == kApertures);
   if (e == hipErrorNotSupported) return RCABEAM_ERROR_ARGUMENT;
   return to_status(e);
+}
+
+namespace rcb {
+namespace {
+
+// XDoppler from separate-aperture IQ (voxels, 2, frames): S_t = RC_t conj(CR_t),
+// PD = |mean_t S| with the imaginary mean in the odd entry, as the XDoppler
+// epilogue of das_angle_major forms them.  One wave per voxel.
+__global__ __launch_bounds__(256) void k_xdoppler_apertures(const float2* __restrict__ ap, long long nv, int nt,
+                                                            float* __restrict__ pd, float2* __restrict__ signal) {
+  const long long v = static_cast<long long>(blockIdx.x) * 4 + threadIdx.x / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
+  if (v >= nv) return;
+  const float2* rc = ap + 2 * v * nt;
+  const float2* cr = rc + nt;
+  float re = 0.f, im = 0.f;
+  for (int t = lane; t < nt; t += kWave) {
+    const float2 value = cmulc(rc[t], cr[t]);
+    if (signal) signal[v * nt + t] = value;
+    re += value.x;
+    im += value.y;
+  }
+  re = wave_sum(re);
+  im = wave_sum(im);
+  if (lane == 0) {
+    const float inv = 1.0f / static_cast<float>(nt);
+    pd[2 * v] = hypotf(re * inv, im * inv);
+    pd[2 * v + 1] = im * inv;
+  }
+}
+
+}  // namespace
+}  // namespace rcb
+
+extern "C" rcabeam_status rcabeam_xdoppler_from_apertures_device(const void* apertures, size_t nv, size_t nt,
+                                                                 float* pd, void* signal) {
+  if (!apertures || !pd || !nv || !nt || nt > 0x7fffffff || nv > (1ull << 40)) return RCABEAM_ERROR_ARGUMENT;
+  const unsigned grid = static_cast<unsigned>((nv + 3) / 4);
+  k_xdoppler_apertures<<<grid, 256>>>(static_cast<const float2*>(apertures), static_cast<long long>(nv),
+                                      static_cast<int>(nt), pd, static_cast<float2*>(signal));
+  return launch_status();
 }
 
 extern "C" rcabeam_status rcabeam_opw_adjoint_grid_device(const void* rc, const void* cr, const float* x,

@@ -19,6 +19,11 @@
 // Per tap and frame: v = a + w (b - a), acc += v e^{i phi} (8 FMA), with the
 // tap (row, weight, phasor) of the lane's own voxel combined from its
 // receive and transmit terms exactly as combine() does.
+//
+// APERTURES (method 5): RC channels come first, so when the first CR channel
+// is reached the sums are the RC IQ; they are stored to plane 0 of the
+// (voxel, aperture, frame) output and restarted, and the CR IQ goes to
+// plane 1 at the end.
 #include "grid_tile.hpp"
 #include "rcabeam_mi355x.h"
 
@@ -47,7 +52,7 @@ __host__ __device__ inline int dynamic_bytes(int nc) { return psum_offset(nc) + 
 // the last pair block of the last row reads past nt.
 __host__ __device__ inline int stage_rows(int nt, int fpp) { return (kVStageWords / 2 - 16 * fpp) / nt; }
 
-template <int FPP>
+template <int FPP, bool APERTURES>
 __global__ __launch_bounds__(kVThreads) __attribute__((amdgpu_waves_per_eu(4))) void das_opw_grid_valu(GridArgs p) {
   extern __shared__ __align__(16) unsigned char lds[];
   __shared__ __align__(16) float2 stage_a[kVStageWords / 2];
@@ -160,6 +165,20 @@ __global__ __launch_bounds__(kVThreads) __attribute__((amdgpu_waves_per_eu(4))) 
     }
   };
 
+  // Apertures: store the sums as plane `cfg` of this lane's voxel and restart them.
+  const long long v = tile.voxel(wave, vy);
+  auto store_plane = [&](int cfg) {
+#pragma unroll
+    for (int j = 0; j < FPP; ++j)
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        const int t = 2 * fl + 16 * j + e;
+        if (t < nt && v >= 0) reinterpret_cast<float2*>(p.signal)[(2 * v + cfg) * nt + t] = acc[2 * j + e];
+        acc[2 * j + e] = make_float2(0.f, 0.f);
+      }
+  };
+  bool rc_stored = false;
+
   // ---- stages of all live channels, one barrier per stage.  Two stages per
   // loop trip, so each consume reads a buffer known at compile time.
   Stage cur{0, 0, -1, 0, true, false};
@@ -188,6 +207,12 @@ __global__ __launch_bounds__(kVThreads) __attribute__((amdgpu_waves_per_eu(4))) 
       crossing = nxt.valid;
     }
     if (nxt.valid) issue(crossing ? nch : ch, nxt, next);
+    if constexpr (APERTURES) {
+      if (!rc_stored && ch.cfg == 1) {  // first CR stage: the RC sums are complete
+        store_plane(0);
+        rc_stored = true;
+      }
+    }
     consume(ch, cur, now);
     __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
     __syncthreads();
@@ -200,8 +225,19 @@ __global__ __launch_bounds__(kVThreads) __attribute__((amdgpu_waves_per_eu(4))) 
     step(stage_b, stage_a);
   }
 
-  // ---- epilogue: IQ (frame pairs as float4 where both frames exist) and power
-  const long long v = tile.voxel(wave, vy);
+  // ---- epilogue: IQ and power (apertures: the last plane, power 0)
+  if constexpr (APERTURES) {
+    if (!rc_stored) store_plane(0);  // no live CR channel: RC sums, then zeros
+    store_plane(1);
+    if (tid < kVTileSlots) {
+      const long long vv = tile.voxel(tid >> 3, tid & 7);
+      if (vv >= 0) {
+        p.pd[2 * vv] = 0.f;
+        p.pd[2 * vv + 1] = 0.f;
+      }
+    }
+    return;
+  }
   float power = 0.f;
 #pragma unroll
   for (int j = 0; j < FPP; ++j) {
@@ -236,7 +272,7 @@ __global__ __launch_bounds__(kVThreads) __attribute__((amdgpu_waves_per_eu(4))) 
   }
 }
 
-template <int FPP>
+template <int FPP, bool APERTURES>
 hipError_t launch_pairs(GridArgs a) {
   a.max_rows = std::min(kVMaxRows, stage_rows(a.nt, FPP));
   if (a.max_rows < kVMinRows) return hipErrorNotSupported;
@@ -248,25 +284,31 @@ hipError_t launch_pairs(GridArgs a) {
   if (a.nt % 2 != 0 || reinterpret_cast<uintptr_t>(a.rc) % 16 || reinterpret_cast<uintptr_t>(a.cr) % 16)
     return hipErrorNotSupported;
   a.vec = 1;
-  auto kernel = das_opw_grid_valu<FPP>;
+  auto kernel = das_opw_grid_valu<FPP, APERTURES>;
   kernel<<<dim3(static_cast<unsigned>(grid)), kVThreads, bytes>>>(a);
   return hipGetLastError();
+}
+
+template <bool APERTURES>
+hipError_t launch_frames(const GridArgs& a) {
+  const int pairs = (a.nt + 15) / 16;
+  if (pairs <= 1) return launch_pairs<1, APERTURES>(a);
+  if (pairs <= 2) return launch_pairs<2, APERTURES>(a);
+  if (pairs <= 4) return launch_pairs<4, APERTURES>(a);
+  if (pairs <= 7) return launch_pairs<7, APERTURES>(a);
+  return launch_pairs<kMaxPairs, APERTURES>(a);
 }
 
 }  // namespace
 
 hipError_t launch_opw_grid_valu(const EnsembleArgs& p, const float* x, const float* z, const float* y, int nx,
-                                int nz, int ny, int signal_storage) {
+                                int nz, int ny, int signal_storage, bool apertures) {
   if (p.na > kGridMaxAngles || p.nt > 16 * kMaxPairs) return hipErrorNotSupported;
   if (static_cast<long long>(p.ns) * p.nt > 0x7fffffffLL) return hipErrorNotSupported;
   if (signal_storage < RCABEAM_STORAGE_C64 || signal_storage > RCABEAM_STORAGE_BF16) return hipErrorInvalidValue;
+  if (apertures && !p.signal) return hipErrorInvalidValue;  // apertures: complex64 planes whatever signal_storage
   const GridArgs a = make_grid_args(p, x, z, y, nx, nz, ny, signal_storage);
-  const int pairs = (p.nt + 15) / 16;
-  if (pairs <= 1) return launch_pairs<1>(a);
-  if (pairs <= 2) return launch_pairs<2>(a);
-  if (pairs <= 4) return launch_pairs<4>(a);
-  if (pairs <= 7) return launch_pairs<7>(a);
-  return launch_pairs<kMaxPairs>(a);
+  return apertures ? launch_frames<true>(a) : launch_frames<false>(a);
 }
 
 }  // namespace rcb

@@ -63,8 +63,9 @@ hipError_t launch_opw_grid_mfma(const EnsembleArgs& p, const float* x, const flo
 // interpolation (das_grid.hip): analytic windows, live-channel skipping.
 // Same results as launch_opw_grid up to float32 summation order;
 // hipErrorNotSupported for na > 16, nt > 208, odd nt or unaligned channels.
+// apertures: separate RC and CR IQ into signal (voxels, 2, nt) complex64.
 hipError_t launch_opw_grid_valu(const EnsembleArgs& p, const float* x, const float* z, const float* y, int nx,
-                                int nz, int ny, int signal_storage);
+                                int nz, int ny, int signal_storage, bool apertures);
 
 // Adjoint of the grid OPW for complex64 channels (das_tile.hip): accumulates
 // the packed channel gradients grc / gcr and dL/dc (*gc) for the IQ gradient

@@ -131,7 +131,9 @@ def run_ensemble(packed_rc, packed_cr, g: _Geometry, shape4, method: int, k: int
     """Enqueue one reconstruction of voxels [first, first + count) over the C ABI.
 
     OPW and separate-aperture IQ of a whole grid run rcabeam_opw_grid_device
-    (8 x 8 lateral tiles; same per-voxel arithmetic as the voxel-list path).
+    (8 x 8 lateral tiles; the same taps as the voxel-list path); XDoppler of a
+    whole grid runs its separate-aperture form followed by
+    rcabeam_xdoppler_from_apertures_device.
     Otherwise float32 channels run rcabeam_ensemble_device and 16-bit channels
     (OPW only) rcabeam_opw_mixed_device, with float32-accurate coefficients.
     """
@@ -140,15 +142,25 @@ def run_ensemble(packed_rc, packed_cr, g: _Geometry, shape4, method: int, k: int
     count = g.nv - first if count is None else count
     c, fs, fd, fn = g.constants
     whole_grid = first == 0 and count == g.nv and g.canonical
-    if whole_grid and method in (_METHODS["opw"], _METHODS["apertures"]) and na <= GRID_MAX_ANGLES \
-            and nt <= GRID_MAX_FRAMES:
+    grid_shape_ok = whole_grid and na <= GRID_MAX_ANGLES and nt <= GRID_MAX_FRAMES
+
+    def grid_call(grid_method, grid_pd, grid_signal, grid_signal_storage):
         nx, nz, ny = g.shape
         _lib.call(
             "rcabeam_opw_grid_device", dev.ptr(packed_rc), dev.ptr(packed_cr), _lib.STORAGE[storage],
             _lib.COEF_FLOAT32, dev.ptr(g.axes[0]), dev.ptr(g.axes[1]), dev.ptr(g.axes[2]), nx, nz, ny,
-            dev.ptr(g.xe), dev.ptr(g.ye), dev.ptr(g.angles), dev.ptr(g.starts), dev.ptr(pd), dev.ptr(signal),
-            signal_storage, method, ns, nc, na, nt, c, fs, fd, fn,
+            dev.ptr(g.xe), dev.ptr(g.ye), dev.ptr(g.angles), dev.ptr(g.starts), dev.ptr(grid_pd),
+            dev.ptr(grid_signal), grid_signal_storage, grid_method, ns, nc, na, nt, c, fs, fd, fn,
         )
+
+    if grid_shape_ok and method in (_METHODS["opw"], _METHODS["apertures"]):
+        grid_call(method, pd, signal, signal_storage)
+        return
+    if grid_shape_ok and method == _METHODS["xdoppler"] and storage == "float32":
+        # XDoppler of a whole grid: separate-aperture IQ on the grid kernel, then the cross product.
+        planes = dev.empty((2 * g.nv, nt), t.complex64)
+        grid_call(_METHODS["apertures"], pd, planes, 0)
+        _lib.call("rcabeam_xdoppler_from_apertures_device", dev.ptr(planes), g.nv, nt, dev.ptr(pd), dev.ptr(signal))
         return
     scan = g.scan[first : first + count]
     pd_view = pd[first : first + count]

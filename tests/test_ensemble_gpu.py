@@ -265,22 +265,25 @@ def test_sixteen_bit_coefficients_error(storage):
     assert e_pd < (1e-3 if storage == "float16" else 8e-3)
 
 
+@pytest.mark.parametrize("shape", [(13, 20, 11), (9, 7, 17)])
 @pytest.mark.parametrize("storage,coefficients", [("float32", 0), ("float16", 0), ("float16", 1), ("bfloat16", 1)])
-def test_grid_tiles_equal_voxel_list(storage, coefficients):
+def test_grid_tiles_equal_voxel_list(storage, coefficients, shape):
     """rcabeam_opw_grid_device against the voxel-list entry point on the same packed channels.  complex64
     channels: the grid kernel (das_grid.hip) accumulates taps in the voxel-list kernel's order, with its
     phasor products rounded independently (rel L2 < 1e-6).  16-bit channels: the grid runs the MFMA
     kernel, whose products are the same exact 16-bit products summed in another order; float32 rounding
     of ~770-term sums in two orders differs by ~2^-24 sqrt(770) ~ 2e-6 relative (bound 5e-6; both stay
     within 1e-5 of the float64 oracle, test_fp16_opw_on_quantized_point_scene).  Grid sizes that are not
-    multiples of the patch leave partial tiles."""
+    multiples of the patch leave partial tiles, and tile counts that are not multiples of the 4 x 4 x 4
+    order blocks leave empty tile slots."""
     import torch
 
     from rcabeam_amd import _device as dev
     from rcabeam_amd import _lib
     from rcabeam_amd.ensemble import _Geometry, pack_channels
 
-    acq = scene.point_scene(grid_shape=(13, 20, 11))
+    print(f"case: grid vs list {storage} coef={coefficients} shape={shape}", flush=True)
+    acq = scene.point_scene(grid_shape=shape)
     lib = _lib.load()
     g = _Geometry(acq.angles, acq.t_start, acq.grid, _geom(acq))
     ns, nc, na, nt = acq.rc.shape
@@ -309,6 +312,62 @@ def test_grid_tiles_equal_voxel_list(storage, coefficients):
     assert rel_l2(out[1][0], out[0][0]) < bound
     assert rel_l2(out[1][1], out[0][1]) < bound
     np.testing.assert_array_equal(out[1][1][:, 1], 0.0)
+
+
+@pytest.mark.parametrize("frames", [16, 200])
+def test_grid_apertures_and_xdoppler_equal_voxel_list(frames):
+    """Separate-aperture IQ (method 5) of the grid kernel against rcabeam_ensemble_device method 5 on the
+    scan list, and XDoppler formed from it by rcabeam_xdoppler_from_apertures_device against method 1
+    (reference ensemble.cu:236-315): the same taps summed in the same order per aperture (rel L2 < 1e-6)."""
+    import torch
+
+    from rcabeam_amd import _device as dev
+    from rcabeam_amd import _lib
+    from rcabeam_amd.ensemble import _Geometry, pack_channels
+
+    print(f"case: grid apertures / XDoppler frames={frames}", flush=True)
+    acq = scene.random_acquisition(samples=64, channels=8, frames=frames,
+                                   grid=(np.linspace(-3e-3, 3e-3, 13).astype(np.float32),
+                                         np.linspace(4e-3, 9e-3, 10).astype(np.float32),
+                                         np.linspace(-3e-3, 3e-3, 11).astype(np.float32)))
+    lib = _lib.load()
+    g = _Geometry(acq.angles, acq.t_start, acq.grid, _geom(acq))
+    ns, nc, na, nt = acq.rc.shape
+    packed = [pack_channels(dev.to_device(x, torch.complex64), "float32") for x in (acq.rc, acq.cr)]
+    c, fs, fd, fn = g.constants
+    nx, nz, ny = g.shape
+
+    def planes(on_grid):
+        pd = torch.full((g.nv, 2), float("nan"), device="cuda")
+        sig = torch.full((2 * g.nv, nt), float("nan"), dtype=torch.complex64, device="cuda")
+        if on_grid:
+            _lib.check(lib.rcabeam_opw_grid_device(
+                dev.ptr(packed[0]), dev.ptr(packed[1]), 0, 0, dev.ptr(g.axes[0]), dev.ptr(g.axes[1]),
+                dev.ptr(g.axes[2]), nx, nz, ny, dev.ptr(g.xe), dev.ptr(g.ye), dev.ptr(g.angles), dev.ptr(g.starts),
+                dev.ptr(pd), dev.ptr(sig), 0, 5, ns, nc, na, nt, c, fs, fd, fn), "grid")
+        else:
+            _lib.check(lib.rcabeam_ensemble_device(
+                dev.ptr(packed[0]), dev.ptr(packed[1]), dev.ptr(g.scan), dev.ptr(g.xe), dev.ptr(g.ye),
+                dev.ptr(g.angles), dev.ptr(g.starts), dev.ptr(pd), dev.ptr(sig), None, None,
+                dev.ptr(g.geometry_workspace), ns, nc, na, nt, g.nv, c, fs, fd, fn, 5, 2), "list")
+        return pd, sig
+
+    (pd_g, ap_g), (_, ap_l) = planes(True), planes(False)
+    assert np.isfinite(ap_g.cpu().numpy()).all()
+    assert rel_l2(ap_g.cpu().numpy(), ap_l.cpu().numpy()) < 1e-6
+    np.testing.assert_array_equal(pd_g.cpu().numpy(), 0.0)
+    xd_pd = torch.full((g.nv, 2), float("nan"), device="cuda")
+    xd_sig = torch.full((g.nv, nt), float("nan"), dtype=torch.complex64, device="cuda")
+    _lib.check(lib.rcabeam_xdoppler_from_apertures_device(dev.ptr(ap_g), g.nv, nt, dev.ptr(xd_pd),
+                                                          dev.ptr(xd_sig)), "xdoppler")
+    ref_pd = torch.full((g.nv, 2), float("nan"), device="cuda")
+    ref_sig = torch.full((g.nv, nt), float("nan"), dtype=torch.complex64, device="cuda")
+    _lib.check(lib.rcabeam_ensemble_device(
+        dev.ptr(packed[0]), dev.ptr(packed[1]), dev.ptr(g.scan), dev.ptr(g.xe), dev.ptr(g.ye), dev.ptr(g.angles),
+        dev.ptr(g.starts), dev.ptr(ref_pd), dev.ptr(ref_sig), None, None, dev.ptr(g.geometry_workspace), ns, nc,
+        na, nt, g.nv, c, fs, fd, fn, 1, 2), "list xdoppler")
+    assert rel_l2(xd_sig.cpu().numpy(), ref_sig.cpu().numpy()) < 1e-6
+    assert rel_l2(xd_pd.cpu().numpy(), ref_pd.cpu().numpy()) < 1e-6
 
 
 def test_grid_entry_rejects_unsupported_shapes():
