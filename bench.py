@@ -279,6 +279,15 @@ def combiner_timing(scene, rc, cr) -> dict:
         torch.cuda.synchronize()
         seconds = time.perf_counter() - begin
         out[key] = {"ms": seconds * 1e3, "voxel_frames_per_s": nx * nz * ny * frames / seconds}
+    # Clutter-filtered XDoppler (separate apertures, shared RC/CR basis, rank_start 3).
+    from rcabeam_amd.ensemble import filtered_xdoppler_pd
+
+    torch.cuda.synchronize()
+    begin = time.perf_counter()
+    filtered_xdoppler_pd(rc, cr, scene.angles, scene.t_start, scene.grid, scene.geom, rank_start=3)
+    torch.cuda.synchronize()
+    seconds = time.perf_counter() - begin
+    out["xdoppler_filtered"] = {"ms": seconds * 1e3, "voxel_frames_per_s": nx * nz * ny * frames / seconds}
     return out
 
 

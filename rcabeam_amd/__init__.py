@@ -11,6 +11,7 @@ from rcabeam_amd.ensemble import (
     beamform_apertures,
     beamform_ensemble,
     ensemble_pd_from_channels,
+    filtered_xdoppler_pd,
     opw_voxel_order,
 )
 from rcabeam_amd.sim import RCAGeometry, delay_rca_channel_data, delay_rca_channels, depth_axis
@@ -24,5 +25,6 @@ __all__ = [
     "delay_rca_channels",
     "depth_axis",
     "ensemble_pd_from_channels",
+    "filtered_xdoppler_pd",
     "opw_voxel_order",
 ]

@@ -258,6 +258,36 @@ def beamform_ensemble(rc, cr, angles, t_start, grid, geom: RCAGeometry, *,
     return _run(rc, cr, angles, t_start, grid, geom, method, 2, True, timings, iq_storage)[1]
 
 
+def filtered_xdoppler_pd(rc, cr, angles, t_start, grid, geom: RCAGeometry, *, rank_start, rank_stop=None,
+                         precision: str = "complex128"):
+    """Clutter-filtered XDoppler power `(nx, nz, ny)` float32, the recipe of reference README.md:33:
+    separately compounded RC and CR IQ (`beamform_apertures`), one temporal basis shared by both apertures
+    (the Gram of both Casorati matrices, as the reference's RC/CR tuple input to `temporal_basis`,
+    gpu_clutter.py:122-127), the singular-vector interval [rank_start, rank_stop) kept in both, then
+    |mean_t RC conj(CR)|.  Everything stays on the GPU: the separate-aperture grid kernel, one Gram over the
+    (voxel, aperture) rows, the float64 EVD, one filter pass and `rcabeam_xdoppler_from_apertures_device`."""
+    from rcabeam_amd import clutter
+
+    _validate(rc, cr, angles, grid, geom, "apertures", 2, True, "float32")
+    t = dev.torch()
+    host_input = not (dev.is_tensor(rc) and dev.is_tensor(cr))
+    shape = (len(grid[0]), len(grid[1]), len(grid[2]))
+    g = _Geometry(angles, t_start, grid, geom)
+    ns, nc, na, nt = tuple(rc.shape)
+    packed = [pack_channels(dev.to_device(x, t.complex64), "float32") for x in (rc, cr)]
+    planes = dev.empty((2 * g.nv, nt), t.complex64)  # (voxel, aperture) rows
+    scratch = dev.empty((g.nv, 2), t.float32)
+    run_ensemble(packed[0], packed[1], g, (ns, nc, na, nt), _METHODS["apertures"], 2, "float32", scratch, planes)
+    del packed
+    _, basis = clutter.temporal_basis(planes, precision=precision)
+    filtered = clutter.apply_temporal_filter(planes, basis, rank_start=rank_start, rank_stop=rank_stop)
+    del planes
+    pd = dev.empty((g.nv, 2), t.float32)
+    _lib.call("rcabeam_xdoppler_from_apertures_device", dev.ptr(filtered), g.nv, nt, dev.ptr(pd), None)
+    power = pd[:, 0].reshape(shape)
+    return dev.host(power) if host_input else power
+
+
 def beamform_apertures(rc, cr, angles, t_start, grid, geom: RCAGeometry):
     """Separately compounded RC and CR IQ, each `(nx, nz, ny, frames)` (ensemble.py:273-287)."""
     _, iq = _run(rc, cr, angles, t_start, grid, geom, "apertures", 2, True, None, "float32")
