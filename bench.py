@@ -291,6 +291,31 @@ def combiner_timing(scene, rc, cr) -> dict:
     return out
 
 
+def ingest_timing(rc) -> dict:
+    """Host-to-device upload of one aperture's complex64 channels (pageable NumPy array): torch's
+    pageable copy against the library's pinned staging (rcabeam_upload_host), best of 3 each."""
+    import torch
+
+    from rcabeam_amd import _device as dev
+
+    host = rc.cpu().numpy()
+    seconds = {}
+    for name, fn in (("pageable", lambda: torch.from_numpy(host).to("cuda")),
+                     ("staged", lambda: dev.upload(host, torch.complex64))):
+        best = float("inf")
+        for _ in range(3):
+            torch.cuda.synchronize()
+            begin = time.perf_counter()
+            out = fn()
+            torch.cuda.synchronize()
+            best = min(best, time.perf_counter() - begin)
+            del out
+        seconds[name] = best
+    return {"bytes": host.nbytes, "threads": dev.upload_threads(),
+            "pageable_GBps": host.nbytes / seconds["pageable"] / 1e9,
+            "staged_GBps": host.nbytes / seconds["staged"] / 1e9}
+
+
 def adjoint_timing(reps: int = 3) -> dict:
     """Differentiable-OPW workload: adjoint kernel time / forward kernel time.
 
@@ -494,6 +519,8 @@ def main() -> int:
     if not args.no_combiners:
         log("bench: combiners at full size")
         record["combiners"] = combiner_timing(scene, rc, cr)
+        log("bench: host ingest")
+        record["host_ingest"] = ingest_timing(rc)
     if not args.no_cpu_baseline:
         log("bench: CPU baseline (oracle) on a voxel sample")
         record["cpu_baseline"] = cpu_baseline(scene, rc, cr, rank_start, args.cpu_voxels)

@@ -80,6 +80,16 @@ rcabeam_status rcabeam_opw_grid_device(const void* rc, const void* cr, int chann
 rcabeam_status rcabeam_unpack_ensemble_device(const void* source, void* target, size_t ns, size_t nc, size_t na,
                                               size_t nt);
 
+/* Host ingest (no reference counterpart; replaces the pageable upload of
+ * src/rcabeam/bindings/python.cpp:527-531): copy `bytes` from pageable host
+ * memory to device memory through two 64 MiB pinned staging buffers, host
+ * threads filling one buffer while the other drains to the device.  threads:
+ * host copy threads (1-64).  Returns after the copy has completed.  The
+ * staging buffers and a stream are created per device on first use and kept
+ * until the process ends (128 MiB of pinned host memory per device); calls on
+ * one device are serialized. */
+rcabeam_status rcabeam_upload_host(const void* host, void* device, size_t bytes, int threads);
+
 /* XDoppler power from separate-aperture IQ: apertures (nv, 2, nt) complex64
  * (RC then CR per voxel, as rcabeam_opw_grid_device method 5 writes them);
  * pd[2v] = |mean_t RC_t conj(CR_t)|, pd[2v + 1] = its imaginary part;

@@ -34,6 +34,29 @@ def device():
     return t.device("cuda", t.cuda.current_device())
 
 
+# Host arrays at least this large go through the library's pinned staging (rcabeam_upload_host).
+STAGED_UPLOAD_BYTES = 32 << 20
+
+
+def upload_threads() -> int:
+    """Host copy threads for staged uploads: the CPUs this process may run on, at most 16."""
+    import os
+
+    return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+def upload(arr: np.ndarray, dtype) -> Any:
+    """Device tensor of `dtype` holding the host array `arr`, copied through pinned staging buffers
+    (rcabeam_upload_host: threaded host copies overlapped with the host-to-device DMA)."""
+    from rcabeam_amd import _lib
+
+    t = torch()
+    host = np.ascontiguousarray(arr, dtype=t.empty((), dtype=dtype).numpy().dtype)
+    out = t.empty(host.shape, dtype=dtype, device=device())
+    _lib.call("rcabeam_upload_host", host.ctypes.data, out.data_ptr(), host.nbytes, upload_threads())
+    return out
+
+
 def to_device(x: Any, dtype) -> Any:
     """Contiguous device tensor of `dtype` from a NumPy array or a tensor (copied only if needed)."""
     t = torch()
@@ -41,7 +64,10 @@ def to_device(x: Any, dtype) -> Any:
     if isinstance(x, t.Tensor):
         y = x.to(device=dev, dtype=dtype)
         return y.contiguous()
-    arr = np.ascontiguousarray(x)
+    arr = np.asarray(x)
+    if arr.nbytes >= STAGED_UPLOAD_BYTES and arr.dtype.kind in "fc":
+        return upload(arr, dtype)
+    arr = np.ascontiguousarray(arr)
     return t.from_numpy(arr).to(device=dev, dtype=dtype, non_blocking=False).contiguous()
 
 
