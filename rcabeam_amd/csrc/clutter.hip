@@ -18,6 +18,7 @@
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
+#include <algorithm>
 #include <mutex>
 
 #include "common.hpp"
@@ -419,6 +420,91 @@ __global__ __launch_bounds__(256) void k_project(const T* __restrict__ x, long l
   }
 }
 
+// The same projection for short rows (frames <= 256) and few vectors (r <= 8),
+// the clutter filter's usual shape: 8 lanes per row (lane f holds frames
+// f + 8 j), 8 rows per wave, so the three coordinate reductions per vector
+// stay inside 8 lanes; the vectors are staged in LDS once per workgroup.  The
+// residual x_t - sum_k c_k conj(V[t, k]) is formed frame by frame in float64
+// from the float32 row kept in registers.
+constexpr int kProjectRowsPerBlock = 32;  // 256 threads, 8 lanes per row
+constexpr int kProjectMaxR = 8;
+
+template <typename T, bool FILTER, int kPer>
+__global__ __launch_bounds__(256) void k_project_rows8(const T* __restrict__ x, long long rows, int frames,
+                                                       const double2* __restrict__ v, int r, int subtract,
+                                                       double* __restrict__ power, float2* __restrict__ filtered) {
+  extern __shared__ double2 vs[];  // [r][frames]
+  for (int i = threadIdx.x; i < r * frames; i += blockDim.x) vs[i] = v[i];
+  __syncthreads();
+  const long long row = static_cast<long long>(blockIdx.x) * kProjectRowsPerBlock + threadIdx.x / 8;
+  const int f = threadIdx.x & 7;
+  if (row >= rows) return;  // after the only barrier
+  const T* xr = x + row * frames;
+  float2 xv[kPer];
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const int t = f + 8 * j;
+    xv[j] = t < frames ? load_sample(xr + t) : make_float2(0.f, 0.f);
+  }
+  // Coordinates c_k = sum_t x_t V[t, k] of this row (all 8 lanes hold them after the reduction).
+  double cre[kProjectMaxR], cim[kProjectMaxR];
+  double kept = 0.0;
+#pragma unroll
+  for (int k = 0; k < kProjectMaxR; ++k) {
+    cre[k] = cim[k] = 0.0;
+    if (k < r) {
+      const double2* vk = vs + k * frames;
+#pragma unroll
+      for (int j = 0; j < kPer; ++j) {
+        const int t = f + 8 * j;
+        if (t < frames) {
+          const double2 b = vk[t];
+          const double xr_ = xv[j].x, xi_ = xv[j].y;
+          cre[k] += xr_ * b.x - xi_ * b.y;
+          cim[k] += xr_ * b.y + xi_ * b.x;
+        }
+      }
+#pragma unroll
+      for (int off = 1; off < 8; off <<= 1) {
+        cre[k] += __shfl_xor(cre[k], off, kWave);
+        cim[k] += __shfl_xor(cim[k], off, kWave);
+      }
+      kept += cre[k] * cre[k] + cim[k] * cim[k];
+    }
+  }
+  if (!FILTER && !subtract) {  // retained energy |c|^2 directly
+    if (f == 0) power[row] = kept / frames;
+    return;
+  }
+  const double sign = subtract ? -1.0 : 1.0;
+  double e = 0.0;
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const int t = f + 8 * j;
+    if (t < frames) {
+      double re = subtract ? static_cast<double>(xv[j].x) : 0.0;
+      double im = subtract ? static_cast<double>(xv[j].y) : 0.0;
+#pragma unroll
+      for (int k = 0; k < kProjectMaxR; ++k) {
+        if (k < r) {
+          const double2 b = vs[k * frames + t];  // c_k conj(V[t, k]) (row-vector convention x V V^H)
+          re += sign * (cre[k] * b.x + cim[k] * b.y);
+          im += sign * (cim[k] * b.x - cre[k] * b.y);
+        }
+      }
+      if constexpr (FILTER)
+        filtered[row * frames + t] = make_float2(static_cast<float>(re), static_cast<float>(im));
+      else
+        e += re * re + im * im;
+    }
+  }
+  if constexpr (!FILTER) {
+#pragma unroll
+    for (int off = 1; off < 8; off <<= 1) e += __shfl_xor(e, off, kWave);
+    if (f == 0) power[row] = e / frames;
+  }
+}
+
 // ---- host helpers -----------------------------------------------------------
 
 static int column_blocks(size_t frames) { return static_cast<int>((2 * frames + 31) / 32); }
@@ -627,6 +713,31 @@ static rcabeam_status project(const void* iq, int storage, size_t rows, size_t f
   const double2* v = static_cast<const double2*>(vectors);
   float2* out = static_cast<float2*>(filtered);
   const int ri = static_cast<int>(r);
+  if (frames <= 256 && r <= static_cast<size_t>(kProjectMaxR)) {
+    // Short rows, few vectors: 8 lanes per row, vectors in LDS.
+    const unsigned blocks = static_cast<unsigned>((rows + kProjectRowsPerBlock - 1) / kProjectRowsPerBlock);
+    const size_t lds = std::max<size_t>(r * frames * sizeof(double2), sizeof(double2));
+#define RCB_PROJECT8(TYPE, PER)                                                                             \
+  k_project_rows8<TYPE, FILTER, PER><<<blocks, 256, lds>>>(static_cast<const TYPE*>(iq), n, t, v, ri, subtract, \
+                                                            power, out)
+#define RCB_PROJECT8_SHAPES(TYPE)              \
+  if (frames <= 64) RCB_PROJECT8(TYPE, 8);      \
+  else if (frames <= 128) RCB_PROJECT8(TYPE, 16); \
+  else RCB_PROJECT8(TYPE, 32)
+    switch (storage) {
+      case RCABEAM_STORAGE_C64:
+        RCB_PROJECT8_SHAPES(float2);
+        break;
+      case RCABEAM_STORAGE_FP16:
+        RCB_PROJECT8_SHAPES(__half2);
+        break;
+      default:
+        RCB_PROJECT8_SHAPES(bf16x2);
+    }
+#undef RCB_PROJECT8_SHAPES
+#undef RCB_PROJECT8
+    return launch_status();
+  }
 #define RCB_PROJECT(TYPE, MPL) \
   k_project<TYPE, FILTER, MPL><<<grid, 256>>>(static_cast<const TYPE*>(iq), n, t, v, ri, subtract, power, out)
   const bool small = frames <= 4 * kWave;
