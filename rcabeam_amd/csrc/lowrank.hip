@@ -21,6 +21,8 @@
 // to the rounding of e and of w; a V that captures the clutter makes e small.
 // (Tap-coefficient rounding is the same for every frame column, so it maps the
 // clutter's temporal subspace onto itself and the filter still removes it.)
+#include <algorithm>
+
 #include "common.hpp"
 #include "rcabeam_mi355x.h"
 #include "tile_staging.hpp"
@@ -29,7 +31,7 @@ namespace rcb {
 namespace {
 
 constexpr int kMaxBasis = 12;        // k <= 12 keeps nt + 2k <= 224 for nt = 200
-constexpr int kMaxFramesLR = 256;    // 4 frames per lane
+constexpr int kMaxFramesLR = 256;    // 32 frames per lane, 8 lanes per row
 
 template <typename T>
 __device__ __forceinline__ bool in_range(float v);
@@ -42,130 +44,143 @@ __device__ __forceinline__ bool in_range<bf16x2>(float v) {
   return fabsf(v) <= 3.3895313892515355e38f;
 }
 
-// One wave per target (channel, angle, sample) row: frames t = lane + 64 j.
-// Coefficients and residual in float64 (the row is read once; HBM-bound).
+// Eight lanes per target (channel, angle, sample) row: lane f holds frames
+// f + 8 j, so the k coordinate reductions stay inside 8 lanes; 32 rows per
+// 256-thread workgroup, the basis staged in LDS.  Coordinates and residual in
+// float64 (the row is read once; HBM-bound).
+constexpr int kPackRowsPerBlock = 32;
+constexpr int kPerLaneLR = kMaxFramesLR / 8;  // 32 frames per lane
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_pack_lowrank(const float2* __restrict__ src, uint32_t* __restrict__ dst,
                                                       const double2* __restrict__ basis, int k, int* range_error,
                                                       int ns, int nc, int na, int nt, long long rows) {
-  const long long row = static_cast<long long>(blockIdx.x) * 4 + threadIdx.x / kWave;
-  const int lane = threadIdx.x & (kWave - 1);
-  if (row >= rows) return;
-  const long long s = row % ns;
-  const long long ca = row / ns;
-  const long long a = ca % na;
-  const long long ch = ca / na;
-  const float2* in = src + ((s * nc + ch) * na + a) * nt;
-  const int ntc = nt + 2 * k;
-  uint32_t* out = dst + row * ntc;
-  double2 x[4];
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int t = lane + kWave * j;
-    const float2 v = t < nt ? in[t] : make_float2(0.f, 0.f);
-    x[j] = make_double2(v.x, v.y);
-  }
-  double2 e[4] = {x[0], x[1], x[2], x[3]};
-  double2 w_mine = make_double2(0.0, 0.0);  // lane j < k keeps w_j
-  for (int q = 0; q < k; ++q) {
-    const double2* vq = basis + static_cast<long long>(q) * nt;
-    double cr = 0.0, ci = 0.0;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int t = lane + kWave * j;
-      if (t < nt) {
-        const double2 b = vq[t];  // w_q = sum_t c_t V[t, q]
-        cr += x[j].x * b.x - x[j].y * b.y;
-        ci += x[j].x * b.y + x[j].y * b.x;
-      }
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      cr += __shfl_xor(cr, off, kWave);
-      ci += __shfl_xor(ci, off, kWave);
-    }
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int t = lane + kWave * j;
-      if (t < nt) {
-        const double2 b = vq[t];  // e_t -= w_q conj(V[t, q])
-        e[j].x -= cr * b.x + ci * b.y;
-        e[j].y -= ci * b.x - cr * b.y;
-      }
-    }
-    if (lane == q) w_mine = make_double2(cr, ci);
-  }
+  extern __shared__ double2 vs[];  // [k][nt]
+  for (int i = threadIdx.x; i < k * nt; i += blockDim.x) vs[i] = basis[i];
+  __syncthreads();
+  const long long row = static_cast<long long>(blockIdx.x) * kPackRowsPerBlock + threadIdx.x / 8;
+  const int f = threadIdx.x & 7;
   bool bad = false;
+  if (row < rows) {
+    const long long s = row % ns;
+    const long long ca = row / ns;
+    const long long a = ca % na;
+    const long long ch = ca / na;
+    const float2* in = src + ((s * nc + ch) * na + a) * nt;
+    const int ntc = nt + 2 * k;
+    uint32_t* out = dst + row * ntc;
+    float2 x[kPerLaneLR];
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int t = lane + kWave * j;
-    if (t < nt) {
-      const float re = static_cast<float>(e[j].x), im = static_cast<float>(e[j].y);
-      bad |= !in_range<T>(re) || !in_range<T>(im);
-      out[t] = pack2<T>(re, im);
+    for (int j = 0; j < kPerLaneLR; ++j) {
+      const int t = f + 8 * j;
+      x[j] = t < nt ? in[t] : make_float2(0.f, 0.f);
+    }
+    double wr[kMaxBasis], wi[kMaxBasis];  // w_q = sum_t c_t V[t, q], in every lane of the row
+#pragma unroll
+    for (int q = 0; q < kMaxBasis; ++q) {
+      wr[q] = wi[q] = 0.0;
+      if (q < k) {
+#pragma unroll
+        for (int j = 0; j < kPerLaneLR; ++j) {
+          const int t = f + 8 * j;
+          if (t < nt) {
+            const double2 b = vs[q * nt + t];
+            wr[q] += static_cast<double>(x[j].x) * b.x - static_cast<double>(x[j].y) * b.y;
+            wi[q] += static_cast<double>(x[j].x) * b.y + static_cast<double>(x[j].y) * b.x;
+          }
+        }
+#pragma unroll
+        for (int off = 1; off < 8; off <<= 1) {
+          wr[q] += __shfl_xor(wr[q], off, kWave);
+          wi[q] += __shfl_xor(wi[q], off, kWave);
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kPerLaneLR; ++j) {
+      const int t = f + 8 * j;
+      if (t < nt) {  // e_t = c_t - sum_q w_q conj(V[t, q])
+        double er = x[j].x, ei = x[j].y;
+#pragma unroll
+        for (int q = 0; q < kMaxBasis; ++q) {
+          if (q < k) {
+            const double2 b = vs[q * nt + t];
+            er -= wr[q] * b.x + wi[q] * b.y;
+            ei -= wi[q] * b.x - wr[q] * b.y;
+          }
+        }
+        const float re = static_cast<float>(er), im = static_cast<float>(ei);
+        bad |= !in_range<T>(re) || !in_range<T>(im);
+        out[t] = pack2<T>(re, im);
+      }
+    }
+    // Frames nt + 2q (hi) and nt + 2q + 1 (lo) hold w_q; lane f writes q = f, f + 8.
+#pragma unroll
+    for (int q = 0; q < kMaxBasis; ++q) {
+      if (q < k && (q & 7) == f) {
+        const float re = static_cast<float>(wr[q]), im = static_cast<float>(wi[q]);
+        bad |= !in_range<T>(re) || !in_range<T>(im);
+        const uint32_t hi = pack2<T>(re, im);
+        const float2 h = widen<T>(hi);
+        out[nt + 2 * q] = hi;
+        out[nt + 2 * q + 1] = pack2<T>(static_cast<float>(wr[q] - h.x), static_cast<float>(wi[q] - h.y));
+      }
     }
   }
-  if (lane < k) {  // frames nt + 2 lane (hi) and nt + 2 lane + 1 (lo) hold w_lane
-    const float re = static_cast<float>(w_mine.x), im = static_cast<float>(w_mine.y);
-    bad |= !in_range<T>(re) || !in_range<T>(im);
-    const uint32_t hi = pack2<T>(re, im);
-    const float2 h = widen<T>(hi);
-    const uint32_t lo = pack2<T>(static_cast<float>(w_mine.x - h.x), static_cast<float>(w_mine.y - h.y));
-    out[nt + 2 * lane] = hi;
-    out[nt + 2 * lane + 1] = lo;
-  }
-  if (range_error && __any(bad) && lane == 0) *range_error = 1;
+  if (range_error && __any(bad) && (threadIdx.x & (kWave - 1)) == 0) *range_error = 1;
 }
 
-// One wave per voxel row of the coded delay-and-sum output (nt + 2k complex64
-// columns): x_t = d_t + sum_j (d[nt + 2j] + d[nt + 2j + 1]) conj(V[t, j]),
-// stored in `O`; power (mean over the nt frames of |x_t|^2) into pd[2v].
+// Eight lanes per voxel row of the coded delay-and-sum output (nt + 2k
+// complex64 columns), lane f holding frames f + 8 j; 32 rows per workgroup,
+// the basis staged in LDS: x_t = d_t + sum_j (d[nt + 2j] + d[nt + 2j + 1])
+// conj(V[t, j]), stored in `O`; power (mean over the nt frames of |x_t|^2)
+// into pd[2v].
 template <typename O>
 __global__ __launch_bounds__(256) void k_expand(const float2* __restrict__ coded, const double2* __restrict__ basis,
                                                 int k, O* __restrict__ out, float* __restrict__ pd, int nt,
                                                 long long rows) {
-  const long long row = static_cast<long long>(blockIdx.x) * 4 + threadIdx.x / kWave;
-  const int lane = threadIdx.x & (kWave - 1);
-  if (row >= rows) return;
+  extern __shared__ double2 vs[];  // [k][nt]
+  for (int i = threadIdx.x; i < k * nt; i += blockDim.x) vs[i] = basis[i];
+  __syncthreads();
+  const long long row = static_cast<long long>(blockIdx.x) * kPackRowsPerBlock + threadIdx.x / 8;
+  const int f = threadIdx.x & 7;
+  if (row >= rows) return;  // after the only barrier
   const int ntc = nt + 2 * k;
   const float2* in = coded + row * ntc;
-  // Lane j < k: the delay-and-sum of w_j (hi + lo parts).
-  float2 dw = make_float2(0.f, 0.f);
-  if (lane < k) {
-    const float2 h = in[nt + 2 * lane], l = in[nt + 2 * lane + 1];
-    dw = make_float2(h.x + l.x, h.y + l.y);
-  }
-  float2 x[4];
+  // The delay-and-sum of w_j (hi + lo parts), read by lane j % 8 and shared in the row's 8 lanes.
+  float dwr[kMaxBasis], dwi[kMaxBasis];
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int t = lane + kWave * j;
-    x[j] = t < nt ? in[t] : make_float2(0.f, 0.f);
-  }
-  for (int q = 0; q < k; ++q) {
-    const float wr = __shfl(dw.x, q, kWave), wi = __shfl(dw.y, q, kWave);
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int t = lane + kWave * j;
-      if (t < nt) {
-        const double2 b = basis[static_cast<long long>(q) * nt + t];
-        const float br = static_cast<float>(b.x), bi = static_cast<float>(b.y);
-        x[j].x = fmaf(wr, br, fmaf(wi, bi, x[j].x));  // + w conj(b)
-        x[j].y = fmaf(wi, br, fmaf(-wr, bi, x[j].y));
-      }
+  for (int q = 0; q < kMaxBasis; ++q) {
+    float2 d = make_float2(0.f, 0.f);
+    if (q < k && (q & 7) == f) {
+      const float2 h = in[nt + 2 * q], l = in[nt + 2 * q + 1];
+      d = make_float2(h.x + l.x, h.y + l.y);
     }
+    dwr[q] = __shfl(d.x, (threadIdx.x & (kWave - 8)) | (q & 7), kWave);
+    dwi[q] = __shfl(d.y, (threadIdx.x & (kWave - 8)) | (q & 7), kWave);
   }
   float power = 0.f;
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int t = lane + kWave * j;
+#pragma unroll 4
+  for (int j = 0; j < kPerLaneLR; ++j) {
+    const int t = f + 8 * j;
     if (t < nt) {
-      power = fmaf(x[j].x, x[j].x, fmaf(x[j].y, x[j].y, power));
-      if (out) store_sample(out + row * nt + t, x[j]);
+      float2 x = in[t];
+#pragma unroll
+      for (int q = 0; q < kMaxBasis; ++q) {
+        if (q < k) {
+          const double2 b = vs[q * nt + t];
+          const float br = static_cast<float>(b.x), bi = static_cast<float>(b.y);
+          x.x = fmaf(dwr[q], br, fmaf(dwi[q], bi, x.x));  // + w conj(b)
+          x.y = fmaf(dwi[q], br, fmaf(-dwr[q], bi, x.y));
+        }
+      }
+      power = fmaf(x.x, x.x, fmaf(x.y, x.y, power));
+      if (out) store_sample(out + row * nt + t, x);
     }
   }
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) power += __shfl_xor(power, off, kWave);
-  if (pd && lane == 0) {
+  for (int off = 1; off < 8; off <<= 1) power += __shfl_xor(power, off, kWave);
+  if (pd && f == 0) {
     pd[2 * row] = power / static_cast<float>(nt);
     pd[2 * row + 1] = 0.f;
   }
@@ -183,16 +198,17 @@ extern "C" rcabeam_status rcabeam_pack_ensemble_lowrank_device(const void* sourc
   if (k > kMaxBasis || nt > kMaxFramesLR || ns * nc * na > (1ull << 40)) return RCABEAM_ERROR_ARGUMENT;
   if (storage != RCABEAM_STORAGE_FP16 && storage != RCABEAM_STORAGE_BF16) return RCABEAM_ERROR_ARGUMENT;
   const long long rows = static_cast<long long>(ns * nc * na);
-  const unsigned grid = static_cast<unsigned>((rows + 3) / 4);
+  const unsigned grid = static_cast<unsigned>((rows + kPackRowsPerBlock - 1) / kPackRowsPerBlock);
+  const size_t lds = std::max<size_t>(k * nt * sizeof(double2), sizeof(double2));  // <= 48 KiB
   const double2* v = static_cast<const double2*>(basis);
   if (storage == RCABEAM_STORAGE_FP16)
-    k_pack_lowrank<__half2><<<grid, 256>>>(static_cast<const float2*>(source), static_cast<uint32_t*>(target), v,
-                                           static_cast<int>(k), range_error, static_cast<int>(ns),
-                                           static_cast<int>(nc), static_cast<int>(na), static_cast<int>(nt), rows);
+    k_pack_lowrank<__half2><<<grid, 256, lds>>>(static_cast<const float2*>(source), static_cast<uint32_t*>(target),
+                                                v, static_cast<int>(k), range_error, static_cast<int>(ns),
+                                                static_cast<int>(nc), static_cast<int>(na), static_cast<int>(nt), rows);
   else
-    k_pack_lowrank<bf16x2><<<grid, 256>>>(static_cast<const float2*>(source), static_cast<uint32_t*>(target), v,
-                                          static_cast<int>(k), range_error, static_cast<int>(ns),
-                                          static_cast<int>(nc), static_cast<int>(na), static_cast<int>(nt), rows);
+    k_pack_lowrank<bf16x2><<<grid, 256, lds>>>(static_cast<const float2*>(source), static_cast<uint32_t*>(target),
+                                               v, static_cast<int>(k), range_error, static_cast<int>(ns),
+                                               static_cast<int>(nc), static_cast<int>(na), static_cast<int>(nt), rows);
   return launch_status();
 }
 
@@ -202,19 +218,20 @@ extern "C" rcabeam_status rcabeam_lowrank_expand_device(const void* coded, const
     return RCABEAM_ERROR_ARGUMENT;
   if (out_storage < RCABEAM_STORAGE_C64 || out_storage > RCABEAM_STORAGE_BF16) return RCABEAM_ERROR_ARGUMENT;
   const long long rows = static_cast<long long>(nv);
-  const unsigned grid = static_cast<unsigned>((rows + 3) / 4);
+  const unsigned grid = static_cast<unsigned>((rows + kPackRowsPerBlock - 1) / kPackRowsPerBlock);
+  const size_t lds = std::max<size_t>(k * nt * sizeof(double2), sizeof(double2));
   const float2* c = static_cast<const float2*>(coded);
   const double2* v = static_cast<const double2*>(basis);
   const int ki = static_cast<int>(k), ti = static_cast<int>(nt);
   switch (out_storage) {
     case RCABEAM_STORAGE_C64:
-      k_expand<float2><<<grid, 256>>>(c, v, ki, static_cast<float2*>(out), pd, ti, rows);
+      k_expand<float2><<<grid, 256, lds>>>(c, v, ki, static_cast<float2*>(out), pd, ti, rows);
       break;
     case RCABEAM_STORAGE_FP16:
-      k_expand<__half2><<<grid, 256>>>(c, v, ki, static_cast<__half2*>(out), pd, ti, rows);
+      k_expand<__half2><<<grid, 256, lds>>>(c, v, ki, static_cast<__half2*>(out), pd, ti, rows);
       break;
     default:
-      k_expand<bf16x2><<<grid, 256>>>(c, v, ki, static_cast<bf16x2*>(out), pd, ti, rows);
+      k_expand<bf16x2><<<grid, 256, lds>>>(c, v, ki, static_cast<bf16x2*>(out), pd, ti, rows);
   }
   return launch_status();
 }
