@@ -233,14 +233,14 @@ def _lib_coef_16bit() -> int:
     return _lib.COEF_16BIT
 
 
-def cpu_baseline(scene, rc, cr, rank_start, voxels: int):
-    """Time the float64 NumPy oracle (OPW + Gram/EVD clutter filter + PD) on a voxel sample of the same workload."""
+def cpu_baseline(scene, rc, cr, rank_start, pick):
+    """Time the float64 NumPy oracle (OPW + Gram/EVD clutter filter + PD) on a voxel sample `pick` of the same
+    workload: (its IQ there, the baseline record).  The IQ gives the float64 oracle values of the parity lines."""
     from oracle import clutter as oc
     from oracle import das
 
-    rng = np.random.default_rng(0)
+    voxels = len(pick)
     scan = das.scan_coords(scene.grid)
-    pick = np.sort(rng.choice(len(scan), size=voxels, replace=False))
     rc_h = rc.cpu().numpy()
     cr_h = cr.cpu().numpy()
     g = scene.geom
@@ -250,7 +250,7 @@ def cpu_baseline(scene, rc, cr, rank_start, voxels: int):
     oc.filtered_power(iq, basis, rank_start=rank_start)
     seconds = time.perf_counter() - begin
     cores = len(os.sched_getaffinity(0))
-    return {
+    return iq, {
         "value": voxels * scene.frames / seconds,
         "unit": "voxel-frames/s",
         "cores": 1,
@@ -470,7 +470,8 @@ def main() -> int:
         },
         "roofline": {
             "bound": "mfma",
-            "engine": "fp32 VALU (delay-and-sum has no GEMM form at this sparsity; fp32 peak used as the roof)",
+            "engine": "fp32 VALU (the float32 delay-and-sum; MI355X fp32 vector peak as the roof, of which the "
+                      "build's non-packed FP32 reaches half)",
             "kernel": DAS_KERNEL[storage],
             "achieved": achieved,
             "peak": FP32_PEAK_TFLOPS,
@@ -479,6 +480,10 @@ def main() -> int:
             "traffic": traffic_from_profiles(storage),
         },
     }
+    # Voxels at which every form's IQ is compared with the float64 oracle (the CPU baseline's sample).
+    pick = np.sort(np.random.default_rng(0).choice(nv, size=args.cpu_voxels, replace=False))
+    pick_d = torch.from_numpy(pick).cuda()
+    sampled = {"float32": pipe.signal[pick_d].clone() if pipe.code == 0 else None}
     if storage == "float32" and not args.no_mixed:
         # Mixed-precision forms on the same scene, with their full-size difference from the float32 pipeline
         # (whose own error against the float64 oracle is ~1e-5 on filtered power; tests/test_mixed_precision_gpu.py).
@@ -492,6 +497,9 @@ def main() -> int:
         for key, form, label in (
             ("mixed", "float16-lowrank",
              f"float16 channels coded as {args.lowrank_modes} temporal modes (16-bit hi + lo) + 16-bit residual, "
+             "MFMA delay-and-sum with one 16-bit coefficient per sample, float32 accumulation, complex64 IQ"),
+            ("mixed_bf16", "bfloat16-lowrank",
+             f"bfloat16 channels coded as {args.lowrank_modes} temporal modes (16-bit hi + lo) + 16-bit residual, "
              "MFMA delay-and-sum with one 16-bit coefficient per sample, float32 accumulation, complex64 IQ"),
             ("mixed_direct", "float16",
              "float16 channels and IQ rounded directly, MFMA delay-and-sum with one 16-bit coefficient per sample"),
@@ -511,6 +519,7 @@ def main() -> int:
                 "rel_l2_vs_float32": {"iq": rel(iq, reference["iq"]), "power": rel(mixed.pd[:, 0], reference["pd"]),
                                       "filtered_power": rel(mixed.power, reference["power"])},
             }
+            sampled[key] = iq[pick_d].clone()
             del mixed, iq
             torch.cuda.empty_cache()
         del reference
@@ -523,7 +532,19 @@ def main() -> int:
         record["host_ingest"] = ingest_timing(rc)
     if not args.no_cpu_baseline:
         log("bench: CPU baseline (oracle) on a voxel sample")
-        record["cpu_baseline"] = cpu_baseline(scene, rc, cr, rank_start, args.cpu_voxels)
+        oracle_iq, record["cpu_baseline"] = cpu_baseline(scene, rc, cr, rank_start, pick)
+        # Each form's IQ and unfiltered power at the sampled voxels against the float64 oracle.
+        for key, iq in sampled.items():
+            if iq is None:
+                continue
+            got = iq.cpu().numpy().astype(np.complex128)
+            target = record if key == "float32" else record[key]
+            target["vs_float64_oracle_sampled"] = {
+                "voxels": len(pick),
+                "iq": float(np.linalg.norm(got - oracle_iq) / np.linalg.norm(oracle_iq)),
+                "power": float(np.linalg.norm((np.abs(got) ** 2).mean(-1) - (np.abs(oracle_iq) ** 2).mean(-1))
+                               / np.linalg.norm((np.abs(oracle_iq) ** 2).mean(-1))),
+            }
     print(json.dumps(record), flush=True)
     return 0
 
