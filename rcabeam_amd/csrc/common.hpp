@@ -160,6 +160,13 @@ __device__ __forceinline__ float uniform(float x) {
   return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
 }
 
+// Allow `kernel` up to `bytes` of dynamic LDS (beyond the 64 KiB default).
+// The attribute belongs to the device current at the call, so it is set
+// before every launch rather than cached (a host call of a few microseconds).
+inline hipError_t allow_dynamic_lds(const void* kernel, int bytes) {
+  return hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+}
+
 inline rcabeam_status launch_status() {
   return hipGetLastError() == hipSuccess ? RCABEAM_SUCCESS : RCABEAM_ERROR_CUDA;
 }

@@ -356,13 +356,8 @@ hipError_t launch_shape(GridArgs a) {
   if (grid > 0x7fffffffLL) return hipErrorInvalidValue;
   a.vec = reinterpret_cast<uintptr_t>(a.rc) % 16 == 0 && reinterpret_cast<uintptr_t>(a.cr) % 16 == 0 && a.nt % 4 == 0;
   auto kernel = das_opw_mfma<T, CP, NB>;
-  static bool attribute_set = false;  // one opt-in per instantiation (beyond 64 KiB of dynamic LDS)
-  if (!attribute_set) {
-    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
-                                             hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytesM);
-    if (e != hipSuccess) return e;
-    attribute_set = true;
-  }
+  const hipError_t e = allow_dynamic_lds(reinterpret_cast<const void*>(kernel), kLdsBytesM);
+  if (e != hipSuccess) return e;
   kernel<<<dim3(static_cast<unsigned>(grid)), kMThreads, bytes>>>(a);
   return hipGetLastError();
 }

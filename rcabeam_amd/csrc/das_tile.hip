@@ -405,14 +405,8 @@ hipError_t launch_shape(const EnsembleArgs& p, const TileMap& map, long long til
   const bool vec = reinterpret_cast<uintptr_t>(p.rc) % 16 == 0 && reinterpret_cast<uintptr_t>(p.cr) % 16 == 0 &&
                    p.nt % U == 0;
   auto kernel = das_opw_tile<T, CP, FPL, APERTURES>;
-  static bool attribute_set = false;  // one opt-in per instantiation (beyond 64 KiB of dynamic LDS)
-  if (!attribute_set) {
-    const hipError_t e =
-        hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
-                            kLdsBytes);
-    if (e != hipSuccess) return e;
-    attribute_set = true;
-  }
+  const hipError_t e = allow_dynamic_lds(reinterpret_cast<const void*>(kernel), kLdsBytes);
+  if (e != hipSuccess) return e;
   kernel<<<dim3(static_cast<unsigned>(grid)), kThreads, layout.bytes>>>(p, map, tiles, max_rows, vec, out_storage);
   return hipGetLastError();
 }
@@ -641,13 +635,8 @@ hipError_t launch_adjoint(const AdjointArgs& a, const TileMap& map, long long ti
   const int bytes = fixed + max_rows * per_row;
   const long long grid = (tiles + 7) / 8 * 8;
   if (grid > 0x7fffffffLL) return hipErrorInvalidValue;
-  static bool attribute_set = false;
-  if (!attribute_set) {
-    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(das_opw_adjoint),
-                                             hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes);
-    if (e != hipSuccess) return e;
-    attribute_set = true;
-  }
+  const hipError_t e = allow_dynamic_lds(reinterpret_cast<const void*>(das_opw_adjoint), kLdsBytes);
+  if (e != hipSuccess) return e;
   das_opw_adjoint<<<dim3(static_cast<unsigned>(grid)), kThreads, bytes>>>(a, map, tiles, max_rows);
   return hipGetLastError();
 }
