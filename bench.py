@@ -71,6 +71,7 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision comparison runs")
     p.add_argument("--no-combiners", action="store_true", help="skip the full-size combiner timings")
+    p.add_argument("--no-two-stage", action="store_true", help="skip the full-size two-stage approximation timings")
     p.add_argument("--cpu-voxels", type=int, default=2400)
     p.add_argument("--copy-calibration", action="store_true",
                    help="only run a plain 1 GiB device-to-device copy three times (HBM counter calibration)")
@@ -288,6 +289,60 @@ def combiner_timing(scene, rc, cr) -> dict:
     torch.cuda.synchronize()
     seconds = time.perf_counter() - begin
     out["xdoppler_filtered"] = {"ms": seconds * 1e3, "voxel_frames_per_s": nx * nz * ny * frames / seconds}
+    return out
+
+
+def two_stage_timing(scene, rc, cr, reps: int = 2) -> dict:
+    """The two-stage approximation (reference planes + not-a-knot spline expansion) on the same full-size
+    acquisition and grid, channels resident in HBM: `st_sw_baseband` (one baseband plane per aperture, four
+    reference depths per wavelength) and OPW IQ (eight per wavelength, all frames per pass).  Timed per
+    `reconstruct` after one warm-up, workspace built beforehand; the entry point's own call (workspace
+    construction included) is timed once.  The approximation's distance from the conventional forms
+    (conventional St-SW k = 2, grid OPW IQ) is reported as a relative L2 error."""
+    import torch
+
+    from rcabeam_amd.ensemble import beamform_ensemble, ensemble_pd_from_channels
+    from rcabeam_amd.two_stage import TwoStageOpwWorkspace, TwoStageStswWorkspace
+
+    nx, nz, ny = (len(a) for a in scene.grid)
+    frames = rc.shape[-1]
+    args = (tuple(rc.shape), scene.angles, scene.t_start, scene.grid, scene.geom)
+
+    def rel(a, b):
+        return float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        begin = time.perf_counter()
+        for _ in range(reps):
+            result = fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - begin) / reps, result
+
+    out = {}
+    stsw = TwoStageStswWorkspace(*args)
+    seconds, (_, power) = timed(lambda: stsw.reconstruct(rc, cr))
+    conventional = ensemble_pd_from_channels(rc, cr, scene.angles, scene.t_start, scene.grid, scene.geom,
+                                             method="st_sw", k=2)
+    out["st_sw_baseband"] = {"ms": seconds * 1e3, "voxel_frames_per_s": nx * nz * ny * frames / seconds,
+                             "bands": len(stsw._plan.bands), "reference_depths": len(stsw.plane_depths),
+                             "rel_l2_vs_conventional_st_sw": rel(power, conventional)}
+    del stsw, power, conventional
+    torch.cuda.synchronize()
+    begin = time.perf_counter()
+    ensemble_pd_from_channels(rc, cr, scene.angles, scene.t_start, scene.grid, scene.geom, method="st_sw_baseband")
+    torch.cuda.synchronize()
+    out["st_sw_baseband"]["entry_point_ms"] = (time.perf_counter() - begin) * 1e3
+    opw = TwoStageOpwWorkspace(*args, samples_per_wavelength=8, frame_chunk=frames)
+    seconds, iq = timed(lambda: opw.reconstruct(rc, cr))
+    conventional = beamform_ensemble(rc, cr, scene.angles, scene.t_start, scene.grid, scene.geom)
+    out["opw_iq"] = {"ms": seconds * 1e3, "voxel_frames_per_s": nx * nz * ny * frames / seconds,
+                     "samples_per_wavelength": 8, "bands": len(opw._plan.bands),
+                     "reference_depths": len(opw.plane_depths),
+                     "rel_l2_vs_conventional_iq": rel(iq, conventional)}
+    del opw, iq, conventional
+    torch.cuda.empty_cache()
     return out
 
 
@@ -530,6 +585,9 @@ def main() -> int:
         record["combiners"] = combiner_timing(scene, rc, cr)
         log("bench: host ingest")
         record["host_ingest"] = ingest_timing(rc)
+    if not args.no_two_stage:
+        log("bench: two-stage approximation at full size")
+        record["two_stage"] = two_stage_timing(scene, rc, cr)
     if not args.no_cpu_baseline:
         log("bench: CPU baseline (oracle) on a voxel sample")
         oracle_iq, record["cpu_baseline"] = cpu_baseline(scene, rc, cr, rank_start, pick)

@@ -230,6 +230,123 @@ rcabeam_status rcabeam_simulate_channels_device(const double* static_pos, const 
                                                 float t_start, float c, float fs, float fd, float sigma_t,
                                                 float noise_std, unsigned long long seed);
 
+/* ---- Two-stage OPW approximation (reference src/rcabeam/two_stage.py,
+ * src/rcabeam/two_stage.cu).  Each aperture is beamformed onto reference
+ * planes (receive line x reference depth, one plane per elevation slab), a
+ * not-a-knot cubic spline is solved along depth, and every voxel is expanded
+ * from it after mapping its elevation offset to a depth shift
+ * offset * tan(angle / 2).  Planes are (n_planes * nl, knots, na, chunk)
+ * complex64 with the line index plane * nl + line. */
+
+/* Reference planes of one aperture over one depth band (reference kernel
+ * `planes`, two_stage.cu:20-84): out[line, k, angle, f] for k < npz, the sum
+ * over receive channels with |lateral - element| <= aperture_radius (all
+ * channels when aperture_radius < 0) of the linearly interpolated sample at
+ * the delay of reference depth knots[knot_first + k], rotated by
+ * e^{+2 pi i fd tau} (baseband = 0) or by the residual e^{+2 pi i fd (r - z)/c}
+ * of the closest-element carrier (baseband = 1).  Knots that also lie in the
+ * previous band [previous_first, previous_first + previous_count) of a line
+ * whose same_aperture[band_id * nl + line % nl] is set are copied from
+ * `previous` (same layout, previous_count knots).  raw: (ns, nc, na, nt)
+ * complex64, frames first .. first + count - 1 of it into chunk slots. */
+rcabeam_status rcabeam_two_stage_planes_device(const void* raw, const float* lateral, const float* elements,
+                                               const float* angles, const float* starts, const float* anchors,
+                                               const float* knots, const unsigned char* same_aperture, void* out,
+                                               const void* previous, int band_id, int knot_first, int previous_first,
+                                               int previous_count, size_t ns, size_t nc, size_t na, size_t nt,
+                                               size_t nl, size_t n_planes, size_t npz, size_t chunk, size_t first,
+                                               size_t count, float c, float fs, float fd, float aperture_radius,
+                                               int baseband);
+
+/* Not-a-knot second derivatives times the knot step squared along the knot
+ * axis of `planes` (reference kernel `spline`, two_stage.cu:86-111; NumPy
+ * two_stage.py:119-143).  upper / inverse: Thomas factors u[1] = 0,
+ * inv[i] = 1 / (4 - u[i-1]), u[i] = inv[i] for 2 <= i < npz - 2.  npz >= 4. */
+rcabeam_status rcabeam_two_stage_spline_device(const void* planes, void* second, const float* upper,
+                                               const float* inverse, size_t lines, size_t npz, size_t na,
+                                               size_t chunk, size_t count);
+
+/* Voxel expansion of depth rows depth_start .. depth_start + depth_count - 1
+ * (reference kernel `expand`, two_stage.cu:155-243).  stsw = 0: out is IQ
+ * (nx, nz, ny, nt) complex64, frames first .. first + count - 1 written.
+ * stsw = 1: out is (10, nx * nz * ny) float32 slow-time sums (stored when
+ * first == 0, added otherwise): XDoppler product, then for the subset pairs
+ * (00, 11) and (01, 10) the cross-coherence product and both energies.
+ * x_phase (nx, na), z_phase (nz, na), y_phase (ny, na): complex64 carrier
+ * e^{i (x|z|y) k_(u|z|u)} restored after the lookup (all three or none). */
+rcabeam_status rcabeam_two_stage_expand_device(const void* rc_planes, const void* cr_planes, const void* rc_second,
+                                               const void* cr_second, const float* z, const float* slopes, void* out,
+                                               int stsw, size_t nx, size_t nz, size_t ny, size_t npz, size_t na,
+                                               size_t nt, size_t chunk, size_t first, size_t count,
+                                               size_t depth_start, size_t depth_count, float plane_z0,
+                                               float plane_dz, const int* x_plane, const float* x_offset,
+                                               const int* y_plane, const float* y_offset, size_t n_planes,
+                                               const void* x_phase, const void* z_phase, const void* y_phase);
+
+/* St-SW weights and power from the sums of rcabeam_two_stage_expand_device
+ * (reference kernel `finish_stsw`, two_stage.cu:138-152, and the
+ * normalisation of two_stage.py:384-392): weights = |sum of the two subset-pair
+ * correlations| / max, power = |mean XDoppler| * weights.  peak: one device
+ * word of workspace. */
+rcabeam_status rcabeam_two_stage_stsw_finish_device(const float* stats, float* weights, float* power, size_t nv,
+                                                    size_t nt, unsigned* peak);
+
+/* Everything rcabeam_two_stage_reconstruct_device needs besides the data. */
+typedef struct rcabeam_two_stage_plan {
+    size_t ns, nc, na, nt;        /* channel ensemble (ns, nc, na, nt) */
+    size_t nx, nz, ny;            /* grid */
+    size_t npz;                   /* reference depths */
+    size_t n_planes;              /* elevation slabs per aperture */
+    size_t chunk;                 /* frames per pass */
+    size_t n_bands;               /* depth bands sharing one receive mask */
+    /* host arrays, n_bands entries */
+    const int* band_depth_start;
+    const int* band_depth_count;
+    const int* band_knot_first;
+    const int* band_knot_count;
+    const float* band_radius;     /* aperture radius at the band's first depth (< 0: full) */
+    const float* knots_host;      /* npz reference depths */
+    /* device arrays */
+    const float* x;               /* nx */
+    const float* z;               /* nz */
+    const float* y;               /* ny */
+    const float* x_el;            /* nc */
+    const float* y_el;            /* nc */
+    const float* angles;          /* na */
+    const float* starts;          /* na */
+    const float* slopes;          /* na: tan(angle / 2) */
+    const float* knots;           /* npz */
+    const float* x_centers;       /* n_planes slabs along x (CR transmit anchors) */
+    const int* x_plane;           /* nx: slab of each x */
+    const float* x_offset;        /* nx: x minus its slab centre */
+    const float* y_centers;       /* n_planes slabs along y (RC transmit anchors) */
+    const int* y_plane;           /* ny */
+    const float* y_offset;        /* ny */
+    const unsigned char* x_same;  /* (n_bands, nx) unchanged RC receive sets */
+    const unsigned char* y_same;  /* (n_bands, ny) unchanged CR receive sets */
+    const float* spline_upper;    /* >= max band knot count */
+    const float* spline_inverse;
+    const void* x_phase;          /* baseband carrier tables (see the expansion) */
+    const void* z_phase;
+    const void* y_phase;
+    float c, fs, fd, dz;
+    int baseband;                 /* remove the closest-element carrier before the spline */
+    int stsw;                     /* 0: IQ out; 1: St-SW sums out */
+} rcabeam_two_stage_plan;
+
+/* complex64 elements of each of the four plane buffers the reconstruction
+ * uses (RC and CR planes and second derivatives, reused across bands). */
+size_t rcabeam_two_stage_buffer_elements(const rcabeam_two_stage_plan* plan);
+
+/* The whole two-stage reconstruction (reference
+ * TwoStageOpwWorkspace.reconstruct, two_stage.py:306-396): for each frame
+ * chunk and depth band, planes and splines of both apertures, then the voxel
+ * expansion.  buffers: 4 * rcabeam_two_stage_buffer_elements complex64;
+ * out: IQ (nx, nz, ny, nt) complex64, or the (10, nv) St-SW sums (follow with
+ * rcabeam_two_stage_stsw_finish_device). */
+rcabeam_status rcabeam_two_stage_reconstruct_device(const rcabeam_two_stage_plan* plan, const void* rc,
+                                                    const void* cr, void* buffers, void* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -69,6 +69,7 @@ def delay_volumes(
     config: str,
     *,
     channels_out: bool = False,
+    aperture_depth=None,
 ):
     """Delay one aperture of channel IQ onto voxels (sim.py:344-393).
 
@@ -86,6 +87,9 @@ def delay_volumes(
         (`None` or <= 0: full aperture).
     config
         "RC" (row transmit / column receive) or "CR".
+    aperture_depth
+        Depth that sizes the receive aperture of every voxel (sim.py:372-373,
+        used by two_stage.py:163-165); None: each voxel's own depth.
 
     Returns
     -------
@@ -126,7 +130,15 @@ def delay_volumes(
         depth = z[sl, None]
         distance = rx_coord[sl, None] - el[None, :]
         receive = np.sqrt(depth**2 + distance**2) / c
-        active = np.ones_like(distance, dtype=bool) if full else np.abs(distance) <= depth / (2 * fn)
+        if full:
+            active = np.ones_like(distance, dtype=bool)
+        elif aperture_depth is None:
+            active = np.abs(distance) <= depth / (2 * fn)
+        else:
+            # A fixed aperture depth is a discrete float32 decision in the reference
+            # (two_stage.py:163-165 with float32 plane grids; two_stage.cu:56).
+            radius = np.float32(aperture_depth) / (np.float32(2) * np.float32(fn))
+            active = np.abs(distance.astype(np.float32)) <= radius
         if counts is not None:
             counts[sl] = active.sum(axis=1)
         for m in range(na):

@@ -71,6 +71,13 @@ _SIGNATURES: dict[str, list] = {
     "rcabeam_eigh_device": [c_void_p, c_void_p, c_void_p, c_void_p, c_size_t],
     "rcabeam_filtered_power_device": [c_void_p, c_int, c_size_t, c_size_t, c_void_p, c_size_t, c_int, c_void_p],
     "rcabeam_temporal_filter_device": [c_void_p, c_int, c_size_t, c_size_t, c_void_p, c_size_t, c_int, c_void_p],
+    "rcabeam_two_stage_planes_device": [c_void_p] * 10 + [c_int] * 4 + [c_size_t] * 10 + [c_float] * 4 + [c_int],
+    "rcabeam_two_stage_spline_device": [c_void_p] * 4 + [c_size_t] * 5,
+    "rcabeam_two_stage_expand_device": [c_void_p] * 7 + [c_int] + [c_size_t] * 11 + [c_float] * 2 + [c_void_p] * 4
+    + [c_size_t] + [c_void_p] * 3,
+    "rcabeam_two_stage_stsw_finish_device": [c_void_p] * 3 + [c_size_t] * 2 + [c_void_p],
+    "rcabeam_two_stage_buffer_elements": [c_void_p],
+    "rcabeam_two_stage_reconstruct_device": [c_void_p] * 5,
 }
 _RESTYPE = {
     "rcabeam_status_string": c_char_p,
@@ -78,7 +85,22 @@ _RESTYPE = {
     "rcabeam_gram_real_elements": c_size_t,
     "rcabeam_gram_f64_workspace_bytes": c_size_t,
     "rcabeam_simulate_lds_bytes": c_size_t,
+    "rcabeam_two_stage_buffer_elements": c_size_t,
 }
+
+
+class TwoStagePlan(ctypes.Structure):
+    """`rcabeam_two_stage_plan` (include/rcabeam_mi355x.h); pointers are plain addresses."""
+
+    _fields_ = [(name, c_size_t) for name in ("ns", "nc", "na", "nt", "nx", "nz", "ny", "npz", "n_planes", "chunk",
+                                               "n_bands")]
+    _fields_ += [(name, c_void_p) for name in (
+        "band_depth_start", "band_depth_count", "band_knot_first", "band_knot_count", "band_radius", "knots_host",
+        "x", "z", "y", "x_el", "y_el", "angles", "starts", "slopes", "knots", "x_centers", "x_plane", "x_offset",
+        "y_centers", "y_plane", "y_offset", "x_same", "y_same", "spline_upper", "spline_inverse", "x_phase",
+        "z_phase", "y_phase")]
+    _fields_ += [(name, c_float) for name in ("c", "fs", "fd", "dz")]
+    _fields_ += [("baseband", c_int), ("stsw", c_int)]
 
 # Symbols declared in include/*.h that the library must export.
 EXPORTED = tuple(_SIGNATURES)

@@ -24,7 +24,7 @@ from rcabeam_amd import _device as dev
 from rcabeam_amd import _lib
 from rcabeam_amd.sim import RCAGeometry, _scan_coords
 
-Method = Literal["opw", "xdoppler", "rc_fmas", "dmas", "st_sw"]
+Method = Literal["opw", "xdoppler", "rc_fmas", "dmas", "st_sw", "st_sw_baseband"]
 IQStorage = Literal["float32", "float16", "bfloat16"]
 _METHODS = {"opw": 0, "xdoppler": 1, "rc_fmas": 2, "dmas": 3, "st_sw": 4, "apertures": 5}
 _RANGE_MESSAGE = {
@@ -241,12 +241,36 @@ def ensemble_pd_from_channels(rc, cr, angles, t_start, grid, geom: RCAGeometry, 
     OPW / RC-FMAS: mean squared magnitude; XDoppler: |mean cross-product|;
     St-SW: XDoppler weighted by normalised ensemble correlations; DMAS: mean
     real DMAS-CCF-ACF response.  Unfiltered (reference ensemble.py:133-218).
+    `st_sw_baseband` approximates the per-angle signals with the two-stage
+    expansion (one baseband reference plane per aperture, four reference depths
+    per wavelength; `rcabeam_amd.two_stage`) and needs k=2, float32 storage.
     `iq_storage` "float16"/"bfloat16" packs channels at 16 bits (OPW only);
     arithmetic stays float32.
     """
     if method == "st_sw_baseband":
-        raise ValueError("st_sw_baseband (two-stage approximation) is not provided by rcabeam_amd")
+        return _stsw_baseband(rc, cr, angles, t_start, grid, geom, k, timings, iq_storage)
     return _run(rc, cr, angles, t_start, grid, geom, method, k, False, timings, iq_storage)[0]
+
+
+def _stsw_baseband(rc, cr, angles, t_start, grid, geom, k, timings, iq_storage):
+    """Two-subset St-SW on the two-stage baseband approximation (reference ensemble.py:198-217):
+    one reference plane per aperture at four reference depths per wavelength."""
+    from rcabeam_amd.two_stage import TwoStageStswWorkspace
+
+    if k != 2 or iq_storage != "float32":
+        raise ValueError("Baseband St-SW requires k=2 and float32 IQ storage")
+    if tuple(rc.shape) != tuple(cr.shape):
+        raise ValueError("RC and CR channel shapes must match")
+    t = dev.torch()
+    workspace = TwoStageStswWorkspace(tuple(rc.shape), angles, t_start, grid, geom)
+    host_input = not (dev.is_tensor(rc) and dev.is_tensor(cr))
+    begin = perf_counter()
+    raw = [dev.to_device(np.ascontiguousarray(x) if not dev.is_tensor(x) else x, t.complex64) for x in (rc, cr)]
+    if timings is not None:
+        dev.synchronize()
+        timings["raw_upload_seconds"] = perf_counter() - begin if host_input else 0.0
+    _, power = workspace.reconstruct(*raw)
+    return dev.host(power) if host_input else power.clone()
 
 
 def beamform_ensemble(rc, cr, angles, t_start, grid, geom: RCAGeometry, *,

@@ -1,0 +1,221 @@
+"""Two-stage OPW approximation on the GPU against the float64 oracle (reference tests/test_two_stage.py).
+
+Bounds: IQ and St-SW outputs within a relative L2 error of 2e-5 of the float64
+oracle (`oracle/two_stage.py`); the kernels run float32 plane sums, a float32
+Thomas sweep and float32 spline lookups at float64 voxel positions.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import two_stage as ots
+from oracle.scene import simulate_point
+
+pytestmark = pytest.mark.gpu
+
+C, FS, FD = 1540.0, 20e6, 15e6
+BOUND = 2e-5
+
+
+def _geom(x_el, y_el, fnumber, fs=FS, fd=FD):
+    from rcabeam_amd import RCAGeometry
+
+    return RCAGeometry(np.asarray(x_el, np.float32), np.asarray(y_el, np.float32), c=C, fs=fs, f_demod=fd,
+                       fnumber=fnumber)
+
+
+def _rel(a, b):
+    b = np.asarray(b)
+    return float(np.linalg.norm(np.asarray(a) - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def _case(frames=3):
+    """reference tests/test_two_stage.py:10-18."""
+    el = np.array([-0.1e-3, 0, 0.1e-3])
+    angles = np.array([-0.12, 0.03, 0.1])
+    starts = np.array([1.9e-6, 2e-6, 2.1e-6])
+    grid = (np.array([-0.12e-3, 0.03e-3]), np.linspace(7.9e-3, 8.1e-3, 5), np.array([-0.2e-3, 0, 0.1e-3]))
+    phase = np.exp(0.2j * np.arange(frames)).astype(np.complex64)
+    raw = [simulate_point(el, el, C, FS, FD, angles, (0, 8e-3, 0), 368, 2e-6, cfg)[..., None] * phase
+           for cfg in ("RC", "CR")]
+    return raw[0], raw[1], angles, starts, grid, el, el, 1.0
+
+
+def _oracle(rc, cr, angles, starts, grid, x_el, y_el, fnumber, fs=FS, fd=FD, **options):
+    return ots.two_stage_opw(rc, cr, angles, starts, grid, x_el, y_el, C, fs, fd, fnumber, **options)
+
+
+def _gpu(x):
+    import torch
+
+    return torch.from_numpy(np.ascontiguousarray(x)).cuda()
+
+
+@pytest.mark.parametrize("frames,chunk", [(1, 1), (7, 3), (35, 32), (70, 64), (300, 300)])
+def test_two_stage_matches_oracle(frames, chunk):
+    """Odd shapes, partial chunks and several frame groups (reference test_two_stage.py:74-86)."""
+    from rcabeam_amd.two_stage import TwoStageOpwWorkspace
+
+    rc, cr, angles, starts, grid, x_el, y_el, fn = _case(frames)
+    expected = _oracle(rc, cr, angles, starts, grid, x_el, y_el, fn)
+    ws = TwoStageOpwWorkspace(rc.shape, angles, starts, grid, _geom(x_el, y_el, fn), frame_chunk=chunk)
+    out = ws.reconstruct(_gpu(rc), _gpu(cr))
+    err = _rel(out.cpu().numpy(), expected)
+    print(f"frames {frames} chunk {chunk}: rel L2 {err:.2e}")
+    assert err < BOUND
+    again = ws.reconstruct(_gpu(rc * np.complex64(1j)), _gpu(cr * np.complex64(1j)))
+    assert again.data_ptr() == out.data_ptr()
+    assert _rel(again.cpu().numpy(), 1j * expected) < BOUND
+
+
+@pytest.mark.parametrize("swap", [False, True])
+def test_overlapping_aperture_bands(swap):
+    """Growing and shrinking apertures share knots across bands of different lengths
+    (reference test_two_stage.py:112-126)."""
+    from rcabeam_amd.two_stage import TwoStageOpwWorkspace
+
+    elements = np.array([-0.00205, 0, 0.0018])
+    angles = np.array([-0.08, 0, 0.13])
+    grid = (np.array([0, 0.0003]), np.array([0.004, 0.0042, 0.0045, 0.004]), np.array([-0.002, 0, 0.002]))
+    rng = np.random.default_rng(5)
+    shape = (160, 3, 3, 5)
+    rc, cr = [(rng.normal(size=shape) + 1j * rng.normal(size=shape)).astype(np.complex64) for _ in range(2)]
+    if swap:
+        rc, cr = cr, rc
+    ws = TwoStageOpwWorkspace(shape, angles, 0, grid, _geom(elements, elements[::-1], 1.0), frame_chunk=3)
+    assert len(ws._plan.bands) > 1
+    expected = _oracle(rc, cr, angles, 0, grid, elements, elements[::-1], 1.0)
+    err = _rel(ws.reconstruct(_gpu(rc), _gpu(cr)).cpu().numpy(), expected)
+    print(f"bands {len(ws._plan.bands)}: rel L2 {err:.2e}")
+    assert err < BOUND
+
+
+def test_receive_window_boundaries():
+    """Two-sample inputs exercise the first and last rows (reference test_two_stage.py:152-164)."""
+    from rcabeam_amd.two_stage import TwoStageOpwWorkspace
+
+    angles = np.array([-0.1, 0.0, 0.1])
+    grid = (np.array([-0.1e-3, 0.0, 0.1e-3]), np.array([7e-3]), np.array([0.0]))
+    rng = np.random.default_rng(5)
+    raw = [(rng.normal(size=(2, 1, 3, 5)) + 1j * rng.normal(size=(2, 1, 3, 5))).astype(np.complex64)
+           for _ in range(2)]
+    el = np.array([0.0])
+    ws = TwoStageOpwWorkspace(raw[0].shape, angles, 9e-6, grid, _geom(el, el, None, fs=1e6, fd=0.1e6), frame_chunk=2)
+    expected = _oracle(raw[0], raw[1], angles, 9e-6, grid, el, el, None, fs=1e6, fd=0.1e6)
+    out = ws.reconstruct(*[_gpu(v) for v in raw]).cpu().numpy()
+    np.testing.assert_allclose(out, expected, rtol=1e-5, atol=1e-5)
+
+
+@pytest.mark.filterwarnings("ignore:Axial undersampling")
+@pytest.mark.parametrize("baseband,n_planes,spw", [(False, 1, 8), (True, 1, 4), (True, 2, 4)])
+def test_aperture_uses_output_depth(baseband, n_planes, spw):
+    """Exact zeros outside the aperture at the output depth (reference test_two_stage.py:192-224)."""
+    from rcabeam_amd.two_stage import TwoStageOpwWorkspace
+
+    x_el, y_el = np.array([0.00205]), np.zeros(1)
+    angles = np.array([0.14])
+    grid = (np.zeros(1), np.array([0.004, 0.0042, 0.004]), np.array([-0.004, 0.004]))
+    rc = np.ones((256, 1, 1, 3), np.complex64)
+    cr = np.zeros_like(rc)
+    opts = dict(samples_per_wavelength=spw, n_planes=n_planes, baseband=baseband)
+    ws = TwoStageOpwWorkspace(rc.shape, angles, 0, grid, _geom(x_el, y_el, 1.0), frame_chunk=2, **opts)
+    out = ws.reconstruct(_gpu(rc), _gpu(cr)).cpu().numpy()
+    np.testing.assert_array_equal(out[:, [0, 2]], 0)
+    expected = _oracle(rc, cr, angles, 0, grid, x_el, y_el, 1.0, **opts)
+    assert _rel(out, expected) < BOUND
+
+
+@pytest.mark.parametrize("n_planes", [1, 2])
+def test_stsw_matches_oracle(n_planes):
+    """Subset assignment, remodulation, frame tails and global normalisation
+    (reference test_two_stage.py:227-259)."""
+    import torch
+
+    from rcabeam_amd.two_stage import TwoStageStswWorkspace
+
+    rng = np.random.default_rng(21)
+    el = np.array([-0.0003, 0.0001])
+    angles = np.array([-0.1, -0.04, 0.03, 0.08, 0.13])
+    grid = (np.array([-0.0005, 0.0002]), np.array([0.003, 0.004]), np.array([-0.0007, 0.0003]))
+    shape = (96, 2, len(angles), 35)
+    raw = [(rng.normal(size=shape) + 1j * rng.normal(size=shape)).astype(np.complex64) for _ in range(2)]
+    ew, ep = ots.two_stage_stsw(*raw, angles, 0, grid, el, el[::-1], C, 12e6, 5e6, 1.0, n_planes=n_planes)
+    ws = TwoStageStswWorkspace(shape, angles, 0, grid, _geom(el, el[::-1], 1.0, fs=12e6, fd=5e6), n_planes=n_planes)
+    w, p = ws.reconstruct(*map(_gpu, raw))
+    print(f"n_planes {n_planes}: weights {_rel(w.cpu().numpy(), ew):.2e} power {_rel(p.cpu().numpy(), ep):.2e}")
+    assert _rel(w.cpu().numpy(), ew) < BOUND
+    assert _rel(p.cpu().numpy(), ep) < BOUND
+    for v in ws.reconstruct(*[torch.zeros(shape, dtype=torch.complex64, device="cuda") for _ in range(2)]):
+        assert torch.count_nonzero(v).item() == 0
+
+
+def test_stsw_mid_size_many_bands():
+    """32 + 32 elements, 16 x 24 x 16 grid, F-number 1: a dozen bands and the knot copies between them."""
+    from rcabeam_amd.two_stage import TwoStageStswWorkspace
+
+    rng = np.random.default_rng(7)
+    el = ((np.arange(32) - 15.5) * 200e-6).astype(np.float32)
+    angles = np.deg2rad(np.linspace(-10, 10, 5)).astype(np.float32)
+    grid = (np.linspace(-3e-3, 3e-3, 16), np.linspace(5e-3, 9e-3, 24), np.linspace(-3e-3, 3e-3, 16))
+    shape = (256, 32, 5, 20)
+    raw = [(rng.normal(size=shape) + 1j * rng.normal(size=shape)).astype(np.complex64) for _ in range(2)]
+    ws = TwoStageStswWorkspace(shape, angles, 0, grid, _geom(el, el, 1.0))
+    assert len(ws._plan.bands) > 10
+    ew, ep = ots.two_stage_stsw(*raw, angles, 0, grid, el, el, C, FS, FD, 1.0)
+    w, p = ws.reconstruct(*map(_gpu, raw))
+    print(f"bands {len(ws._plan.bands)}: weights {_rel(w.cpu().numpy(), ew):.2e} power {_rel(p.cpu().numpy(), ep):.2e}")
+    assert _rel(w.cpu().numpy(), ew) < BOUND
+    assert _rel(p.cpu().numpy(), ep) < BOUND
+
+
+def test_baseband_stsw_entry_point():
+    """`ensemble_pd_from_channels(method="st_sw_baseband")` equals the workspace, host staging included
+    (reference test_two_stage.py:262-276)."""
+    from rcabeam_amd import ensemble_pd_from_channels
+    from rcabeam_amd.two_stage import TwoStageStswWorkspace
+
+    rc, cr, angles, starts, grid, x_el, y_el, fn = _case(7)
+    rc, cr = rc[..., ::2], cr[..., ::2]
+    geom = _geom(x_el, y_el, fn)
+    ws = TwoStageStswWorkspace(rc.shape, angles, starts, grid, geom)
+    _, expected = ws.reconstruct(_gpu(rc), _gpu(cr))
+    timings = {}
+    power = ensemble_pd_from_channels(rc, cr, angles, starts, grid, geom, method="st_sw_baseband", timings=timings)
+    assert isinstance(power, np.ndarray) and power.dtype == np.float32
+    assert timings["raw_upload_seconds"] >= 0
+    np.testing.assert_array_equal(power, expected.cpu().numpy())
+
+
+def test_input_guards():
+    import torch
+
+    from rcabeam_amd.two_stage import TwoStageOpwWorkspace
+
+    rc, cr, angles, starts, grid, x_el, y_el, fn = _case()
+    ws = TwoStageOpwWorkspace(rc.shape, angles, starts, grid, _geom(x_el, y_el, fn))
+    a, b = _gpu(rc), _gpu(cr)
+    with pytest.raises(ValueError, match="complex64"):
+        ws.reconstruct(a.to(torch.complex128), b)
+    strided = torch.zeros((*a.shape[:-1], 2 * a.shape[-1]), dtype=torch.complex64, device="cuda")[..., ::2]
+    assert strided.shape == a.shape and not strided.is_contiguous()
+    with pytest.raises(ValueError, match="contiguous"):
+        ws.reconstruct(strided, b)
+    with torch.cuda.stream(torch.cuda.Stream()), pytest.raises(ValueError, match="default CUDA stream"):
+        ws.reconstruct(a, b)
+
+
+def test_geometry_snapshot():
+    """Later changes to the caller's arrays do not reach the workspace (reference test_two_stage.py:137-149)."""
+    from rcabeam_amd.two_stage import TwoStageOpwWorkspace
+
+    rc, cr, angles, starts, grid, x_el, y_el, fn = _case()
+    ws = TwoStageOpwWorkspace(rc.shape, angles, starts, grid, _geom(x_el, y_el, fn))
+    expected = ws.reconstruct(_gpu(rc), _gpu(cr)).clone()
+    angles[:] = 0
+    grid[0][:] = 0
+    ws.plane_depths[:] = 0
+    again = ws.reconstruct(_gpu(rc), _gpu(cr))
+    assert bool((again == expected).all())
+    assert ws.plane_depths.min() < grid[1].min() and ws.plane_depths.max() > grid[1].max()
