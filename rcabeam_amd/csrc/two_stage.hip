@@ -47,6 +47,7 @@ constexpr int kWaves = 4;           // waves per workgroup
 constexpr int kKnotBlock = 8;       // knots per plane wave
 constexpr int kChannelGroup = 8;    // channels whose taps one wave forms together (8 x 8 = 64 lanes)
 constexpr int kMaxGroups = 4;       // frame groups of 64 per wave (256 frames)
+constexpr int kXcds = 8;            // MI355X accelerator complex dies
 
 struct PlaneArgs {
   const float2* raw;
@@ -69,13 +70,20 @@ struct PlaneArgs {
 template <int G>
 __global__ __launch_bounds__(256) void ts_planes(PlaneArgs a) {
   const int lane = threadIdx.x & 63;
-  const long long task = static_cast<long long>(blockIdx.x) * kWaves + (threadIdx.x >> 6);
+  // A workgroup is (angle, 4 adjacent lines, knot block): adjacent lines read
+  // nearly the same sample rows, which the CU's L1 then serves.  Workgroups
+  // are dealt to XCDs in contiguous runs (the grid is padded to a multiple of
+  // 8), so neighbouring lines also share an L2.
   const int nkb = (a.npz + kKnotBlock - 1) / kKnotBlock;
-  const long long tasks = static_cast<long long>(a.n_planes) * a.nl * a.na * nkb;
-  if (task >= tasks) return;
-  const int kb = static_cast<int>(task % nkb);
-  const int angle = static_cast<int>((task / nkb) % a.na);
-  const int line = static_cast<int>(task / nkb / a.na);
+  const int lines_total = a.n_planes * a.nl;
+  const int nlg = (lines_total + kWaves - 1) / kWaves;
+  const long long per_xcd = gridDim.x / kXcds;
+  const long long lb = static_cast<long long>(blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
+  if (lb >= static_cast<long long>(a.na) * nlg * nkb) return;
+  const int kb = static_cast<int>(lb % nkb);
+  const int angle = static_cast<int>(lb / nkb / nlg);
+  const int line = static_cast<int>((lb / nkb) % nlg) * kWaves + (threadIdx.x >> 6);
+  if (line >= lines_total) return;
   const int il = line % a.nl;
   const int k0 = kb * kKnotBlock;
   const int nk = min(kKnotBlock, a.npz - k0);
@@ -484,9 +492,10 @@ int frame_groups(int count) { return std::min(kMaxGroups, std::max(1, (count + 6
 rcabeam_status launch_planes(const PlaneArgs& a) {
   const int G = frame_groups(a.count);
   const long long nkb = (a.npz + kKnotBlock - 1) / kKnotBlock;
-  const long long tasks = static_cast<long long>(a.n_planes) * a.nl * a.na * nkb;
-  if (tasks == 0 || a.count == 0) return RCABEAM_SUCCESS;
-  const dim3 grid(static_cast<unsigned>((tasks + kWaves - 1) / kWaves),
+  const long long nlg = (static_cast<long long>(a.n_planes) * a.nl + kWaves - 1) / kWaves;
+  const long long blocks = nlg * a.na * nkb;
+  if (blocks == 0 || a.count == 0) return RCABEAM_SUCCESS;
+  const dim3 grid(static_cast<unsigned>((blocks + kXcds - 1) / kXcds * kXcds),
                   static_cast<unsigned>((a.count + 64 * G - 1) / (64 * G)));
   switch (G) {
     case 1: ts_planes<1><<<grid, 256>>>(a); break;
