@@ -230,6 +230,37 @@ rcabeam_status rcabeam_simulate_channels_device(const double* static_pos, const 
                                                 float t_start, float c, float fs, float fd, float sigma_t,
                                                 float noise_std, unsigned long long seed);
 
+/* ---- Angle-by-angle combiners on a grid.  RC-FMAS (method 2) and St-SW
+ * (method 4) need each angle's channel sum per aperture (reference
+ * src/rcabeam/ensemble.cu:236-315 with METHOD 2, 459-590): run
+ * rcabeam_opw_grid_device once per angle with method 5 (one angle, channels
+ * packed angle-major) and fold each angle's (nv, 2, nt) separate-aperture IQ
+ * into an accumulator with rcabeam_angle_accumulate_device (angles in
+ * increasing order starting at 0; the first touch of a field stores, so the
+ * accumulator needs no clearing), then reduce with
+ * rcabeam_angle_combine_finish_device. */
+
+/* float32 accumulator fields per (voxel, frame): 4 for RC-FMAS, 4 + 6 k for
+ * St-SW with k subsets (2 <= k <= 4); 0 for unsupported arguments.  The
+ * accumulator holds (nv, fields, nt) float32. */
+size_t rcabeam_angle_accumulator_fields(int method, int k);
+
+/* Fold angle `angle`'s separate-aperture IQ (nv, 2, nt) complex64 into acc:
+ * RC-FMAS sums sgnsqrt(RC), sgnsqrt(CR) (reference fmas.py:31-51); St-SW sums
+ * RC, CR and, for subset angle % k, RC, CR, |RC|^2, |CR|^2 (reference
+ * ensemble.cu:487-497). */
+rcabeam_status rcabeam_angle_accumulate_device(const void* apertures, size_t nv, size_t nt, int method, int k,
+                                               int angle, void* acc);
+
+/* Reduce the accumulator over frames.  RC-FMAS: signal (optional, (nv, nt)
+ * complex64) = A B, pd[2v] = mean |A B|^2 (reference ensemble.cu:261-275).
+ * St-SW over na angles: pd[2v] = |mean R conj(Q)|, pd[2v + 1] = its imaginary
+ * part, weights[v] = the unnormalised St-SW weight (reference
+ * ensemble.cu:499-590; the caller divides by the volume maximum as
+ * rcabeam_ensemble_device's callers do). */
+rcabeam_status rcabeam_angle_combine_finish_device(const void* acc, size_t nv, size_t nt, size_t na, int method,
+                                                   int k, float* pd, void* signal, float* weights);
+
 /* ---- Two-stage OPW approximation (reference src/rcabeam/two_stage.py,
  * src/rcabeam/two_stage.cu).  Each aperture is beamformed onto reference
  * planes (receive line x reference depth, one plane per elevation slab), a

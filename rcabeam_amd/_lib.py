@@ -71,6 +71,10 @@ _SIGNATURES: dict[str, list] = {
     "rcabeam_eigh_device": [c_void_p, c_void_p, c_void_p, c_void_p, c_size_t],
     "rcabeam_filtered_power_device": [c_void_p, c_int, c_size_t, c_size_t, c_void_p, c_size_t, c_int, c_void_p],
     "rcabeam_temporal_filter_device": [c_void_p, c_int, c_size_t, c_size_t, c_void_p, c_size_t, c_int, c_void_p],
+    "rcabeam_angle_accumulator_fields": [c_int, c_int],
+    "rcabeam_angle_accumulate_device": [c_void_p, c_size_t, c_size_t, c_int, c_int, c_int, c_void_p],
+    "rcabeam_angle_combine_finish_device": [c_void_p, c_size_t, c_size_t, c_size_t, c_int, c_int, c_void_p, c_void_p,
+                                            c_void_p],
     "rcabeam_two_stage_planes_device": [c_void_p] * 10 + [c_int] * 4 + [c_size_t] * 10 + [c_float] * 4 + [c_int],
     "rcabeam_two_stage_spline_device": [c_void_p] * 4 + [c_size_t] * 5,
     "rcabeam_two_stage_expand_device": [c_void_p] * 7 + [c_int] + [c_size_t] * 11 + [c_float] * 2 + [c_void_p] * 4
@@ -86,6 +90,7 @@ _RESTYPE = {
     "rcabeam_gram_f64_workspace_bytes": c_size_t,
     "rcabeam_simulate_lds_bytes": c_size_t,
     "rcabeam_two_stage_buffer_elements": c_size_t,
+    "rcabeam_angle_accumulator_fields": c_size_t,
 }
 
 

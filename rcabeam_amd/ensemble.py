@@ -36,6 +36,8 @@ GRID_MAX_ANGLES = 16
 GRID_MAX_FRAMES = 208
 # Bound on the St-SW correlation workspace per launch (voxel tiles beyond it).
 _STSW_WORKSPACE_BYTES = 1 << 28
+# Device bytes the angle-by-angle combiners may hold at once (per-angle IQ + accumulators).
+_COMBINE_BYTES = 32 << 30
 
 
 class EnsembleTiming(TypedDict):
@@ -187,6 +189,56 @@ def run_ensemble(packed_rc, packed_cr, g: _Geometry, shape4, method: int, k: int
         )
 
 
+def _per_angle_ok(method: str, storage: str, na: int, nt: int, k: int) -> bool:
+    """RC-FMAS and St-SW of a whole grid run angle by angle on the grid kernel."""
+    if storage != "float32" or nt > GRID_MAX_FRAMES:
+        return False
+    return method == "rc_fmas" or (method == "st_sw" and 2 <= k <= 4 and na >= k)
+
+
+def run_per_angle(raw_rc, raw_cr, g: _Geometry, shape4, method: int, k: int, pd, signal, weights):
+    """RC-FMAS / St-SW of a whole grid, angle by angle (reference ensemble.cu:236-315, 459-590).
+
+    The channels are packed angle-major, (angles, channels, samples, frames), so each angle is a
+    valid one-angle packed ensemble.  For each x-slab of the grid (sized to a device-memory
+    budget, a multiple of the 8-wide lateral tile): one separate-aperture grid launch per angle,
+    each folded into the combiner's accumulator, then the reduction over frames.
+    """
+    t = dev.torch()
+    ns, nc, na, nt = shape4
+    nx, nz, ny = g.shape
+    c, fs, fd, fn = g.constants
+    lib = _lib.load()
+    fields = int(lib.rcabeam_angle_accumulator_fields(method, k))
+    by_angle = [x.permute(2, 1, 0, 3).contiguous() for x in (raw_rc, raw_cr)]
+    per_voxel = nt * (2 * 8 + 4 * fields)
+    budget = min(_COMBINE_BYTES, t.cuda.mem_get_info()[0] // 2)
+    nxs = min(nx, max(1, budget // (per_voxel * nz * ny)))
+    if nxs < nx and nxs >= 8:
+        nxs -= nxs % 8
+    slab = nxs * nz * ny
+    planes = dev.empty((2 * slab, nt), t.complex64)
+    acc = dev.empty((slab * fields * nt,), t.float32)
+    scratch = dev.empty((slab, 2), t.float32)
+    for x0 in range(0, nx, nxs):
+        x1 = min(nx, x0 + nxs)
+        nvs = (x1 - x0) * nz * ny
+        v0 = x0 * nz * ny
+        for a in range(na):
+            _lib.call(
+                "rcabeam_opw_grid_device", dev.ptr(by_angle[0][a]), dev.ptr(by_angle[1][a]), _lib.STORAGE["float32"],
+                _lib.COEF_FLOAT32, dev.ptr(g.axes[0][x0:x1]), dev.ptr(g.axes[1]), dev.ptr(g.axes[2]), x1 - x0, nz, ny,
+                dev.ptr(g.xe), dev.ptr(g.ye), dev.ptr(g.angles[a : a + 1]), dev.ptr(g.starts[a : a + 1]),
+                dev.ptr(scratch), dev.ptr(planes), 0, _METHODS["apertures"], ns, nc, 1, nt, c, fs, fd, fn,
+            )
+            _lib.call("rcabeam_angle_accumulate_device", dev.ptr(planes), nvs, nt, method, k, a, dev.ptr(acc))
+        _lib.call(
+            "rcabeam_angle_combine_finish_device", dev.ptr(acc), nvs, nt, na, method, k, dev.ptr(pd[v0 : v0 + nvs]),
+            dev.ptr(None if signal is None else signal[v0 : v0 + nvs]),
+            dev.ptr(None if weights is None else weights[v0 : v0 + nvs]),
+        )
+
+
 def _run(rc, cr, angles, t_start, grid, geom, method, k, return_signal, timings, iq_storage):
     """Validate, upload, pack, reconstruct and download (ensemble.py:52-130)."""
     _validate(rc, cr, angles, grid, geom, method, k, return_signal, iq_storage)
@@ -209,16 +261,11 @@ def _run(rc, cr, angles, t_start, grid, geom, method, k, return_signal, timings,
     raw_cr = dev.to_device(cr, t.complex64)
     dev.synchronize()
     upload = perf_counter() - begin if host_input else 0.0
-    packed_rc = pack_channels(raw_rc, iq_storage)
-    packed_cr = pack_channels(raw_cr, iq_storage)
-    del raw_rc, raw_cr
-    if method == "st_sw":
-        tile = max(1, _STSW_WORKSPACE_BYTES // (k * k * nt * 8))
-        for first in range(0, nv, tile):
-            run_ensemble(packed_rc, packed_cr, g, (ns, nc, na, nt), code, k, iq_storage, pd, signal, weights,
-                         first, min(tile, nv - first))
+    if _per_angle_ok(method, iq_storage, na, nt, k):
+        run_per_angle(raw_rc, raw_cr, g, (ns, nc, na, nt), code, k, pd, signal, weights)
+        del raw_rc, raw_cr
     else:
-        run_ensemble(packed_rc, packed_cr, g, (ns, nc, na, nt), code, k, iq_storage, pd, signal, weights)
+        _run_packed(raw_rc, raw_cr, g, (ns, nc, na, nt), method, code, k, iq_storage, pd, signal, weights)
     dev.synchronize()
     if timings is not None:
         timings["raw_upload_seconds"] = upload
@@ -231,6 +278,22 @@ def _run(rc, cr, angles, t_start, grid, geom, method, k, return_signal, timings,
         peak = weights.max()
         power = power * (weights / peak) if float(peak) > 0 else power * 0
     return to_host(power.reshape(shape)), None
+
+
+def _run_packed(raw_rc, raw_cr, g, shape4, method, code, k, iq_storage, pd, signal, weights):
+    """Pack and reconstruct through rcabeam_ensemble_device / the grid kernel (every other case)."""
+    ns, nc, na, nt = shape4
+    nv = g.nv
+    packed_rc = pack_channels(raw_rc, iq_storage)
+    packed_cr = pack_channels(raw_cr, iq_storage)
+    del raw_rc, raw_cr
+    if method == "st_sw":
+        tile = max(1, _STSW_WORKSPACE_BYTES // (k * k * nt * 8))
+        for first in range(0, nv, tile):
+            run_ensemble(packed_rc, packed_cr, g, (ns, nc, na, nt), code, k, iq_storage, pd, signal, weights,
+                         first, min(tile, nv - first))
+    else:
+        run_ensemble(packed_rc, packed_cr, g, (ns, nc, na, nt), code, k, iq_storage, pd, signal, weights)
 
 
 def ensemble_pd_from_channels(rc, cr, angles, t_start, grid, geom: RCAGeometry, *, method: Method = "opw",
