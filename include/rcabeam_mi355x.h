@@ -261,6 +261,25 @@ rcabeam_status rcabeam_angle_accumulate_device(const void* apertures, size_t nv,
 rcabeam_status rcabeam_angle_combine_finish_device(const void* acc, size_t nv, size_t nt, size_t na, int method,
                                                    int k, float* pd, void* signal, float* weights);
 
+/* DMAS-CCF-ACF on a grid (reference src/rcabeam/dmas.py:32-85, the DMAS kernel
+ * of src/rcabeam/ensemble.cu:143-234): the accumulator holds (nv, 9, nt)
+ * float32.  Fold in the separate-aperture IQ (nv, 2, nt) of every channel
+ * (per_channel = 1: rcabeam_opw_grid_device method 5 on channel i of both
+ * apertures, all angles; adds sgnsqrt(RC_i) + sgnsqrt(CR_i) and
+ * |RC_i| + |CR_i|) and of every angle (per_channel = 0: one angle, all
+ * channels; adds RC_a, |RC_a|^2, CR_a, |CR_a|^2); first = 1 on the first
+ * call of each kind stores instead of adding. */
+size_t rcabeam_dmas_accumulator_fields(void);
+rcabeam_status rcabeam_dmas_accumulate_device(const void* apertures, size_t nv, size_t nt, int per_channel,
+                                              int first, void* acc);
+
+/* pd[2v] = mean over frames of DMAS x CCF x ACF for the grid (x, z, y) the
+ * accumulator covers; the CCF divides by the channels of both apertures
+ * inside the dynamic aperture (fn <= 0: all). */
+rcabeam_status rcabeam_dmas_finish_device(const void* acc, const float* x, const float* z, const float* y, size_t nx,
+                                          size_t nz, size_t ny, const float* xe, const float* ye, size_t nc,
+                                          size_t na, size_t nt, float fn, float* pd);
+
 /* ---- Two-stage OPW approximation (reference src/rcabeam/two_stage.py,
  * src/rcabeam/two_stage.cu).  Each aperture is beamformed onto reference
  * planes (receive line x reference depth, one plane per elevation slab), a

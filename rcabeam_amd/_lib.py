@@ -75,6 +75,10 @@ _SIGNATURES: dict[str, list] = {
     "rcabeam_angle_accumulate_device": [c_void_p, c_size_t, c_size_t, c_int, c_int, c_int, c_void_p],
     "rcabeam_angle_combine_finish_device": [c_void_p, c_size_t, c_size_t, c_size_t, c_int, c_int, c_void_p, c_void_p,
                                             c_void_p],
+    "rcabeam_dmas_accumulator_fields": [],
+    "rcabeam_dmas_accumulate_device": [c_void_p, c_size_t, c_size_t, c_int, c_int, c_void_p],
+    "rcabeam_dmas_finish_device": [c_void_p] * 4 + [c_size_t] * 3 + [c_void_p] * 2 + [c_size_t] * 3
+    + [c_float, c_void_p],
     "rcabeam_two_stage_planes_device": [c_void_p] * 10 + [c_int] * 4 + [c_size_t] * 10 + [c_float] * 4 + [c_int],
     "rcabeam_two_stage_spline_device": [c_void_p] * 4 + [c_size_t] * 5,
     "rcabeam_two_stage_expand_device": [c_void_p] * 7 + [c_int] + [c_size_t] * 11 + [c_float] * 2 + [c_void_p] * 4
@@ -91,6 +95,7 @@ _RESTYPE = {
     "rcabeam_simulate_lds_bytes": c_size_t,
     "rcabeam_two_stage_buffer_elements": c_size_t,
     "rcabeam_angle_accumulator_fields": c_size_t,
+    "rcabeam_dmas_accumulator_fields": c_size_t,
 }
 
 

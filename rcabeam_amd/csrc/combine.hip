@@ -1,4 +1,5 @@
-// Angle-by-angle combiners for MI355X (gfx950): RC-FMAS and St-SW on a grid.
+// Combiners on a grid from per-angle / per-channel grid launches for MI355X
+// (gfx950): RC-FMAS, St-SW and DMAS-CCF-ACF.
 //
 // RC-FMAS and St-SW need, per voxel and frame, each angle's channel sum of each
 // aperture (reference src/rcabeam/ensemble.cu:236-315 with METHOD 2 and
@@ -22,6 +23,8 @@
 // The first angle that touches a field stores instead of adding, so the
 // accumulators need no clearing.
 #include <hip/hip_runtime.h>
+
+#include <cmath>
 
 #include "common.hpp"
 #include "ensemble.hpp"
@@ -186,6 +189,91 @@ __global__ __launch_bounds__(256) void k_stsw_finish(const float* __restrict__ a
   }
 }
 
+// DMAS-CCF-ACF (reference src/rcabeam/dmas.py:32-85, ensemble.cu:143-234) from
+// two kinds of grid launches: per channel i (both apertures' channel i, all
+// angles: the angle-summed channel signals s) and per angle (each aperture's
+// channel sum of one angle: the ACF's per-angle signals).  Fields, float32
+// [voxel][field][frame]: 0-1 sum sgnsqrt(s), 2 sum |s|, 3-4 sum RC_a,
+// 5 sum |RC_a|^2, 6-7 sum CR_a, 8 sum |CR_a|^2.
+constexpr int kDmasFields = 9;
+
+template <bool PER_CHANNEL>
+__global__ __launch_bounds__(256) void k_dmas_accumulate(const float2* __restrict__ ap, long long nv, int nt,
+                                                         bool first, float* __restrict__ acc) {
+  const long long idx = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (idx >= nv * nt) return;
+  const long long v = idx / nt;
+  const int t = static_cast<int>(idx - v * nt);
+  const float2 r = ap[2 * v * nt + t];
+  const float2 q = ap[(2 * v + 1) * nt + t];
+  float* base = acc + v * kDmasFields * nt + t;
+  auto put = [&](int field, float value) {
+    float* p = base + static_cast<long long>(field) * nt;
+    *p = first ? value : *p + value;
+  };
+  if (PER_CHANNEL) {
+    const float2 sum = cadd(sgnsqrt(r), sgnsqrt(q));
+    put(0, sum.x);
+    put(1, sum.y);
+    put(2, cabs(r) + cabs(q));
+  } else {
+    put(3, r.x);
+    put(4, r.y);
+    put(5, cnorm(r));
+    put(6, q.x);
+    put(7, q.y);
+    put(8, cnorm(q));
+  }
+}
+
+// One wave per voxel of the grid slab (x, z, y): channels inside the dynamic
+// aperture of both apertures (the CCF normaliser, float64 mask as the
+// delay-and-sum kernels apply it), then per frame DMAS x CCF x ACF, averaged.
+__global__ __launch_bounds__(256) void k_dmas_finish(const float* __restrict__ acc, long long nv, int nt, int na,
+                                                     const float* __restrict__ x, const float* __restrict__ z,
+                                                     const float* __restrict__ y, int nz, int ny,
+                                                     const float* __restrict__ xe, const float* __restrict__ ye,
+                                                     int nc, double fn, float* __restrict__ pd) {
+  const long long v = static_cast<long long>(blockIdx.x) * 4 + threadIdx.x / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
+  if (v >= nv) return;
+  const int iy = static_cast<int>(v % ny);
+  const int iz = static_cast<int>((v / ny) % nz);
+  const long long ix = v / ny / nz;
+  const double vx = x[ix], vz = z[iz], vy = y[iy];
+  float active = 0.f;
+  for (int ch = lane; ch < nc; ch += kWave) {
+    const double radius = vz / (2.0 * fn);
+    active += (fn <= 0.0 || fabs(vx - static_cast<double>(xe[ch])) <= radius) ? 1.f : 0.f;
+    active += (fn <= 0.0 || fabs(vy - static_cast<double>(ye[ch])) <= radius) ? 1.f : 0.f;
+  }
+  active = wave_sum(active);
+  const float* base = acc + v * kDmasFields * nt;
+  const float inv = 1.0f / static_cast<float>(na);
+  float total = 0.f;
+  for (int t = lane; t < nt; t += kWave) {
+    const float2 sqrt_sum = make_float2(base[t], base[nt + t]);
+    const float abs_sum = base[2 * nt + t];
+    const float2 r = make_float2(base[3 * nt + t], base[4 * nt + t]);
+    const float2 q = make_float2(base[6 * nt + t], base[7 * nt + t]);
+    const float er = base[5 * nt + t], eq = base[8 * nt + t];
+    const float dm = cnorm(sqrt_sum) - abs_sum;
+    const float wc = (active > 0.f && abs_sum > 0.f) ? fmaxf(dm, 0.f) / active / abs_sum : 0.f;
+    const float nr = cnorm(r) * inv * inv, nq = cnorm(q) * inv * inv;
+    const float var = fmaxf(er * inv - nr, 0.f) + fmaxf(eq * inv - nq, 0.f);
+    const float num = sqrtf(nr) * sqrtf(nq);
+    const float den = nr + nq + 0.5f * var - num;
+    float power = den > 0.f ? dm * wc * (num / den) : 0.f;
+    if (!isfinite(power)) power = 0.f;
+    total += power;
+  }
+  total = wave_sum(total);
+  if (lane == 0) {
+    pd[2 * v] = total / static_cast<float>(nt);
+    pd[2 * v + 1] = 0.f;
+  }
+}
+
 bool combine_args_ok(int method, int k) {
   if (method == kRcFmas) return true;
   return method == kStsw && k >= 2 && k <= kMaxSubsets;
@@ -241,5 +329,36 @@ extern "C" rcabeam_status rcabeam_angle_combine_finish_device(const void* acc, s
       default: k_stsw_finish<4><<<blocks, 256>>>(a, lnv, int_nt, int_na, pd, weights); break;
     }
   }
+  return launch_status();
+}
+
+extern "C" size_t rcabeam_dmas_accumulator_fields(void) { return kDmasFields; }
+
+extern "C" rcabeam_status rcabeam_dmas_accumulate_device(const void* apertures, size_t nv, size_t nt,
+                                                         int per_channel, int first, void* acc) {
+  if (!apertures || !acc || nt < 1) return RCABEAM_ERROR_ARGUMENT;
+  if (nv == 0) return RCABEAM_SUCCESS;
+  const long long n = static_cast<long long>(nv) * static_cast<long long>(nt);
+  const unsigned blocks = static_cast<unsigned>((n + 255) / 256);
+  const float2* ap = static_cast<const float2*>(apertures);
+  float* a = static_cast<float*>(acc);
+  if (per_channel)
+    k_dmas_accumulate<true><<<blocks, 256>>>(ap, static_cast<long long>(nv), static_cast<int>(nt), first != 0, a);
+  else
+    k_dmas_accumulate<false><<<blocks, 256>>>(ap, static_cast<long long>(nv), static_cast<int>(nt), first != 0, a);
+  return launch_status();
+}
+
+extern "C" rcabeam_status rcabeam_dmas_finish_device(const void* acc, const float* x, const float* z, const float* y,
+                                                     size_t nx, size_t nz, size_t ny, const float* xe,
+                                                     const float* ye, size_t nc, size_t na, size_t nt, float fn,
+                                                     float* pd) {
+  if (!acc || !x || !z || !y || !xe || !ye || !pd || nc < 1 || na < 1 || nt < 1 || std::isnan(fn))
+    return RCABEAM_ERROR_ARGUMENT;
+  const size_t nv = nx * nz * ny;
+  if (nv == 0) return RCABEAM_SUCCESS;
+  k_dmas_finish<<<static_cast<unsigned>((nv + 3) / 4), 256>>>(
+      static_cast<const float*>(acc), static_cast<long long>(nv), static_cast<int>(nt), static_cast<int>(na), x, z, y,
+      static_cast<int>(nz), static_cast<int>(ny), xe, ye, static_cast<int>(nc), static_cast<double>(fn), pd);
   return launch_status();
 }

@@ -37,7 +37,7 @@ GRID_MAX_FRAMES = 208
 # Bound on the St-SW correlation workspace per launch (voxel tiles beyond it).
 _STSW_WORKSPACE_BYTES = 1 << 28
 # Device bytes the angle-by-angle combiners may hold at once (per-angle IQ + accumulators).
-_COMBINE_BYTES = 32 << 30
+_COMBINE_BYTES = 64 << 30
 
 
 class EnsembleTiming(TypedDict):
@@ -239,6 +239,59 @@ def run_per_angle(raw_rc, raw_cr, g: _Geometry, shape4, method: int, k: int, pd,
         )
 
 
+def run_dmas_grid(raw_rc, raw_cr, g: _Geometry, shape4, pd):
+    """DMAS-CCF-ACF of a whole grid from grid-kernel launches (reference dmas.py:32-85).
+
+    In the packed (channels, angles, samples, frames) layout channel i of each aperture is itself a
+    valid one-channel ensemble: one separate-aperture grid launch per channel gives both
+    apertures' angle-summed channel signals (the DMAS and CCF terms), and one launch per angle on
+    angle-major channels gives the per-angle signals (the ACF).  x-slabs bound the accumulator.
+    """
+    t = dev.torch()
+    ns, nc, na, nt = shape4
+    nx, nz, ny = g.shape
+    c, fs, fd, fn = g.constants
+    lib = _lib.load()
+    fields = int(lib.rcabeam_dmas_accumulator_fields())
+    packed = [pack_channels(x, "float32") for x in (raw_rc, raw_cr)]
+    by_angle = [x.permute(2, 1, 0, 3).contiguous() for x in (raw_rc, raw_cr)]
+    per_voxel = nt * (2 * 8 + 4 * fields)
+    budget = min(_COMBINE_BYTES, t.cuda.mem_get_info()[0] // 2)
+    nxs = min(nx, max(1, budget // (per_voxel * nz * ny)))
+    if nxs < nx and nxs >= 8:
+        nxs -= nxs % 8
+    slab = nxs * nz * ny
+    planes = dev.empty((2 * slab, nt), t.complex64)
+    acc = dev.empty((slab * fields * nt,), t.float32)
+    scratch = dev.empty((slab, 2), t.float32)
+    plane_elems = na * ns * nt
+
+    def grid(rc_ptr, cr_ptr, x0, x1, xe, ye, angles, starts, channels, angle_count):
+        _lib.call(
+            "rcabeam_opw_grid_device", rc_ptr, cr_ptr, _lib.STORAGE["float32"], _lib.COEF_FLOAT32,
+            dev.ptr(g.axes[0][x0:x1]), dev.ptr(g.axes[1]), dev.ptr(g.axes[2]), x1 - x0, nz, ny, xe, ye, angles,
+            starts, dev.ptr(scratch), dev.ptr(planes), 0, _METHODS["apertures"], ns, channels, angle_count, nt, c, fs,
+            fd, fn,
+        )
+
+    for x0 in range(0, nx, nxs):
+        x1 = min(nx, x0 + nxs)
+        nvs = (x1 - x0) * nz * ny
+        v0 = x0 * nz * ny
+        for i in range(nc):
+            offset = i * plane_elems * 8
+            grid(dev.ptr(packed[0]) + offset, dev.ptr(packed[1]) + offset, x0, x1, dev.ptr(g.xe[i : i + 1]),
+                 dev.ptr(g.ye[i : i + 1]), dev.ptr(g.angles), dev.ptr(g.starts), 1, na)
+            _lib.call("rcabeam_dmas_accumulate_device", dev.ptr(planes), nvs, nt, 1, int(i == 0), dev.ptr(acc))
+        for a in range(na):
+            grid(dev.ptr(by_angle[0][a]), dev.ptr(by_angle[1][a]), x0, x1, dev.ptr(g.xe), dev.ptr(g.ye),
+                 dev.ptr(g.angles[a : a + 1]), dev.ptr(g.starts[a : a + 1]), nc, 1)
+            _lib.call("rcabeam_dmas_accumulate_device", dev.ptr(planes), nvs, nt, 0, int(a == 0), dev.ptr(acc))
+        _lib.call("rcabeam_dmas_finish_device", dev.ptr(acc), dev.ptr(g.axes[0][x0:x1]), dev.ptr(g.axes[1]),
+                  dev.ptr(g.axes[2]), x1 - x0, nz, ny, dev.ptr(g.xe), dev.ptr(g.ye), nc, na, nt, fn,
+                  dev.ptr(pd[v0 : v0 + nvs]))
+
+
 def _run(rc, cr, angles, t_start, grid, geom, method, k, return_signal, timings, iq_storage):
     """Validate, upload, pack, reconstruct and download (ensemble.py:52-130)."""
     _validate(rc, cr, angles, grid, geom, method, k, return_signal, iq_storage)
@@ -263,6 +316,9 @@ def _run(rc, cr, angles, t_start, grid, geom, method, k, return_signal, timings,
     upload = perf_counter() - begin if host_input else 0.0
     if _per_angle_ok(method, iq_storage, na, nt, k):
         run_per_angle(raw_rc, raw_cr, g, (ns, nc, na, nt), code, k, pd, signal, weights)
+        del raw_rc, raw_cr
+    elif method == "dmas" and iq_storage == "float32" and na <= GRID_MAX_ANGLES and nt <= GRID_MAX_FRAMES:
+        run_dmas_grid(raw_rc, raw_cr, g, (ns, nc, na, nt), pd)
         del raw_rc, raw_cr
     else:
         _run_packed(raw_rc, raw_cr, g, (ns, nc, na, nt), method, code, k, iq_storage, pd, signal, weights)

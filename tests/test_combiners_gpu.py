@@ -1,4 +1,5 @@
-"""RC-FMAS and St-SW of a whole grid, angle by angle on the grid kernel, against the float64 oracle.
+"""RC-FMAS, St-SW and DMAS-CCF-ACF of a whole grid from per-angle / per-channel grid launches, against
+the float64 oracle.
 
 Bar: relative L2 error 1e-5 (float32 storage), as the ensemble parity tests.
 """
@@ -43,6 +44,8 @@ def test_slabs_match_oracle(monkeypatch, budget):
     for k in (2, 4):
         power, _ = combiners.ensemble_power("st_sw", rc_res, cr_res, k=k)
         assert rel_l2(rb.ensemble_pd_from_channels(*args, method="st_sw", k=k).ravel(), power) < TOL, k
+    power, _ = combiners.ensemble_power("dmas", rc_res, cr_res)
+    assert rel_l2(rb.ensemble_pd_from_channels(*args, method="dmas").ravel(), power) < TOL
 
 
 def test_many_angles():
@@ -59,6 +62,6 @@ def test_frames_beyond_grid_limit_keep_voxel_list_path():
     """210 frames (more than the grid kernel's 208) run the voxel-list kernels, still matching the oracle."""
     acq, geom, (rc_res, cr_res) = _case(4, 210, (4, 3, 4), 3)
     args = (acq.rc, acq.cr, acq.angles, acq.t_start, acq.grid, geom)
-    for method, k in (("rc_fmas", 2), ("st_sw", 4)):
+    for method, k in (("rc_fmas", 2), ("st_sw", 4), ("dmas", 2)):
         power, _ = combiners.ensemble_power(method, rc_res, cr_res, k=k)
         assert rel_l2(rb.ensemble_pd_from_channels(*args, method=method, k=k).ravel(), power) < TOL, method
