@@ -74,6 +74,8 @@ def parse():
     p.add_argument("--no-two-stage", action="store_true", help="skip the full-size two-stage approximation timings")
     p.add_argument("--two-stage-only", action="store_true",
                    help="only time the two-stage approximation on the full-size scene (profiling runs)")
+    p.add_argument("--combiners-only", action="store_true",
+                   help="only time the combiners on the full-size scene (profiling runs)")
     p.add_argument("--cpu-voxels", type=int, default=2400)
     p.add_argument("--copy-calibration", action="store_true",
                    help="only run a plain 1 GiB device-to-device copy three times (HBM counter calibration)")
@@ -490,6 +492,9 @@ def main() -> int:
     scene, rc, cr = mixture_scene(grid_shape=tuple(args.grid))
     if args.two_stage_only:
         print(json.dumps({"two_stage": two_stage_timing(scene, rc, cr)}), flush=True)
+        return 0
+    if args.combiners_only:
+        print(json.dumps({"combiners": combiner_timing(scene, rc, cr)}), flush=True)
         return 0
     storage = args.precision
     pipe = Pipeline(scene, rc, cr, storage, rank_start, args.lowrank_modes)
