@@ -268,10 +268,12 @@ rcabeam_status rcabeam_angle_combine_finish_device(const void* acc, size_t nv, s
  * apertures, all angles; adds sgnsqrt(RC_i) + sgnsqrt(CR_i) and
  * |RC_i| + |CR_i|) and of every angle (per_channel = 0: one angle, all
  * channels; adds RC_a, |RC_a|^2, CR_a, |CR_a|^2); first = 1 on the first
- * call of each kind stores instead of adding. */
+ * call of each kind stores instead of adding.  Per channel, `count` (<= 8)
+ * outputs `stride` complex64 apart are folded in one pass; per angle,
+ * count = 1. */
 size_t rcabeam_dmas_accumulator_fields(void);
-rcabeam_status rcabeam_dmas_accumulate_device(const void* apertures, size_t nv, size_t nt, int per_channel,
-                                              int first, void* acc);
+rcabeam_status rcabeam_dmas_accumulate_device(const void* apertures, size_t count, size_t stride, size_t nv,
+                                              size_t nt, int per_channel, int first, void* acc);
 
 /* pd[2v] = mean over frames of DMAS x CCF x ACF for the grid (x, z, y) the
  * accumulator covers; the CCF divides by the channels of both apertures

@@ -76,7 +76,7 @@ _SIGNATURES: dict[str, list] = {
     "rcabeam_angle_combine_finish_device": [c_void_p, c_size_t, c_size_t, c_size_t, c_int, c_int, c_void_p, c_void_p,
                                             c_void_p],
     "rcabeam_dmas_accumulator_fields": [],
-    "rcabeam_dmas_accumulate_device": [c_void_p, c_size_t, c_size_t, c_int, c_int, c_void_p],
+    "rcabeam_dmas_accumulate_device": [c_void_p] + [c_size_t] * 4 + [c_int, c_int, c_void_p],
     "rcabeam_dmas_finish_device": [c_void_p] * 4 + [c_size_t] * 3 + [c_void_p] * 2 + [c_size_t] * 3
     + [c_float, c_void_p],
     "rcabeam_two_stage_planes_device": [c_void_p] * 10 + [c_int] * 4 + [c_size_t] * 10 + [c_float] * 4 + [c_int],

@@ -197,26 +197,38 @@ __global__ __launch_bounds__(256) void k_stsw_finish(const float* __restrict__ a
 // 5 sum |RC_a|^2, 6-7 sum CR_a, 8 sum |CR_a|^2.
 constexpr int kDmasFields = 9;
 
+constexpr int kDmasMaxBatch = 8;
+
+// PER_CHANNEL folds `count` separate-aperture outputs (one per channel, `stride`
+// complex64 apart) in one pass over the accumulator.
 template <bool PER_CHANNEL>
-__global__ __launch_bounds__(256) void k_dmas_accumulate(const float2* __restrict__ ap, long long nv, int nt,
-                                                         bool first, float* __restrict__ acc) {
+__global__ __launch_bounds__(256) void k_dmas_accumulate(const float2* __restrict__ ap, long long stride, int count,
+                                                         long long nv, int nt, bool first, float* __restrict__ acc) {
   const long long idx = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (idx >= nv * nt) return;
   const long long v = idx / nt;
   const int t = static_cast<int>(idx - v * nt);
-  const float2 r = ap[2 * v * nt + t];
-  const float2 q = ap[(2 * v + 1) * nt + t];
   float* base = acc + v * kDmasFields * nt + t;
   auto put = [&](int field, float value) {
     float* p = base + static_cast<long long>(field) * nt;
     *p = first ? value : *p + value;
   };
   if (PER_CHANNEL) {
-    const float2 sum = cadd(sgnsqrt(r), sgnsqrt(q));
+    float2 sum = make_float2(0.f, 0.f);
+    float mag = 0.f;
+    for (int b = 0; b < count; ++b) {
+      const float2* a = ap + b * stride;
+      const float2 r = a[2 * v * nt + t];
+      const float2 q = a[(2 * v + 1) * nt + t];
+      sum = cadd(sum, cadd(sgnsqrt(r), sgnsqrt(q)));
+      mag += cabs(r) + cabs(q);
+    }
     put(0, sum.x);
     put(1, sum.y);
-    put(2, cabs(r) + cabs(q));
+    put(2, mag);
   } else {
+    const float2 r = ap[2 * v * nt + t];
+    const float2 q = ap[(2 * v + 1) * nt + t];
     put(3, r.x);
     put(4, r.y);
     put(5, cnorm(r));
@@ -334,18 +346,24 @@ extern "C" rcabeam_status rcabeam_angle_combine_finish_device(const void* acc, s
 
 extern "C" size_t rcabeam_dmas_accumulator_fields(void) { return kDmasFields; }
 
-extern "C" rcabeam_status rcabeam_dmas_accumulate_device(const void* apertures, size_t nv, size_t nt,
-                                                         int per_channel, int first, void* acc) {
-  if (!apertures || !acc || nt < 1) return RCABEAM_ERROR_ARGUMENT;
+extern "C" rcabeam_status rcabeam_dmas_accumulate_device(const void* apertures, size_t count, size_t stride,
+                                                         size_t nv, size_t nt, int per_channel, int first,
+                                                         void* acc) {
+  if (!apertures || !acc || nt < 1 || count < 1 || count > kDmasMaxBatch || (!per_channel && count != 1) ||
+      (count > 1 && stride < 2 * nv * nt))
+    return RCABEAM_ERROR_ARGUMENT;
   if (nv == 0) return RCABEAM_SUCCESS;
   const long long n = static_cast<long long>(nv) * static_cast<long long>(nt);
   const unsigned blocks = static_cast<unsigned>((n + 255) / 256);
   const float2* ap = static_cast<const float2*>(apertures);
   float* a = static_cast<float*>(acc);
+  const long long ls = static_cast<long long>(stride);
+  const int c = static_cast<int>(count);
   if (per_channel)
-    k_dmas_accumulate<true><<<blocks, 256>>>(ap, static_cast<long long>(nv), static_cast<int>(nt), first != 0, a);
+    k_dmas_accumulate<true><<<blocks, 256>>>(ap, ls, c, static_cast<long long>(nv), static_cast<int>(nt), first != 0, a);
   else
-    k_dmas_accumulate<false><<<blocks, 256>>>(ap, static_cast<long long>(nv), static_cast<int>(nt), first != 0, a);
+    k_dmas_accumulate<false><<<blocks, 256>>>(ap, ls, c, static_cast<long long>(nv), static_cast<int>(nt), first != 0,
+                                              a);
   return launch_status();
 }
 

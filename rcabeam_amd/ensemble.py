@@ -37,7 +37,9 @@ GRID_MAX_FRAMES = 208
 # Bound on the St-SW correlation workspace per launch (voxel tiles beyond it).
 _STSW_WORKSPACE_BYTES = 1 << 28
 # Device bytes the angle-by-angle combiners may hold at once (per-angle IQ + accumulators).
-_COMBINE_BYTES = 64 << 30
+_COMBINE_BYTES = 96 << 30
+# Channels per DMAS accumulator pass.
+_DMAS_BATCH = 4
 
 
 class EnsembleTiming(TypedDict):
@@ -255,22 +257,23 @@ def run_dmas_grid(raw_rc, raw_cr, g: _Geometry, shape4, pd):
     fields = int(lib.rcabeam_dmas_accumulator_fields())
     packed = [pack_channels(x, "float32") for x in (raw_rc, raw_cr)]
     by_angle = [x.permute(2, 1, 0, 3).contiguous() for x in (raw_rc, raw_cr)]
-    per_voxel = nt * (2 * 8 + 4 * fields)
+    batch = _DMAS_BATCH  # channels whose outputs are folded in one pass over the accumulator
+    per_voxel = nt * (batch * 2 * 8 + 4 * fields)
     budget = min(_COMBINE_BYTES, t.cuda.mem_get_info()[0] // 2)
     nxs = min(nx, max(1, budget // (per_voxel * nz * ny)))
     if nxs < nx and nxs >= 8:
         nxs -= nxs % 8
     slab = nxs * nz * ny
-    planes = dev.empty((2 * slab, nt), t.complex64)
+    planes = dev.empty((batch, 2 * slab, nt), t.complex64)
     acc = dev.empty((slab * fields * nt,), t.float32)
     scratch = dev.empty((slab, 2), t.float32)
     plane_elems = na * ns * nt
 
-    def grid(rc_ptr, cr_ptr, x0, x1, xe, ye, angles, starts, channels, angle_count):
+    def grid(rc_ptr, cr_ptr, x0, x1, xe, ye, angles, starts, channels, angle_count, out):
         _lib.call(
             "rcabeam_opw_grid_device", rc_ptr, cr_ptr, _lib.STORAGE["float32"], _lib.COEF_FLOAT32,
             dev.ptr(g.axes[0][x0:x1]), dev.ptr(g.axes[1]), dev.ptr(g.axes[2]), x1 - x0, nz, ny, xe, ye, angles,
-            starts, dev.ptr(scratch), dev.ptr(planes), 0, _METHODS["apertures"], ns, channels, angle_count, nt, c, fs,
+            starts, dev.ptr(scratch), dev.ptr(out), 0, _METHODS["apertures"], ns, channels, angle_count, nt, c, fs,
             fd, fn,
         )
 
@@ -278,15 +281,20 @@ def run_dmas_grid(raw_rc, raw_cr, g: _Geometry, shape4, pd):
         x1 = min(nx, x0 + nxs)
         nvs = (x1 - x0) * nz * ny
         v0 = x0 * nz * ny
-        for i in range(nc):
-            offset = i * plane_elems * 8
-            grid(dev.ptr(packed[0]) + offset, dev.ptr(packed[1]) + offset, x0, x1, dev.ptr(g.xe[i : i + 1]),
-                 dev.ptr(g.ye[i : i + 1]), dev.ptr(g.angles), dev.ptr(g.starts), 1, na)
-            _lib.call("rcabeam_dmas_accumulate_device", dev.ptr(planes), nvs, nt, 1, int(i == 0), dev.ptr(acc))
+        for i0 in range(0, nc, batch):
+            count = min(batch, nc - i0)
+            for b in range(count):
+                offset = (i0 + b) * plane_elems * 8
+                grid(dev.ptr(packed[0]) + offset, dev.ptr(packed[1]) + offset, x0, x1,
+                     dev.ptr(g.xe[i0 + b : i0 + b + 1]), dev.ptr(g.ye[i0 + b : i0 + b + 1]), dev.ptr(g.angles),
+                     dev.ptr(g.starts), 1, na, planes[b])
+            _lib.call("rcabeam_dmas_accumulate_device", dev.ptr(planes), count, 2 * slab * nt, nvs, nt, 1,
+                      int(i0 == 0), dev.ptr(acc))
         for a in range(na):
             grid(dev.ptr(by_angle[0][a]), dev.ptr(by_angle[1][a]), x0, x1, dev.ptr(g.xe), dev.ptr(g.ye),
-                 dev.ptr(g.angles[a : a + 1]), dev.ptr(g.starts[a : a + 1]), nc, 1)
-            _lib.call("rcabeam_dmas_accumulate_device", dev.ptr(planes), nvs, nt, 0, int(a == 0), dev.ptr(acc))
+                 dev.ptr(g.angles[a : a + 1]), dev.ptr(g.starts[a : a + 1]), nc, 1, planes[0])
+            _lib.call("rcabeam_dmas_accumulate_device", dev.ptr(planes), 1, 2 * slab * nt, nvs, nt, 0, int(a == 0),
+                      dev.ptr(acc))
         _lib.call("rcabeam_dmas_finish_device", dev.ptr(acc), dev.ptr(g.axes[0][x0:x1]), dev.ptr(g.axes[1]),
                   dev.ptr(g.axes[2]), x1 - x0, nz, ny, dev.ptr(g.xe), dev.ptr(g.ye), nc, na, nt, fn,
                   dev.ptr(pd[v0 : v0 + nvs]))

@@ -34,7 +34,7 @@ def test_slabs_match_oracle(monkeypatch, budget):
 
     acq, geom, (rc_res, cr_res) = _case(4, 24, (20, 6, 9), 11)
     if budget is not None:
-        # One byte of budget: the smallest slab, 8 lines of x when the grid is wider.
+        # One byte of budget: one line of x per slab.
         monkeypatch.setattr(ensemble, "_COMBINE_BYTES", budget)
     print(f"case: slabs budget={budget}", flush=True)
     args = (acq.rc, acq.cr, acq.angles, acq.t_start, acq.grid, geom)
