@@ -245,12 +245,13 @@ rcabeam_status rcabeam_simulate_channels_device(const double* static_pos, const 
  * accumulator holds (nv, fields, nt) float32. */
 size_t rcabeam_angle_accumulator_fields(int method, int k);
 
-/* Fold angle `angle`'s separate-aperture IQ (nv, 2, nt) complex64 into acc:
- * RC-FMAS sums sgnsqrt(RC), sgnsqrt(CR) (reference fmas.py:31-51); St-SW sums
- * RC, CR and, for subset angle % k, RC, CR, |RC|^2, |CR|^2 (reference
- * ensemble.cu:487-497). */
-rcabeam_status rcabeam_angle_accumulate_device(const void* apertures, size_t nv, size_t nt, int method, int k,
-                                               int angle, void* acc);
+/* Fold angles first_angle .. first_angle + count - 1 (count <= 8; their
+ * separate-aperture IQ (nv, 2, nt) complex64, `stride` complex64 apart) into
+ * acc in one pass: RC-FMAS sums sgnsqrt(RC), sgnsqrt(CR) (reference
+ * fmas.py:31-51); St-SW sums RC, CR and, for subset angle % k, RC, CR,
+ * |RC|^2, |CR|^2 (reference ensemble.cu:487-497). */
+rcabeam_status rcabeam_angle_accumulate_device(const void* apertures, size_t count, size_t stride, size_t nv,
+                                               size_t nt, int method, int k, int first_angle, void* acc);
 
 /* Reduce the accumulator over frames.  RC-FMAS: signal (optional, (nv, nt)
  * complex64) = A B, pd[2v] = mean |A B|^2 (reference ensemble.cu:261-275).

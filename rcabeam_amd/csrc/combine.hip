@@ -37,41 +37,71 @@ constexpr int kMaxSubsets = 4;
 
 __host__ __device__ constexpr int accumulator_fields(int method, int k) { return method == kRcFmas ? 4 : 4 + 6 * k; }
 
+constexpr int kAngleMaxBatch = 8;
+
+// Fold `count` consecutive angles a0 .. a0 + count - 1 (their separate-aperture
+// outputs `stride` complex64 apart) in one pass over the accumulator.
 template <int METHOD, int K>
-__global__ __launch_bounds__(256) void k_angle_accumulate(const float2* __restrict__ ap, long long nv, int nt,
-                                                          int angle, float* __restrict__ acc) {
+__global__ __launch_bounds__(256) void k_angle_accumulate(const float2* __restrict__ ap, long long stride,
+                                                          int count, long long nv, int nt, int a0,
+                                                          float* __restrict__ acc) {
   constexpr int F = accumulator_fields(METHOD, K);
   const long long idx = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (idx >= nv * nt) return;
   const long long v = idx / nt;
   const int t = static_cast<int>(idx - v * nt);
-  const float2 r = ap[2 * v * nt + t];
-  const float2 q = ap[(2 * v + 1) * nt + t];
   float* base = acc + v * F * nt + t;
   auto put = [&](int field, float value, bool first) {
     float* p = base + static_cast<long long>(field) * nt;
     *p = first ? value : *p + value;
   };
-  if (METHOD == kRcFmas) {
-    const float2 a = sgnsqrt(r), b = sgnsqrt(q);
-    const bool first = angle == 0;
-    put(0, a.x, first);
-    put(1, a.y, first);
-    put(2, b.x, first);
-    put(3, b.y, first);
-  } else {
-    put(0, r.x, angle == 0);
-    put(1, r.y, angle == 0);
-    put(2, q.x, angle == 0);
-    put(3, q.y, angle == 0);
-    const int s = angle % K;
-    const bool first = angle < K;
-    put(4 + 6 * s, r.x, first);
-    put(5 + 6 * s, r.y, first);
-    put(6 + 6 * s, q.x, first);
-    put(7 + 6 * s, q.y, first);
-    put(8 + 6 * s, cnorm(r), first);
-    put(9 + 6 * s, cnorm(q), first);
+  float2 r_sum = make_float2(0.f, 0.f), q_sum = make_float2(0.f, 0.f);
+  float2 mr[K], mq[K];
+  float er[K], eq[K];
+#pragma unroll
+  for (int s = 0; s < K; ++s) {
+    mr[s] = mq[s] = make_float2(0.f, 0.f);
+    er[s] = eq[s] = 0.f;
+  }
+  for (int b = 0; b < count; ++b) {
+    const float2* a = ap + b * stride;
+    const float2 r = a[2 * v * nt + t];
+    const float2 q = a[(2 * v + 1) * nt + t];
+    if (METHOD == kRcFmas) {
+      r_sum = cadd(r_sum, sgnsqrt(r));
+      q_sum = cadd(q_sum, sgnsqrt(q));
+    } else {
+      r_sum = cadd(r_sum, r);
+      q_sum = cadd(q_sum, q);
+      const int sub = (a0 + b) % K;
+#pragma unroll
+      for (int s = 0; s < K; ++s)
+        if (s == sub) {
+          mr[s] = cadd(mr[s], r);
+          mq[s] = cadd(mq[s], q);
+          er[s] += cnorm(r);
+          eq[s] += cnorm(q);
+        }
+    }
+  }
+  put(0, r_sum.x, a0 == 0);
+  put(1, r_sum.y, a0 == 0);
+  put(2, q_sum.x, a0 == 0);
+  put(3, q_sum.y, a0 == 0);
+  if (METHOD == kStsw) {
+#pragma unroll
+    for (int s = 0; s < K; ++s) {
+      // Subset s holds the angles a = s mod K; its first is angle s.
+      const bool touched = count >= K || ((s - a0 % K + K) % K) < count;
+      if (!touched) continue;
+      const bool first = s >= a0 && s < a0 + count;
+      put(4 + 6 * s, mr[s].x, first);
+      put(5 + 6 * s, mr[s].y, first);
+      put(6 + 6 * s, mq[s].x, first);
+      put(7 + 6 * s, mq[s].y, first);
+      put(8 + 6 * s, er[s], first);
+      put(9 + 6 * s, eq[s], first);
+    }
   }
 }
 
@@ -300,23 +330,26 @@ extern "C" size_t rcabeam_angle_accumulator_fields(int method, int k) {
   return combine_args_ok(method, k) ? static_cast<size_t>(accumulator_fields(method, k)) : 0;
 }
 
-extern "C" rcabeam_status rcabeam_angle_accumulate_device(const void* apertures, size_t nv, size_t nt, int method,
-                                                          int k, int angle, void* acc) {
-  if (!apertures || !acc || nt < 1 || angle < 0 || !combine_args_ok(method, k)) return RCABEAM_ERROR_ARGUMENT;
+extern "C" rcabeam_status rcabeam_angle_accumulate_device(const void* apertures, size_t count, size_t stride,
+                                                          size_t nv, size_t nt, int method, int k, int first_angle,
+                                                          void* acc) {
+  if (!apertures || !acc || nt < 1 || first_angle < 0 || count < 1 || count > kAngleMaxBatch ||
+      (count > 1 && stride < 2 * nv * nt) || !combine_args_ok(method, k))
+    return RCABEAM_ERROR_ARGUMENT;
   if (nv == 0) return RCABEAM_SUCCESS;
   const long long n = static_cast<long long>(nv) * static_cast<long long>(nt);
   const unsigned blocks = static_cast<unsigned>((n + 255) / 256);
   const float2* ap = static_cast<const float2*>(apertures);
   float* a = static_cast<float*>(acc);
-  const long long lnv = static_cast<long long>(nv);
-  const int int_nt = static_cast<int>(nt);
+  const long long lnv = static_cast<long long>(nv), ls = static_cast<long long>(stride);
+  const int int_nt = static_cast<int>(nt), c = static_cast<int>(count);
   if (method == kRcFmas) {
-    k_angle_accumulate<kRcFmas, 1><<<blocks, 256>>>(ap, lnv, int_nt, angle, a);
+    k_angle_accumulate<kRcFmas, 1><<<blocks, 256>>>(ap, ls, c, lnv, int_nt, first_angle, a);
   } else {
     switch (k) {
-      case 2: k_angle_accumulate<kStsw, 2><<<blocks, 256>>>(ap, lnv, int_nt, angle, a); break;
-      case 3: k_angle_accumulate<kStsw, 3><<<blocks, 256>>>(ap, lnv, int_nt, angle, a); break;
-      default: k_angle_accumulate<kStsw, 4><<<blocks, 256>>>(ap, lnv, int_nt, angle, a); break;
+      case 2: k_angle_accumulate<kStsw, 2><<<blocks, 256>>>(ap, ls, c, lnv, int_nt, first_angle, a); break;
+      case 3: k_angle_accumulate<kStsw, 3><<<blocks, 256>>>(ap, ls, c, lnv, int_nt, first_angle, a); break;
+      default: k_angle_accumulate<kStsw, 4><<<blocks, 256>>>(ap, ls, c, lnv, int_nt, first_angle, a); break;
     }
   }
   return launch_status();

@@ -213,27 +213,34 @@ def run_per_angle(raw_rc, raw_cr, g: _Geometry, shape4, method: int, k: int, pd,
     lib = _lib.load()
     fields = int(lib.rcabeam_angle_accumulator_fields(method, k))
     by_angle = [x.permute(2, 1, 0, 3).contiguous() for x in (raw_rc, raw_cr)]
-    per_voxel = nt * (2 * 8 + 4 * fields)
+    # Angles folded per accumulator pass: k for St-SW (each subset once per pass), 4 for RC-FMAS.
+    batch = k if method == _METHODS["st_sw"] else 4
+    per_voxel = nt * (batch * 2 * 8 + 4 * fields)
     budget = min(_COMBINE_BYTES, t.cuda.mem_get_info()[0] // 2)
     nxs = min(nx, max(1, budget // (per_voxel * nz * ny)))
     if nxs < nx and nxs >= 8:
         nxs -= nxs % 8
     slab = nxs * nz * ny
-    planes = dev.empty((2 * slab, nt), t.complex64)
+    planes = dev.empty((batch, 2 * slab, nt), t.complex64)
     acc = dev.empty((slab * fields * nt,), t.float32)
     scratch = dev.empty((slab, 2), t.float32)
     for x0 in range(0, nx, nxs):
         x1 = min(nx, x0 + nxs)
         nvs = (x1 - x0) * nz * ny
         v0 = x0 * nz * ny
-        for a in range(na):
-            _lib.call(
-                "rcabeam_opw_grid_device", dev.ptr(by_angle[0][a]), dev.ptr(by_angle[1][a]), _lib.STORAGE["float32"],
-                _lib.COEF_FLOAT32, dev.ptr(g.axes[0][x0:x1]), dev.ptr(g.axes[1]), dev.ptr(g.axes[2]), x1 - x0, nz, ny,
-                dev.ptr(g.xe), dev.ptr(g.ye), dev.ptr(g.angles[a : a + 1]), dev.ptr(g.starts[a : a + 1]),
-                dev.ptr(scratch), dev.ptr(planes), 0, _METHODS["apertures"], ns, nc, 1, nt, c, fs, fd, fn,
-            )
-            _lib.call("rcabeam_angle_accumulate_device", dev.ptr(planes), nvs, nt, method, k, a, dev.ptr(acc))
+        for a0 in range(0, na, batch):
+            count = min(batch, na - a0)
+            for b in range(count):
+                a = a0 + b
+                _lib.call(
+                    "rcabeam_opw_grid_device", dev.ptr(by_angle[0][a]), dev.ptr(by_angle[1][a]),
+                    _lib.STORAGE["float32"], _lib.COEF_FLOAT32, dev.ptr(g.axes[0][x0:x1]), dev.ptr(g.axes[1]),
+                    dev.ptr(g.axes[2]), x1 - x0, nz, ny, dev.ptr(g.xe), dev.ptr(g.ye), dev.ptr(g.angles[a : a + 1]),
+                    dev.ptr(g.starts[a : a + 1]), dev.ptr(scratch), dev.ptr(planes[b]), 0, _METHODS["apertures"], ns,
+                    nc, 1, nt, c, fs, fd, fn,
+                )
+            _lib.call("rcabeam_angle_accumulate_device", dev.ptr(planes), count, 2 * slab * nt, nvs, nt, method, k,
+                      a0, dev.ptr(acc))
         _lib.call(
             "rcabeam_angle_combine_finish_device", dev.ptr(acc), nvs, nt, na, method, k, dev.ptr(pd[v0 : v0 + nvs]),
             dev.ptr(None if signal is None else signal[v0 : v0 + nvs]),
