@@ -198,6 +198,17 @@ def _per_angle_ok(method: str, storage: str, na: int, nt: int, k: int) -> bool:
     return method == "rc_fmas" or (method == "st_sw" and 2 <= k <= 4 and na >= k)
 
 
+def _slab_lines(shape, bytes_per_voxel: int) -> int:
+    """Lines of x per slab so that the combiners' per-voxel buffers fit the device-memory budget
+    (a multiple of the 8-wide lateral tile when the grid needs more than one slab)."""
+    nx, nz, ny = shape
+    budget = min(_COMBINE_BYTES, dev.torch().cuda.mem_get_info()[0] // 2)
+    lines = min(nx, max(1, budget // (bytes_per_voxel * nz * ny)))
+    if lines < nx and lines >= 8:
+        lines -= lines % 8
+    return lines
+
+
 def run_per_angle(raw_rc, raw_cr, g: _Geometry, shape4, method: int, k: int, pd, signal, weights):
     """RC-FMAS / St-SW of a whole grid, angle by angle (reference ensemble.cu:236-315, 459-590).
 
@@ -215,11 +226,7 @@ def run_per_angle(raw_rc, raw_cr, g: _Geometry, shape4, method: int, k: int, pd,
     by_angle = [x.permute(2, 1, 0, 3).contiguous() for x in (raw_rc, raw_cr)]
     # Angles folded per accumulator pass: k for St-SW (each subset once per pass), 4 for RC-FMAS.
     batch = k if method == _METHODS["st_sw"] else 4
-    per_voxel = nt * (batch * 2 * 8 + 4 * fields)
-    budget = min(_COMBINE_BYTES, t.cuda.mem_get_info()[0] // 2)
-    nxs = min(nx, max(1, budget // (per_voxel * nz * ny)))
-    if nxs < nx and nxs >= 8:
-        nxs -= nxs % 8
+    nxs = _slab_lines(g.shape, nt * (batch * 2 * 8 + 4 * fields))
     slab = nxs * nz * ny
     planes = dev.empty((batch, 2 * slab, nt), t.complex64)
     acc = dev.empty((slab * fields * nt,), t.float32)
@@ -265,11 +272,7 @@ def run_dmas_grid(raw_rc, raw_cr, g: _Geometry, shape4, pd):
     packed = [pack_channels(x, "float32") for x in (raw_rc, raw_cr)]
     by_angle = [x.permute(2, 1, 0, 3).contiguous() for x in (raw_rc, raw_cr)]
     batch = _DMAS_BATCH  # channels whose outputs are folded in one pass over the accumulator
-    per_voxel = nt * (batch * 2 * 8 + 4 * fields)
-    budget = min(_COMBINE_BYTES, t.cuda.mem_get_info()[0] // 2)
-    nxs = min(nx, max(1, budget // (per_voxel * nz * ny)))
-    if nxs < nx and nxs >= 8:
-        nxs -= nxs % 8
+    nxs = _slab_lines(g.shape, nt * (batch * 2 * 8 + 4 * fields))
     slab = nxs * nz * ny
     planes = dev.empty((batch, 2 * slab, nt), t.complex64)
     acc = dev.empty((slab * fields * nt,), t.float32)
