@@ -230,6 +230,15 @@ rcabeam_status rcabeam_simulate_channels_device(const double* static_pos, const 
                                                 float t_start, float c, float fs, float fd, float sigma_t,
                                                 float noise_std, unsigned long long seed);
 
+/* Complex mean over frames of the XDoppler signal of per-angle volumes:
+ * out[2v] + i out[2v + 1] = mean_t (sum_{rc span} iq) conj(sum_{cr span} iq);
+ * arguments as rcabeam_xdoppler_pd_device, whose output is its modulus
+ * (reference xdoppler.py:65-117 output="real" / "complex", computed there in
+ * NumPy). */
+rcabeam_status rcabeam_xdoppler_mean_device(const void* iq, float* out, size_t n_voxels, size_t n_angles,
+                                            size_t n_frames, size_t rc_start, size_t rc_count, size_t cr_start,
+                                            size_t cr_count);
+
 /* ---- Angle-by-angle combiners on a grid.  RC-FMAS (method 2) and St-SW
  * (method 4) need each angle's channel sum per aperture (reference
  * src/rcabeam/ensemble.cu:236-315 with METHOD 2, 459-590): run

@@ -14,6 +14,16 @@ from rcabeam_amd.ensemble import (
     filtered_xdoppler_pd,
     opw_voxel_order,
 )
+from rcabeam_amd.fused import (
+    dmas_ccf_acf_from_channels,
+    fast_pd_from_channels,
+    opw,
+    opw_pd_from_channels,
+    rc_fmas_pd,
+    rc_fmas_pd_from_channels,
+    xdoppler_pd,
+    xdoppler_pd_from_channels,
+)
 from rcabeam_amd.sim import RCAGeometry, delay_rca_channel_data, delay_rca_channels, depth_axis
 
 __all__ = [
@@ -24,7 +34,15 @@ __all__ = [
     "delay_rca_channel_data",
     "delay_rca_channels",
     "depth_axis",
+    "dmas_ccf_acf_from_channels",
     "ensemble_pd_from_channels",
+    "fast_pd_from_channels",
     "filtered_xdoppler_pd",
+    "opw",
+    "opw_pd_from_channels",
     "opw_voxel_order",
+    "rc_fmas_pd",
+    "rc_fmas_pd_from_channels",
+    "xdoppler_pd",
+    "xdoppler_pd_from_channels",
 ]

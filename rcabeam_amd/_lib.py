@@ -71,6 +71,7 @@ _SIGNATURES: dict[str, list] = {
     "rcabeam_eigh_device": [c_void_p, c_void_p, c_void_p, c_void_p, c_size_t],
     "rcabeam_filtered_power_device": [c_void_p, c_int, c_size_t, c_size_t, c_void_p, c_size_t, c_int, c_void_p],
     "rcabeam_temporal_filter_device": [c_void_p, c_int, c_size_t, c_size_t, c_void_p, c_size_t, c_int, c_void_p],
+    "rcabeam_xdoppler_mean_device": [c_void_p, c_void_p] + [c_size_t] * 7,
     "rcabeam_angle_accumulator_fields": [c_int, c_int],
     "rcabeam_angle_accumulate_device": [c_void_p] + [c_size_t] * 4 + [c_int] * 3 + [c_void_p],
     "rcabeam_angle_combine_finish_device": [c_void_p, c_size_t, c_size_t, c_size_t, c_int, c_int, c_void_p, c_void_p,

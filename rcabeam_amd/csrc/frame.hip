@@ -169,18 +169,23 @@ __global__ void k_span_pd(const float2* iq, float* out, long long nv, long long 
     float2 r = make_float2(0.f, 0.f), q = r;
     for (long long a = 0; a < rcn; ++a) {
       const float2 x = base[(rc0 + a) * nt + t];
-      r = cadd(r, fmas ? sgnsqrt(x) : x);
+      r = cadd(r, fmas == 1 ? sgnsqrt(x) : x);
     }
     for (long long a = 0; a < crn; ++a) {
       const float2 x = base[(cr0 + a) * nt + t];
-      q = cadd(q, fmas ? sgnsqrt(x) : x);
+      q = cadd(q, fmas == 1 ? sgnsqrt(x) : x);
     }
-    if (fmas)
+    if (fmas == 1)
       power += cnorm(cmul(r, q));
     else
       mean = cadd(mean, cmulc(r, q));
   }
   const float inv = 1.0f / static_cast<float>(nt);
+  if (fmas == 2) {  // complex mean of the XDoppler signal
+    out[2 * v] = mean.x * inv;
+    out[2 * v + 1] = mean.y * inv;
+    return;
+  }
   out[v] = fmas ? power * inv : hypotf(mean.x * inv, mean.y * inv);
 }
 
@@ -251,6 +256,12 @@ extern "C" rcabeam_status rcabeam_xdoppler_pd_device(const void* iq, float* out,
                                                      size_t n_frames, size_t rc_start, size_t rc_count,
                                                      size_t cr_start, size_t cr_count) {
   return span_pd(iq, out, n_voxels, n_angles, n_frames, rc_start, rc_count, cr_start, cr_count, 0);
+}
+
+extern "C" rcabeam_status rcabeam_xdoppler_mean_device(const void* iq, float* out, size_t n_voxels, size_t n_angles,
+                                                       size_t n_frames, size_t rc_start, size_t rc_count,
+                                                       size_t cr_start, size_t cr_count) {
+  return span_pd(iq, out, n_voxels, n_angles, n_frames, rc_start, rc_count, cr_start, cr_count, 2);
 }
 
 extern "C" rcabeam_status rcabeam_rc_fmas_pd_device(const void* iq, float* out, size_t n_voxels, size_t n_angles,

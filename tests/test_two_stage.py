@@ -260,3 +260,16 @@ def test_plan_struct_matches_c_header(tmp_path):
     for n in names:
         assert int(out[n]) == getattr(TwoStagePlan, n).offset, n
     assert int(out["sizeof"]) == ctypes.sizeof(TwoStagePlan)
+
+
+def test_per_frame_functions_refuse_a_cpu_path():
+    """This package has no CPU path: `use_cuda=False` is refused before any device work."""
+    import rcabeam_amd as rb
+
+    rc, cr, angles, starts, grid, x_el, y_el, fn = _case(1)
+    geom = rb.RCAGeometry(x_el, y_el, c=C, fs=FS, f_demod=FD, fnumber=fn)
+    for f in (rb.opw_pd_from_channels, rb.fast_pd_from_channels, rb.dmas_ccf_acf_from_channels):
+        with pytest.raises(ValueError, match="no CPU path"):
+            f(rc[..., 0], cr[..., 0], angles, starts, grid, geom, use_cuda=False)
+    with pytest.raises(ValueError, match="no CPU path"):
+        rb.opw(np.zeros((2, 3), np.complex64), use_cuda=False)
