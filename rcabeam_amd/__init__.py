@@ -6,6 +6,7 @@ point calls the HIP library through the C ABI (include/rcabeam.h,
 include/rcabeam_mi355x.h); there is no CPU fallback.
 """
 
+from rcabeam_amd.clutter import svd_clutter_filter, temporal_basis
 from rcabeam_amd.ensemble import (
     EnsembleTiming,
     beamform_apertures,
@@ -43,6 +44,8 @@ __all__ = [
     "opw_voxel_order",
     "rc_fmas_pd",
     "rc_fmas_pd_from_channels",
+    "svd_clutter_filter",
+    "temporal_basis",
     "xdoppler_pd",
     "xdoppler_pd_from_channels",
 ]
