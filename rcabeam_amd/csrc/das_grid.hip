@@ -20,6 +20,13 @@
 // tap (row, weight, phasor) of the lane's own voxel combined from its
 // receive and transmit terms exactly as combine() does.
 //
+// Single-angle launches (na == 1, MULTI): a channel has one window, so a
+// stage packs the windows of up to 4 consecutive live channels of one
+// aperture instead (a window longer than a stage is still copied in chunks).
+// The receive terms of a stage's channels (8 tile positions each) go to a
+// small LDS slot when the stage is planned, so reading the stage needs
+// neither registers held across it nor a second receive-term fill.
+//
 // APERTURES (method 5): RC channels come first, so when the first CR channel
 // is reached the sums are the RC IQ; they are stored to plane 0 of the
 // (voxel, aperture, frame) output and restarted, and the CR IQ goes to
@@ -44,15 +51,29 @@ constexpr int kMaxPairs = 13;          // nt <= 208
 // the other, and does not wait for those copies before the reads.
 constexpr int kVStageWords = 9216;
 
-// Dynamic LDS: tile tables at 0, then psum: float[64].
+constexpr int kStageChannels = 4;  // channel windows per stage of a single-angle launch
+
+// Dynamic LDS: tile tables at 0, then psum: float[64], then (single-angle
+// launches) two stage slots: receive terms Term[kStageChannels][8] and row
+// shifts int[kStageChannels] each.
+constexpr int kSlotTermBytes = kStageChannels * kGridPatch * 16;
+constexpr int kSlotBytes = kSlotTermBytes + kStageChannels * 4;
 __host__ __device__ inline int psum_offset(int nc) { return tile_table_bytes(nc); }
-__host__ __device__ inline int dynamic_bytes(int nc) { return psum_offset(nc) + kVTileSlots * 4; }
+__host__ __device__ inline int slot_offset(int nc) { return psum_offset(nc) + kVTileSlots * 4; }
+__host__ __device__ inline int dynamic_bytes(int nc, bool multi) { return slot_offset(nc) + (multi ? 2 * kSlotBytes : 0); }
 
 // Rows of nt complex64 samples a stage buffer holds, with room for the frames
 // the last pair block of the last row reads past nt.
 __host__ __device__ inline int stage_rows(int nt, int fpp) { return (kVStageWords / 2 - 16 * fpp) / nt; }
 
-template <int FPP, bool APERTURES>
+// A stage of a single-angle launch: n channel windows (or one chunk of a
+// long window, taps filtered to rows [t_lo, t_hi]) of aperture cfg.
+struct MultiStage {
+  int n, cfg, t_lo, t_hi;
+  bool valid;
+};
+
+template <int FPP, bool APERTURES, bool MULTI>
 __global__ __launch_bounds__(kVThreads) __attribute__((amdgpu_waves_per_eu(4))) void das_opw_grid_valu(GridArgs p) {
   extern __shared__ __align__(16) unsigned char lds[];
   __shared__ __align__(16) float2 stage_a[kVStageWords / 2];
@@ -78,17 +99,29 @@ __global__ __launch_bounds__(kVThreads) __attribute__((amdgpu_waves_per_eu(4))) 
   tile.build_tables();  // ends with a barrier
 
   const size_t plane_size = static_cast<size_t>(p.ns) * nt;
-  auto plane_of = [&](const Channel& ch) {
+  auto plane_of = [&](const Channel& ch) __attribute__((always_inline)) {
     const float2* data = static_cast<const float2*>(ch.cfg == 0 ? p.rc : p.cr);
     return data + static_cast<size_t>(ch.q - ch.cfg * p.nc) * p.na * plane_size;
   };
   // Receive positions of this lane's voxel: x for RC, y for CR.
-  auto setup = [&](int q) { return tile.setup(q, wave, vy, 0); };
+  auto setup = [&](int q) __attribute__((always_inline)) { return tile.setup(q, wave, vy, 0); };
 
+  // Rows [0, rows) of src (nt samples each) to dst, 1 KiB per wave
+  // instruction spread over the waves, asynchronous.
+  auto copy_rows = [&](const float2* src, float2* dst, int rows) __attribute__((always_inline)) {
+    const int samples = rows * nt;
+    const int chunks = (samples + 127) >> 7;  // 128 samples = 1 KiB
+    for (int i = wave; i < chunks; i += kVWaves) {
+      const int so = i * 128 + lane * 2;
+      if (so < samples)
+        __builtin_amdgcn_global_load_lds(reinterpret_cast<const void*>(src + so),
+                                         (__attribute__((address_space(3))) void*)(dst + i * 128), 16, 0, 0);
+    }
+  };
   // Copy a stage into buf: windows of angles [a0, a1) back to back (or one
   // chunk of a long window), contiguous in HBM and in LDS (rows of nt
   // samples), 1 KiB per wave instruction spread over the waves.
-  auto issue = [&](const Channel& ch, const Stage& s, float2* buf) {
+  auto issue = [&](const Channel& ch, const Stage& s, float2* buf) __attribute__((always_inline)) {
     const float2* plane0 = plane_of(ch);
     const int a0 = uniform(s.a0), a1 = uniform(s.a1), sr0 = uniform(s.r0), srows = uniform(s.rows);
     int off = 0;
@@ -103,17 +136,8 @@ __global__ __launch_bounds__(kVThreads) __attribute__((amdgpu_waves_per_eu(4))) 
         lo = b.x;
         rows = b.y - b.x + 1;
       }
-      const float2* src = plane0 + a * plane_size + static_cast<size_t>(lo) * nt;
-      float2* dst = buf + off * nt;
+      copy_rows(plane0 + a * plane_size + static_cast<size_t>(lo) * nt, buf + off * nt, rows);
       off += rows;
-      const int samples = rows * nt;
-      const int chunks = (samples + 127) >> 7;  // 128 samples = 1 KiB
-      for (int i = wave; i < chunks; i += kVWaves) {
-        const int so = i * 128 + lane * 2;
-        if (so < samples)
-          __builtin_amdgcn_global_load_lds(reinterpret_cast<const void*>(src + so),
-                                           (__attribute__((address_space(3))) void*)(dst + i * 128), 16, 0, 0);
-      }
     }
   };
 
@@ -121,13 +145,32 @@ __global__ __launch_bounds__(kVThreads) __attribute__((amdgpu_waves_per_eu(4))) 
 #pragma unroll
   for (int j = 0; j < 2 * FPP; ++j) acc[j] = make_float2(0.f, 0.f);
 
-  auto consume = [&](const Channel& ch, const Stage& cur, const float2* base) {
+  const uint32_t row_bytes = static_cast<uint32_t>(nt) * 8u;
+  // One tap over all frames from staged row r0 (0 <= r0 < 2^24 here, so a
+  // 24-bit multiply gives its byte offset); row r0 + 1 only where the weight
+  // is non-zero.
+  auto accumulate = [&](const Tap& tp, uint32_t r0, const float2* base) __attribute__((always_inline)) {
+    const unsigned char* row = reinterpret_cast<const unsigned char*>(base) + __umul24(r0, row_bytes) + 16 * fl;
+    const float4* s0 = reinterpret_cast<const float4*>(row);
+    const float4* s1 = reinterpret_cast<const float4*>(row + (tp.w != 0.f ? row_bytes : 0u));
+#pragma unroll
+    for (int j = 0; j < FPP; ++j) {
+      const float4 va = s0[8 * j], vb = s1[8 * j];  // frames 2 fl + 16 j, + 1
+      const float ax = fmaf(tp.w, vb.x - va.x, va.x), ay = fmaf(tp.w, vb.y - va.y, va.y);
+      const float bx = fmaf(tp.w, vb.z - va.z, va.z), by = fmaf(tp.w, vb.w - va.w, va.w);
+      acc[2 * j].x = fmaf(tp.c, ax, fmaf(-tp.s, ay, acc[2 * j].x));
+      acc[2 * j].y = fmaf(tp.c, ay, fmaf(tp.s, ax, acc[2 * j].y));
+      acc[2 * j + 1].x = fmaf(tp.c, bx, fmaf(-tp.s, by, acc[2 * j + 1].x));
+      acc[2 * j + 1].y = fmaf(tp.c, by, fmaf(tp.s, bx, acc[2 * j + 1].y));
+    }
+  };
+
+  auto consume = [&](const Channel& ch, const Stage& cur, const float2* base) __attribute__((always_inline)) {
     // Stage fields are wave-uniform; saying so keeps the angle loop scalar.
     const int a0 = uniform(cur.a0), a1 = uniform(cur.a1), cr0 = uniform(cur.r0);
     const int t_lo = cr0 >= 0 ? cr0 : -kBig;
     const int t_hi = cr0 >= 0 && !cur.last ? cr0 + max_rows - 2 : kBig;
     const int tpos = ch.cfg == 0 ? vy : wave;  // transmit position of this lane's voxel
-    const uint32_t row_bytes = static_cast<uint32_t>(nt) * 8u;
     int off = 0;
     for (int a = a0; a < a1; ++a) {
       int first, row_off;
@@ -144,30 +187,14 @@ __global__ __launch_bounds__(kVThreads) __attribute__((amdgpu_waves_per_eu(4))) 
       const Tap tp = tap_of(ch.rx, tile.transmit(ch.cfg, a, tpos), p.ns);
       // A lane-dependent branch around the body (not a `continue`) keeps the
       // angle loop itself wave-uniform.
-      if (tp.i0 >= 0 && tp.i0 >= t_lo && tp.i0 <= t_hi) {
-        // Row r0 of the staged window (0 <= r0 < 2^24 here, so a 24-bit multiply
-        // gives its byte offset); row r0 + 1 only where the weight is non-zero.
-        const uint32_t r0 = static_cast<uint32_t>(tp.i0 - first + row_off);
-        const unsigned char* row = reinterpret_cast<const unsigned char*>(base) + __umul24(r0, row_bytes) + 16 * fl;
-        const float4* s0 = reinterpret_cast<const float4*>(row);
-        const float4* s1 = reinterpret_cast<const float4*>(row + (tp.w != 0.f ? row_bytes : 0u));
-#pragma unroll
-        for (int j = 0; j < FPP; ++j) {
-          const float4 va = s0[8 * j], vb = s1[8 * j];  // frames 2 fl + 16 j, + 1
-          const float ax = fmaf(tp.w, vb.x - va.x, va.x), ay = fmaf(tp.w, vb.y - va.y, va.y);
-          const float bx = fmaf(tp.w, vb.z - va.z, va.z), by = fmaf(tp.w, vb.w - va.w, va.w);
-          acc[2 * j].x = fmaf(tp.c, ax, fmaf(-tp.s, ay, acc[2 * j].x));
-          acc[2 * j].y = fmaf(tp.c, ay, fmaf(tp.s, ax, acc[2 * j].y));
-          acc[2 * j + 1].x = fmaf(tp.c, bx, fmaf(-tp.s, by, acc[2 * j + 1].x));
-          acc[2 * j + 1].y = fmaf(tp.c, by, fmaf(tp.s, bx, acc[2 * j + 1].y));
-        }
-      }
+      if (tp.i0 >= 0 && tp.i0 >= t_lo && tp.i0 <= t_hi)
+        accumulate(tp, static_cast<uint32_t>(tp.i0 - first + row_off), base);
     }
   };
 
   // Apertures: store the sums as plane `cfg` of this lane's voxel and restart them.
   const long long v = tile.voxel(wave, vy);
-  auto store_plane = [&](int cfg) {
+  auto store_plane = [&](int cfg) __attribute__((always_inline)) {
 #pragma unroll
     for (int j = 0; j < FPP; ++j)
 #pragma unroll
@@ -179,50 +206,159 @@ __global__ __launch_bounds__(kVThreads) __attribute__((amdgpu_waves_per_eu(4))) 
   };
   bool rc_stored = false;
 
-  // ---- stages of all live channels, one barrier per stage.  Two stages per
-  // loop trip, so each consume reads a buffer known at compile time.
-  Stage cur{0, 0, -1, 0, true, false};
-  Channel ch;
-  ch.q = -1;
-  for (int q = tile.next_live(-1); q < nq; q = tile.next_live(q)) {
-    ch = setup(q);
-    cur = plan_stage(ch.w, 0, p.na, max_rows);
-    if (cur.valid) break;
-  }
-  if (cur.valid) issue(ch, cur, stage_a);
-  __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
-  __syncthreads();
-  // One stage: copy the next stage into `next` while the current one is read
-  // from `now`.
-  auto step = [&](const float2* now, float2* next) {
-    Stage nxt = next_stage(cur, ch.w, p.na, max_rows);
-    Channel nch = ch;
-    bool crossing = false;
-    if (!nxt.valid) {
-      for (int q = tile.next_live(ch.q); q < nq; q = tile.next_live(q)) {
-        nch = setup(q);
-        nxt = plan_stage(nch.w, 0, p.na, max_rows);
-        if (nxt.valid) break;
+  if constexpr (MULTI) {
+    // ---- single angle: stages of up to kStageChannels channel windows.
+    Term* slot_rx = reinterpret_cast<Term*>(lds + slot_offset(p.nc));  // [2][kStageChannels][8]
+    int* slot_shift = reinterpret_cast<int*>(lds + slot_offset(p.nc) + 2 * kSlotTermBytes);  // [2][kStageChannels]
+    // Next piece to stage: channel cq from row cr (cr < 0: its whole window).
+    int cq = tile.next_live(-1), cr = -1;
+    // Plan the stage at (cq, cr), issue its copies into buf and record its
+    // receive terms and row shifts in slot s (wave 0 writes; every wave plans
+    // the same stage from tile-wide windows).  Advances (cq, cr).
+    auto plan_issue = [&](float2* buf, int s) __attribute__((always_inline)) {
+      MultiStage st{0, 0, -kBig, kBig, false};
+      int total = 0;
+      auto record = [&](int k, int q, int lo) __attribute__((always_inline)) {
+        const int first = uniform(8 * (q - tile.rx_base));
+        if (wave == 0) {
+          if (lane >= first && lane < first + 8) slot_rx[(s * kStageChannels + k) * 8 + (lane - first)] = tile.rx_term;
+          if (lane == 0) slot_shift[s * kStageChannels + k] = total - lo;
+        }
+      };
+      const float2* const planes[2] = {static_cast<const float2*>(p.rc), static_cast<const float2*>(p.cr)};
+      auto plane = [&](int q) __attribute__((always_inline)) {
+        const int cfg = q >= p.nc ? 1 : 0;
+        return (cfg == 0 ? planes[0] : planes[1]) + static_cast<size_t>(q - cfg * p.nc) * plane_size;
+      };
+      if (cr >= 0) {  // next chunk of a window longer than a stage
+        const Channel ch = setup(cq);
+        const int hi = ch.w.at(0).y;
+        const bool last = cr + max_rows - 1 >= hi;
+        const int rows = last ? hi + 1 - cr : max_rows;
+        record(0, cq, cr);
+        copy_rows(plane(cq) + static_cast<size_t>(cr) * nt, buf, rows);
+        st = MultiStage{1, ch.cfg, cr, last ? kBig : cr + max_rows - 2, true};
+        if (last) {
+          cq = tile.next_live(cq);
+          cr = -1;
+        } else {
+          cr += max_rows - 1;
+        }
+        return st;
       }
-      crossing = nxt.valid;
-    }
-    if (nxt.valid) issue(crossing ? nch : ch, nxt, next);
-    if constexpr (APERTURES) {
-      if (!rc_stored && ch.cfg == 1) {  // first CR stage: the RC sums are complete
-        store_plane(0);
-        rc_stored = true;
+      // Receive terms of the stage's channels filled before its first copy:
+      // a fill's coordinate loads would otherwise wait for the copies too.
+      if (cq + kStageChannels > tile.rx_base + 8) tile.fill_receive(cq);
+      int q = cq;
+      for (; q < nq; q = tile.next_live(q)) {
+        const Channel ch = setup(q);
+        const int2 b = ch.w.at(0);
+        if (b.x > b.y) continue;
+        const int rows = b.y - b.x + 1;
+        if (st.n > 0 && (ch.cfg != st.cfg || rows > max_rows || total + rows > max_rows)) break;
+        if (rows > max_rows) {  // first chunk of a long window: a stage of its own
+          record(0, q, b.x);
+          copy_rows(plane(q) + static_cast<size_t>(b.x) * nt, buf, max_rows);
+          cq = q;
+          cr = b.x + max_rows - 1;
+          return MultiStage{1, ch.cfg, b.x, b.x + max_rows - 2, true};
+        }
+        record(st.n, q, b.x);
+        copy_rows(plane(q) + static_cast<size_t>(b.x) * nt, buf + total * nt, rows);
+        total += rows;
+        st.n += 1;
+        st.cfg = ch.cfg;
+        st.valid = true;
+        if (st.n == kStageChannels) {
+          q = tile.next_live(q);
+          break;
+        }
       }
-    }
-    consume(ch, cur, now);
+      cq = q;
+      cr = -1;
+      return st;
+    };
+    auto consume_multi = [&](const MultiStage& st, const float2* base, int s) __attribute__((always_inline)) {
+      const int n = uniform(st.n), cfg = uniform(st.cfg), t_lo = uniform(st.t_lo), t_hi = uniform(st.t_hi);
+      const int tpos = cfg == 0 ? vy : wave;  // transmit position of this lane's voxel
+      const int rpos = cfg == 0 ? wave : vy;  // receive position
+      const Term tx = tile.transmit(cfg, 0, tpos);
+      for (int k = 0; k < n; ++k) {
+        const Term rx = slot_rx[(s * kStageChannels + k) * 8 + rpos];
+        const int shift = uniform(slot_shift[s * kStageChannels + k]);
+        const Tap tp = tap_of(rx, tx, p.ns);
+        if (tp.i0 >= 0 && tp.i0 >= t_lo && tp.i0 <= t_hi) accumulate(tp, static_cast<uint32_t>(tp.i0 + shift), base);
+      }
+    };
+    MultiStage cur{0, 0, -kBig, kBig, false};
+    if (cq < nq) cur = plan_issue(stage_a, 0);
     __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
     __syncthreads();
-    if (crossing) ch = nch;
-    cur = nxt;
-  };
-  while (cur.valid) {
-    step(stage_a, stage_b);
-    if (!cur.valid) break;
-    step(stage_b, stage_a);
+    auto step = [&](const float2* now, float2* next, int s_now, int s_next) __attribute__((always_inline)) {
+      MultiStage nxt{0, 0, -kBig, kBig, false};
+      if (cq < nq) nxt = plan_issue(next, s_next);
+      if constexpr (APERTURES) {
+        if (!rc_stored && cur.cfg == 1) {  // first CR stage: the RC sums are complete
+          store_plane(0);
+          rc_stored = true;
+        }
+      }
+      consume_multi(cur, now, s_now);
+      __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+      __syncthreads();
+      cur = nxt;
+    };
+    while (cur.valid) {
+      step(stage_a, stage_b, 0, 1);
+      if (!cur.valid) break;
+      step(stage_b, stage_a, 1, 0);
+    }
+  } else {
+    // ---- stages of all live channels, one barrier per stage.  Two stages per
+    // loop trip, so each consume reads a buffer known at compile time.
+    Stage cur{0, 0, -1, 0, true, false};
+    Channel ch;
+    ch.q = -1;
+    for (int q = tile.next_live(-1); q < nq; q = tile.next_live(q)) {
+      ch = setup(q);
+      cur = plan_stage(ch.w, 0, p.na, max_rows);
+      if (cur.valid) break;
+    }
+    if (cur.valid) issue(ch, cur, stage_a);
+    __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+    __syncthreads();
+    // One stage: copy the next stage into `next` while the current one is read
+    // from `now`.
+    auto step = [&](const float2* now, float2* next) __attribute__((always_inline)) {
+      Stage nxt = next_stage(cur, ch.w, p.na, max_rows);
+      Channel nch = ch;
+      bool crossing = false;
+      if (!nxt.valid) {
+        for (int q = tile.next_live(ch.q); q < nq; q = tile.next_live(q)) {
+          nch = setup(q);
+          nxt = plan_stage(nch.w, 0, p.na, max_rows);
+          if (nxt.valid) break;
+        }
+        crossing = nxt.valid;
+      }
+      if (nxt.valid) issue(crossing ? nch : ch, nxt, next);
+      if constexpr (APERTURES) {
+        if (!rc_stored && ch.cfg == 1) {  // first CR stage: the RC sums are complete
+          store_plane(0);
+          rc_stored = true;
+        }
+      }
+      consume(ch, cur, now);
+      __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+      __syncthreads();
+      if (crossing) ch = nch;
+      cur = nxt;
+    };
+    while (cur.valid) {
+      step(stage_a, stage_b);
+      if (!cur.valid) break;
+      step(stage_b, stage_a);
+    }
   }
 
   // ---- epilogue: IQ and power (apertures: the last plane, power 0)
@@ -276,7 +412,8 @@ template <int FPP, bool APERTURES>
 hipError_t launch_pairs(GridArgs a) {
   a.max_rows = std::min(kVMaxRows, stage_rows(a.nt, FPP));
   if (a.max_rows < kVMinRows) return hipErrorNotSupported;
-  const int bytes = dynamic_bytes(a.nc);
+  const bool multi = a.na == 1;
+  const int bytes = dynamic_bytes(a.nc, multi);
   if (bytes + 2 * kVStageWords * 4 > kVLdsBytes) return hipErrorNotSupported;
   const long long grid = (a.tiles + 7) / 8 * 8;
   if (grid > 0x7fffffffLL) return hipErrorInvalidValue;
@@ -284,7 +421,7 @@ hipError_t launch_pairs(GridArgs a) {
   if (a.nt % 2 != 0 || reinterpret_cast<uintptr_t>(a.rc) % 16 || reinterpret_cast<uintptr_t>(a.cr) % 16)
     return hipErrorNotSupported;
   a.vec = 1;
-  auto kernel = das_opw_grid_valu<FPP, APERTURES>;
+  auto kernel = multi ? das_opw_grid_valu<FPP, APERTURES, true> : das_opw_grid_valu<FPP, APERTURES, false>;
   kernel<<<dim3(static_cast<unsigned>(grid)), kVThreads, bytes>>>(a);
   return hipGetLastError();
 }

@@ -238,6 +238,29 @@ def test_scattered_voxel_order_stages_windows_in_chunks(fnumber):
     assert rel_l2(power, combiners.power_doppler(expected)) < TOL
 
 
+@pytest.mark.parametrize("frames", [16, 200])
+@pytest.mark.parametrize("fnumber", [None, 1.0])
+def test_single_angle_stages_pack_channel_windows(frames, fnumber):
+    """One angle per launch: a stage packs the windows of up to 4 channels.  Coarse 8 x 8 tiles
+    (0.85 mm voxels) give windows longer than a stage at fnumber None (staged in chunks), runs of
+    live channels cross receive-term fills and the RC -> CR boundary, and 200 frames take the
+    smallest stage (22 rows)."""
+    acq = scene.point_scene(grid_shape=(16, 8, 16), frames=frames)
+    one = scene.Acquisition(acq.rc[:, :, 2:3], acq.cr[:, :, 2:3], acq.x_el, acq.y_el, acq.angles[2:3], acq.t_start,
+                            acq.grid, acq.c, acq.fs, acq.fd, fnumber)
+    print(f"case: single angle frames={frames} fnumber={fnumber}", flush=True)
+    (ra, _, _), (qa, _, _) = _oracle(one)
+    args = (one.rc, one.cr, one.angles, one.t_start, one.grid, _geom(one))
+    r, q = rb.beamform_apertures(*args)
+    assert rel_l2(r.reshape(-1, frames), ra[:, 0]) < TOL
+    assert rel_l2(q.reshape(-1, frames), qa[:, 0]) < TOL
+    expected = ra[:, 0] + qa[:, 0]
+    iq = rb.beamform_ensemble(*args)
+    assert rel_l2(iq.reshape(-1, frames), expected) < TOL
+    pd = rb.ensemble_pd_from_channels(*args)
+    assert rel_l2(pd.ravel(), combiners.power_doppler(expected)) < TOL
+
+
 @pytest.mark.parametrize("storage", ["float16", "bfloat16"])
 def test_sixteen_bit_coefficients_error(storage):
     """RCABEAM_COEF_16BIT: one 16-bit coefficient per sample.  Against the float64 oracle of the
