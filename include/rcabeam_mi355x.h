@@ -292,6 +292,42 @@ rcabeam_status rcabeam_dmas_finish_device(const void* acc, const float* x, const
                                           size_t nz, size_t ny, const float* xe, const float* ye, size_t nc,
                                           size_t na, size_t nt, float fn, float* pd);
 
+/* ---- Hybrid DMAS-CCF-ACF approximation (reference
+ * examples/_support/dmas_delay.cu:21-151, driven by
+ * examples/_support/dmas_delay.py:61-154).  Channels are packed complex64
+ * (channels, angles, samples, frames); tables are float32 quadruples.
+ *
+ * Angle planes (dmas_delay.cu:96-114): planes (nv, nk, na, nt) complex64,
+ * entry (v, k, a) = sum over channels with prx.w != 0 of the sample at
+ * prx.x + ptx[k, a] rotated by (prx.y, prx.z); prx (nv, nc, nk, 4), ptx
+ * (nk, na).  Follow with rcabeam_two_stage_spline_device (lines nv, npz nk,
+ * chunk = count = nt) for the second derivatives. */
+rcabeam_status rcabeam_hybrid_planes_device(const void* packed, const float* prx, const float* ptx, void* planes,
+                                            size_t nv, size_t nc, size_t na, size_t ns, size_t nt, size_t nk);
+
+/* Channel tables of depth slab `slab` (dmas_delay.cu:21-38): tables
+ * (nu, nc, nq, nt) complex64, entry (u, ch, k) = sum over angles of the sample
+ * at anchor.x + k step rotated by (anchor.y, anchor.z), anchor =
+ * anchors[slab, u, a] of anchors (n_slabs, nu, na, 4); slab < n_slabs is the
+ * caller's to keep. */
+rcabeam_status rcabeam_hybrid_tables_device(const void* packed, const float* anchors, void* tables, size_t ns,
+                                            size_t nc, size_t na, size_t nt, size_t nu, size_t slab, size_t nq,
+                                            float step);
+
+/* Power of depths [first, first + depth) (dmas_delay.cu:116-151): for each
+ * voxel and frame, channels with rx.x >= 0 (rx (nz, nv, nc, 4); RC receives
+ * along x, CR along y) read their table row (u, ch) at rx.x * inverse_step
+ * (cubic != 0: Catmull-Rom inside the table), rotated by (rx.y, rx.z), into
+ * the signed-root DMAS and CCF sums; each angle's spline value of the planes
+ * at tx.x (tx (nz, nu, na, 4)), rotated by (tx.y, tx.z), into the ACF sums;
+ * power[(ix nz + iz) ny + iy] = mean over frames of DMAS x CCF x ACF. */
+rcabeam_status rcabeam_hybrid_finish_device(const void* tables_rc, const void* tables_cr, const void* planes_rc,
+                                            const void* planes_cr, const void* second_rc, const void* second_cr,
+                                            const float* rx_rc, const float* tx_rc, const float* rx_cr,
+                                            const float* tx_cr, float* power, size_t nx, size_t nz, size_t ny,
+                                            size_t nc, size_t na, size_t nt, size_t first, size_t depth, size_t nq,
+                                            size_t nk, float inverse_step, int cubic);
+
 /* ---- Two-stage OPW approximation (reference src/rcabeam/two_stage.py,
  * src/rcabeam/two_stage.cu).  Each aperture is beamformed onto reference
  * planes (receive line x reference depth, one plane per elevation slab), a

@@ -87,6 +87,9 @@ _SIGNATURES: dict[str, list] = {
     "rcabeam_two_stage_stsw_finish_device": [c_void_p] * 3 + [c_size_t] * 2 + [c_void_p],
     "rcabeam_two_stage_buffer_elements": [c_void_p],
     "rcabeam_two_stage_reconstruct_device": [c_void_p] * 5,
+    "rcabeam_hybrid_planes_device": [c_void_p] * 4 + [c_size_t] * 6,
+    "rcabeam_hybrid_tables_device": [c_void_p] * 3 + [c_size_t] * 7 + [c_float],
+    "rcabeam_hybrid_finish_device": [c_void_p] * 11 + [c_size_t] * 10 + [c_float, c_int],
 }
 _RESTYPE = {
     "rcabeam_status_string": c_char_p,

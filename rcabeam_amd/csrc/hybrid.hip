@@ -1,0 +1,306 @@
+// Hybrid DMAS-CCF-ACF approximation (reference examples/_support/dmas_delay.cu:21-151,
+// driven by examples/_support/dmas_delay.py:61-154): delay tables for the two
+// marginals DMAS-CCF-ACF needs, instead of per-channel and per-angle
+// delay-and-sum.
+//
+// * Channel tables (per depth slab): C[(u, ch, k), t] = sum over angles of
+//   the sample at the slab's steering anchor + k * step, rotated by the
+//   anchor's carrier phase.  One wave per (u, ch, k), lanes over frames.
+// * Angle planes: A[(v, k, a), t] = sum over aperture channels of the sample
+//   at the receive term r(v, ch, k) + transmit term (k, a), rotated by the
+//   baseband excess phase; a not-a-knot spline along k follows (the two-stage
+//   spline entry point).  One wave per (v, k, a), lanes over frames.
+// * Finish: one wave per voxel, lanes over frames.  Each active channel's
+//   excess receive delay indexes its C row (linear, or Catmull-Rom inside
+//   the table), the rotated values give the signed-root DMAS sums and the CCF
+//   magnitudes; each angle's spline lookup of A, rotated by the residual
+//   carrier, gives the ACF sums; DMAS x CCF x ACF is averaged over frames.
+//   Voxels run depth-fastest within a slab so consecutive waves read the same
+//   table rows at nearby knots, and each XCD takes a contiguous run of them.
+//
+// Tables are float32 (x, y, z, w) quadruples as the caller forms them; all
+// reads of packed channels (channels, angles, samples, frames) are rows of
+// contiguous frames, so every wave load is one coalesced 512-byte access.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "common.hpp"
+#include "rcabeam_mi355x.h"
+
+namespace rcb {
+namespace {
+
+constexpr int kHybridWaves = 4;  // waves per 256-thread workgroup
+constexpr int kXcdCount = 8;
+
+__device__ __forceinline__ float2 rotate(float2 x, float c, float s) {
+  return make_float2(x.x * c - x.y * s, x.x * s + x.y * c);
+}
+__device__ __forceinline__ float2 lerp2(float2 a, float2 b, float w) {
+  return make_float2(fmaf(w, b.x - a.x, a.x), fmaf(w, b.y - a.y, a.y));
+}
+__device__ __forceinline__ float2 add2(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 scale2(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+
+// Linear interpolation of a (ns, nt) trace at a wave-uniform position; zero
+// outside the closed window [0, ns - 1].
+__device__ __forceinline__ float2 trace_sample(const float2* trace, float pos, int ns, int nt, int t) {
+  if (!(pos >= 0.f) || pos > static_cast<float>(ns - 1)) return make_float2(0.f, 0.f);
+  const int i = static_cast<int>(pos);
+  return lerp2(trace[static_cast<size_t>(i) * nt + t], trace[static_cast<size_t>(min(i + 1, ns - 1)) * nt + t],
+               pos - static_cast<float>(i));
+}
+
+// Block b's logical index: each XCD (blocks b % 8) takes a contiguous run.
+__device__ __forceinline__ long long xcd_block(long long blocks) {
+  const long long per = (blocks + kXcdCount - 1) / kXcdCount;
+  return static_cast<long long>(blockIdx.x % kXcdCount) * per + blockIdx.x / kXcdCount;
+}
+
+__global__ __launch_bounds__(256) void k_hybrid_planes(const float2* __restrict__ raw, const float4* __restrict__ prx,
+                                                       const float* __restrict__ ptx, float2* __restrict__ out,
+                                                       long long entries, int nc, int na, int ns, int nt, int nk) {
+  const long long entry = static_cast<long long>(blockIdx.x) * kHybridWaves + (threadIdx.x >> 6);
+  if (entry >= entries) return;
+  const int lane = threadIdx.x & 63;
+  const int a = static_cast<int>(entry % na);
+  const int k = static_cast<int>((entry / na) % nk);
+  const long long v = entry / na / nk;
+  const float tx = ptx[k * na + a];
+  for (int t0 = 0; t0 < nt; t0 += 64) {
+    const int t = t0 + lane;
+    const int tt = t < nt ? t : nt - 1;
+    float2 sum = make_float2(0.f, 0.f);
+    for (int ch = 0; ch < nc; ++ch) {
+      const float4 r = prx[(v * nc + ch) * nk + k];
+      if (r.w == 0.f) continue;
+      const float2* trace = raw + (static_cast<size_t>(ch) * na + a) * ns * static_cast<size_t>(nt);
+      sum = add2(sum, rotate(trace_sample(trace, r.x + tx, ns, nt, tt), r.y, r.z));
+    }
+    if (t < nt) out[entry * nt + t] = sum;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_hybrid_tables(const float2* __restrict__ raw,
+                                                       const float4* __restrict__ anchors, float2* __restrict__ out,
+                                                       long long entries, int ns, int nc, int na, int nt, int nu,
+                                                       int slab, int nq, float step) {
+  const long long entry = static_cast<long long>(blockIdx.x) * kHybridWaves + (threadIdx.x >> 6);
+  if (entry >= entries) return;
+  const int lane = threadIdx.x & 63;
+  const int k = static_cast<int>(entry % nq);
+  const int ch = static_cast<int>((entry / nq) % nc);
+  const int u = static_cast<int>(entry / nq / nc);
+  const float4* b = anchors + (static_cast<size_t>(slab) * nu + u) * na;
+  const float shift = static_cast<float>(k) * step;
+  for (int t0 = 0; t0 < nt; t0 += 64) {
+    const int t = t0 + lane;
+    const int tt = t < nt ? t : nt - 1;
+    float2 sum = make_float2(0.f, 0.f);
+    for (int a = 0; a < na; ++a) {
+      const float4 g = b[a];
+      const float2* trace = raw + (static_cast<size_t>(ch) * na + a) * ns * static_cast<size_t>(nt);
+      sum = add2(sum, rotate(trace_sample(trace, g.x + shift, ns, nt, tt), g.y, g.z));
+    }
+    if (t < nt) out[entry * nt + t] = sum;
+  }
+}
+
+// Table row `row` of nq knots at a non-negative position: linear between the
+// two knots around it (the last interval extended), or Catmull-Rom where the
+// four knots around it exist.
+__device__ __forceinline__ float2 table_value(const float2* table, size_t row, int nq, int nt, int t, float position,
+                                              bool cubic) {
+  const int ip = static_cast<int>(position);
+  if (cubic && ip != 0 && ip + 2 < nq) {
+    const float2* p = table + (row * nq + ip) * nt + t;
+    const float w = position - static_cast<float>(ip), v = 1.f - w;
+    const float2 p0 = p[0], p1 = p[nt], pm = p[-nt], p2 = p[2 * nt];
+    const float2 value = lerp2(p0, p1, w);
+    const float2 left = add2(add2(pm, p1), scale2(p0, -2.f));
+    const float2 right = add2(add2(p0, p2), scale2(p1, -2.f));
+    return add2(value, scale2(lerp2(left, right, w), -0.5f * w * v));
+  }
+  const int i = min(ip, nq - 2);
+  const float2* p = table + (row * nq + i) * nt + t;
+  return lerp2(p[0], p[nt], position - static_cast<float>(i));
+}
+
+// Not-a-knot spline value of plane line `line`, angle a, at knot position pos
+// (zero outside the knots): the two-stage planes layout (line, knot, angle, frame).
+__device__ __forceinline__ float2 spline_value(const float2* plane, const float2* second, size_t line, int a, int t,
+                                               int na, int nk, int nt, float pos) {
+  const int i = static_cast<int>(floorf(pos));
+  if (i < 0 || i >= nk - 1) return make_float2(0.f, 0.f);
+  const float w = pos - static_cast<float>(i), u = 1.f - w;
+  const size_t offset = ((line * nk + i) * na + a) * static_cast<size_t>(nt) + t;
+  const size_t stride = static_cast<size_t>(na) * nt;
+  const float2 value = add2(scale2(plane[offset], u), scale2(plane[offset + stride], w));
+  return add2(value, add2(scale2(second[offset], (u * u * u - u) / 6.f),
+                          scale2(second[offset + stride], (w * w * w - w) / 6.f)));
+}
+
+struct FinishArgs {
+  const float2* tables[2];
+  const float2* planes[2];
+  const float2* second[2];
+  const float4* rx[2];
+  const float4* tx[2];
+  float* power;
+  int nx, nz, ny, nc, na, nt, first, depth, nq, nk;
+  float inverse_step;
+  int cubic;
+};
+
+__global__ __launch_bounds__(256) void k_hybrid_finish(FinishArgs p) {
+  const long long blocks = gridDim.x;
+  const long long voxel = xcd_block(blocks) * kHybridWaves + (threadIdx.x >> 6);
+  const long long count = static_cast<long long>(p.depth) * p.nx * p.ny;
+  if (voxel >= count) return;
+  const int lane = threadIdx.x & 63;
+  // Depth fastest within the slab, then y, then x.
+  const int dz = static_cast<int>(voxel % p.depth);
+  const int iy = static_cast<int>((voxel / p.depth) % p.ny);
+  const int ix = static_cast<int>(voxel / p.depth / p.ny);
+  const int z = p.first + dz;
+  const float inv_na = 1.f / static_cast<float>(p.na);
+  float total = 0.f;
+  for (int t0 = 0; t0 < p.nt; t0 += 64) {
+    const int t = t0 + lane;
+    const int tt = t < p.nt ? t : p.nt - 1;
+    float2 roots = make_float2(0.f, 0.f), coherent[2];
+    float magnitude = 0.f, energy[2];
+    int active = 0;
+#pragma unroll
+    for (int conf = 0; conf < 2; ++conf) {
+      const int v = conf ? iy : ix, u = conf ? ix : iy;
+      const int nv = conf ? p.ny : p.nx, nu = conf ? p.nx : p.ny;
+      const float4* rx = p.rx[conf] + (static_cast<size_t>(z) * nv + v) * p.nc;
+      const float4* tx = p.tx[conf] + (static_cast<size_t>(z) * nu + u) * p.na;
+      for (int ch = 0; ch < p.nc; ++ch) {
+        const float4 r = rx[ch];
+        if (r.x < 0.f) continue;
+        const float2 val = rotate(table_value(p.tables[conf], static_cast<size_t>(u) * p.nc + ch, p.nq, p.nt, tt,
+                                              r.x * p.inverse_step, p.cubic != 0),
+                                  r.y, r.z);
+        const float m = sqrtf(val.x * val.x + val.y * val.y);
+        const float w = m > 0.f ? 1.f / sqrtf(m) : 0.f;
+        roots = add2(roots, scale2(val, w));
+        magnitude += m;
+        ++active;
+      }
+      float2 sum = make_float2(0.f, 0.f);
+      float e = 0.f;
+      for (int a = 0; a < p.na; ++a) {
+        const float4 b = tx[a];
+        const float2 val = rotate(spline_value(p.planes[conf], p.second[conf], v, a, tt, p.na, p.nk, p.nt, b.x),
+                                  b.y, b.z);
+        sum = add2(sum, val);
+        e += val.x * val.x + val.y * val.y;
+      }
+      coherent[conf] = sum;
+      energy[conf] = e;
+    }
+    const float dm = roots.x * roots.x + roots.y * roots.y - magnitude;
+    float power = 0.f;
+    if (dm > 0.f && magnitude > 0.f && active > 0) {
+      const float r = (coherent[0].x * coherent[0].x + coherent[0].y * coherent[0].y) * inv_na * inv_na;
+      const float q = (coherent[1].x * coherent[1].x + coherent[1].y * coherent[1].y) * inv_na * inv_na;
+      const float var = fmaxf(0.f, energy[0] * inv_na - r) + fmaxf(0.f, energy[1] * inv_na - q);
+      const float num = sqrtf(r) * sqrtf(q), den = r + q + 0.5f * var - num;
+      if (den > 0.f) power = dm * (dm / static_cast<float>(active) / magnitude) * (num / den);
+    }
+    if (t < p.nt) total += power;
+  }
+  total = wave_sum(total);
+  if (lane == 0) p.power[(static_cast<size_t>(ix) * p.nz + z) * p.ny + iy] = total / static_cast<float>(p.nt);
+}
+
+bool fits(size_t n) { return n <= 0x7fffffffu; }
+
+}  // namespace
+}  // namespace rcb
+
+using namespace rcb;
+
+extern "C" rcabeam_status rcabeam_hybrid_planes_device(const void* packed, const float* prx, const float* ptx,
+                                                       void* planes, size_t nv, size_t nc, size_t na, size_t ns,
+                                                       size_t nt, size_t nk) {
+  if (!packed || !prx || !ptx || !planes || ns < 2 || nc < 1 || na < 1 || nt < 1 || nk < 2 || !fits(nc) ||
+      !fits(na) || !fits(ns) || !fits(nt) || !fits(nk) || !fits(nv))
+    return RCABEAM_ERROR_ARGUMENT;
+  const long long entries = static_cast<long long>(nv) * nk * na;
+  if (entries == 0) return RCABEAM_SUCCESS;
+  const long long blocks = (entries + kHybridWaves - 1) / kHybridWaves;
+  if (blocks > 0x7fffffffLL) return RCABEAM_ERROR_ARGUMENT;
+  k_hybrid_planes<<<static_cast<unsigned>(blocks), 256>>>(
+      static_cast<const float2*>(packed), reinterpret_cast<const float4*>(prx), ptx, static_cast<float2*>(planes),
+      entries, static_cast<int>(nc), static_cast<int>(na), static_cast<int>(ns), static_cast<int>(nt),
+      static_cast<int>(nk));
+  return launch_status();
+}
+
+extern "C" rcabeam_status rcabeam_hybrid_tables_device(const void* packed, const float* anchors, void* tables,
+                                                       size_t ns, size_t nc, size_t na, size_t nt, size_t nu,
+                                                       size_t slab, size_t nq, float step) {
+  if (!packed || !anchors || !tables || ns < 2 || nc < 1 || na < 1 || nt < 1 || nq < 2 || !fits(ns) || !fits(nc) ||
+      !fits(na) || !fits(nt) || !fits(nu) || !fits(slab) || !fits(nq) || !std::isfinite(step) || step <= 0.f)
+    return RCABEAM_ERROR_ARGUMENT;
+  const long long entries = static_cast<long long>(nu) * nc * nq;
+  if (entries == 0) return RCABEAM_SUCCESS;
+  const long long blocks = (entries + kHybridWaves - 1) / kHybridWaves;
+  if (blocks > 0x7fffffffLL) return RCABEAM_ERROR_ARGUMENT;
+  k_hybrid_tables<<<static_cast<unsigned>(blocks), 256>>>(
+      static_cast<const float2*>(packed), reinterpret_cast<const float4*>(anchors), static_cast<float2*>(tables),
+      entries, static_cast<int>(ns), static_cast<int>(nc), static_cast<int>(na), static_cast<int>(nt),
+      static_cast<int>(nu), static_cast<int>(slab), static_cast<int>(nq), step);
+  return launch_status();
+}
+
+extern "C" rcabeam_status rcabeam_hybrid_finish_device(const void* tables_rc, const void* tables_cr,
+                                                       const void* planes_rc, const void* planes_cr,
+                                                       const void* second_rc, const void* second_cr,
+                                                       const float* rx_rc, const float* tx_rc, const float* rx_cr,
+                                                       const float* tx_cr, float* power, size_t nx, size_t nz,
+                                                       size_t ny, size_t nc, size_t na, size_t nt, size_t first,
+                                                       size_t depth, size_t nq, size_t nk, float inverse_step,
+                                                       int cubic) {
+  if (!tables_rc || !tables_cr || !planes_rc || !planes_cr || !second_rc || !second_cr || !rx_rc || !tx_rc ||
+      !rx_cr || !tx_cr || !power || nc < 1 || na < 1 || nt < 1 || nq < 2 || nk < 2 || first + depth > nz ||
+      !fits(nx) || !fits(nz) || !fits(ny) || !fits(nc) || !fits(na) || !fits(nt) || !fits(nq) || !fits(nk) ||
+      !std::isfinite(inverse_step) || inverse_step <= 0.f)
+    return RCABEAM_ERROR_ARGUMENT;
+  const long long count = static_cast<long long>(depth) * nx * ny;
+  if (count == 0) return RCABEAM_SUCCESS;
+  long long blocks = (count + kHybridWaves - 1) / kHybridWaves;
+  blocks = (blocks + kXcdCount - 1) / kXcdCount * kXcdCount;
+  if (blocks > 0x7fffffffLL) return RCABEAM_ERROR_ARGUMENT;
+  FinishArgs p;
+  p.tables[0] = static_cast<const float2*>(tables_rc);
+  p.tables[1] = static_cast<const float2*>(tables_cr);
+  p.planes[0] = static_cast<const float2*>(planes_rc);
+  p.planes[1] = static_cast<const float2*>(planes_cr);
+  p.second[0] = static_cast<const float2*>(second_rc);
+  p.second[1] = static_cast<const float2*>(second_cr);
+  p.rx[0] = reinterpret_cast<const float4*>(rx_rc);
+  p.rx[1] = reinterpret_cast<const float4*>(rx_cr);
+  p.tx[0] = reinterpret_cast<const float4*>(tx_rc);
+  p.tx[1] = reinterpret_cast<const float4*>(tx_cr);
+  p.power = power;
+  p.nx = static_cast<int>(nx);
+  p.nz = static_cast<int>(nz);
+  p.ny = static_cast<int>(ny);
+  p.nc = static_cast<int>(nc);
+  p.na = static_cast<int>(na);
+  p.nt = static_cast<int>(nt);
+  p.first = static_cast<int>(first);
+  p.depth = static_cast<int>(depth);
+  p.nq = static_cast<int>(nq);
+  p.nk = static_cast<int>(nk);
+  p.inverse_step = inverse_step;
+  p.cubic = cubic;
+  k_hybrid_finish<<<static_cast<unsigned>(blocks), 256>>>(p);
+  return launch_status();
+}
