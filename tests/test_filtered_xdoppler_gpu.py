@@ -42,3 +42,29 @@ def test_filtered_xdoppler_matches_oracle(rank_start, rank_stop):
         plain = rb.ensemble_pd_from_channels(acq.rc, acq.cr, acq.angles, acq.t_start, acq.grid, geom,
                                              method="xdoppler")
         assert rel_l2(got, plain) < 1e-5
+
+
+@pytest.mark.parametrize("rank_start,rank_stop", [(2, None), (1, 10)])
+def test_channel_domain_filter_then_beamform(rank_start, rank_stop):
+    """Channel-domain clutter filtering (reference examples/_support/clutter_pipeline.py:87-90): the
+    (RC, CR) channel tuple `(samples, channels, angles, frames)` shares one temporal basis over all
+    channel rows of both apertures, the interval is kept, and the filtered channels are beamformed.
+    Oracle: float64 SVD of the stacked channel rows, then its delay-and-sum and power Doppler.
+    Bars: filtered channels rel L2 1e-6 (complex64 output of a complex128 projector), power 1e-5."""
+    from oracle import combiners
+
+    print(f"case: channel-domain filter rank [{rank_start}, {rank_stop})", flush=True)
+    acq = scene.point_scene(grid_shape=(8, 12, 8), frames=20)
+    geom = rb.RCAGeometry(acq.x_el, acq.y_el, c=acq.c, fs=acq.fs, f_demod=acq.fd, fnumber=acq.fnumber)
+    rc_f, cr_f = rb.svd_clutter_filter((acq.rc, acq.cr), rank_start=rank_start, rank_stop=rank_stop)
+    assert rc_f.shape == acq.rc.shape and cr_f.dtype == np.complex64
+    nt = acq.rc.shape[-1]
+    rows = np.concatenate([acq.rc.reshape(-1, nt), acq.cr.reshape(-1, nt)])
+    kept = clutter.svd_clutter_filter(rows, rank_start=rank_start, rank_stop=rank_stop)
+    exp_rc, exp_cr = kept[: len(rows) // 2].reshape(acq.rc.shape), kept[len(rows) // 2 :].reshape(acq.cr.shape)
+    assert rel_l2(rc_f.ravel(), exp_rc.ravel()) < 1e-6
+    assert rel_l2(cr_f.ravel(), exp_cr.ravel()) < 1e-6
+    pd = rb.ensemble_pd_from_channels(rc_f, cr_f, acq.angles, acq.t_start, acq.grid, geom)
+    (ra, _, _), (qa, _, _) = das.delay_apertures(exp_rc, exp_cr, das.scan_coords(acq.grid), acq.x_el, acq.y_el,
+                                                 acq.angles, acq.t_start, acq.c, acq.fs, acq.fd, acq.fnumber)
+    assert rel_l2(pd.ravel(), combiners.power_doppler(ra.sum(axis=1) + qa.sum(axis=1))) < 1e-5
