@@ -10,7 +10,8 @@
 //   at the receive term r(v, ch, k) + transmit term (k, a), rotated by the
 //   baseband excess phase; a not-a-knot spline along k follows (the two-stage
 //   spline entry point).  One wave per (v, k, a), lanes over frames.
-// * Finish: one wave per voxel, lanes over frames.  Each active channel's
+// * Finish: one wave per voxel, lanes over frames (two per lane, one
+//   16-byte load, for an even frame count).  Each active channel's
 //   excess receive delay indexes its C row (linear, or Catmull-Rom inside
 //   the table), the rotated values give the signed-root DMAS sums and the CCF
 //   magnitudes; each angle's spline lookup of A, rotated by the residual
@@ -108,38 +109,76 @@ __global__ __launch_bounds__(256) void k_hybrid_tables(const float2* __restrict_
   }
 }
 
-// Table row `row` of nq knots at a non-negative position: linear between the
-// two knots around it (the last interval extended), or Catmull-Rom where the
-// four knots around it exist.
-__device__ __forceinline__ float2 table_value(const float2* table, size_t row, int nq, int nt, int t, float position,
-                                              bool cubic) {
+// F consecutive frames from p (F = 2: one 16-byte load; the caller keeps p
+// 16-byte aligned).
+template <int F>
+__device__ __forceinline__ void load_frames(const float2* p, float2 (&v)[F]) {
+  if constexpr (F == 1) {
+    v[0] = p[0];
+  } else {
+    const float4 q = *reinterpret_cast<const float4*>(p);
+    v[0] = make_float2(q.x, q.y);
+    v[1] = make_float2(q.z, q.w);
+  }
+}
+
+// Table row `row` of nq knots at a non-negative position, frames [t, t + F):
+// linear between the two knots around it (the last interval extended), or
+// Catmull-Rom where the four knots around it exist.
+template <int F>
+__device__ __forceinline__ void table_value(const float2* table, size_t row, int nq, int nt, int t, float position,
+                                            bool cubic, float2 (&out)[F]) {
   const int ip = static_cast<int>(position);
   if (cubic && ip != 0 && ip + 2 < nq) {
     const float2* p = table + (row * nq + ip) * nt + t;
     const float w = position - static_cast<float>(ip), v = 1.f - w;
-    const float2 p0 = p[0], p1 = p[nt], pm = p[-nt], p2 = p[2 * nt];
-    const float2 value = lerp2(p0, p1, w);
-    const float2 left = add2(add2(pm, p1), scale2(p0, -2.f));
-    const float2 right = add2(add2(p0, p2), scale2(p1, -2.f));
-    return add2(value, scale2(lerp2(left, right, w), -0.5f * w * v));
+    float2 p0[F], p1[F], pm[F], p2[F];
+    load_frames<F>(p, p0);
+    load_frames<F>(p + nt, p1);
+    load_frames<F>(p - nt, pm);
+    load_frames<F>(p + 2 * nt, p2);
+#pragma unroll
+    for (int f = 0; f < F; ++f) {
+      const float2 value = lerp2(p0[f], p1[f], w);
+      const float2 left = add2(add2(pm[f], p1[f]), scale2(p0[f], -2.f));
+      const float2 right = add2(add2(p0[f], p2[f]), scale2(p1[f], -2.f));
+      out[f] = add2(value, scale2(lerp2(left, right, w), -0.5f * w * v));
+    }
+    return;
   }
   const int i = min(ip, nq - 2);
   const float2* p = table + (row * nq + i) * nt + t;
-  return lerp2(p[0], p[nt], position - static_cast<float>(i));
+  float2 a[F], b[F];
+  load_frames<F>(p, a);
+  load_frames<F>(p + nt, b);
+#pragma unroll
+  for (int f = 0; f < F; ++f) out[f] = lerp2(a[f], b[f], position - static_cast<float>(i));
 }
 
 // Not-a-knot spline value of plane line `line`, angle a, at knot position pos
-// (zero outside the knots): the two-stage planes layout (line, knot, angle, frame).
-__device__ __forceinline__ float2 spline_value(const float2* plane, const float2* second, size_t line, int a, int t,
-                                               int na, int nk, int nt, float pos) {
+// (zero outside the knots), frames [t, t + F): the two-stage planes layout
+// (line, knot, angle, frame).
+template <int F>
+__device__ __forceinline__ void spline_value(const float2* plane, const float2* second, size_t line, int a, int t,
+                                             int na, int nk, int nt, float pos, float2 (&out)[F]) {
   const int i = static_cast<int>(floorf(pos));
-  if (i < 0 || i >= nk - 1) return make_float2(0.f, 0.f);
+  if (i < 0 || i >= nk - 1) {
+#pragma unroll
+    for (int f = 0; f < F; ++f) out[f] = make_float2(0.f, 0.f);
+    return;
+  }
   const float w = pos - static_cast<float>(i), u = 1.f - w;
   const size_t offset = ((line * nk + i) * na + a) * static_cast<size_t>(nt) + t;
   const size_t stride = static_cast<size_t>(na) * nt;
-  const float2 value = add2(scale2(plane[offset], u), scale2(plane[offset + stride], w));
-  return add2(value, add2(scale2(second[offset], (u * u * u - u) / 6.f),
-                          scale2(second[offset + stride], (w * w * w - w) / 6.f)));
+  float2 p0[F], p1[F], m0[F], m1[F];
+  load_frames<F>(plane + offset, p0);
+  load_frames<F>(plane + offset + stride, p1);
+  load_frames<F>(second + offset, m0);
+  load_frames<F>(second + offset + stride, m1);
+  const float cu = (u * u * u - u) / 6.f, cw = (w * w * w - w) / 6.f;
+#pragma unroll
+  for (int f = 0; f < F; ++f)
+    out[f] = add2(add2(scale2(p0[f], u), scale2(p1[f], w)), add2(scale2(m0[f], cu), scale2(m1[f], cw)));
 }
 
 struct FinishArgs {
@@ -154,6 +193,9 @@ struct FinishArgs {
   int cubic;
 };
 
+// F frames per lane (F = 2 needs an even frame count): a wave covers 64 F
+// frames of its voxel per pass.
+template <int F>
 __global__ __launch_bounds__(256) void k_hybrid_finish(FinishArgs p) {
   const long long blocks = gridDim.x;
   const long long voxel = xcd_block(blocks) * kHybridWaves + (threadIdx.x >> 6);
@@ -167,11 +209,16 @@ __global__ __launch_bounds__(256) void k_hybrid_finish(FinishArgs p) {
   const int z = p.first + dz;
   const float inv_na = 1.f / static_cast<float>(p.na);
   float total = 0.f;
-  for (int t0 = 0; t0 < p.nt; t0 += 64) {
-    const int t = t0 + lane;
-    const int tt = t < p.nt ? t : p.nt - 1;
-    float2 roots = make_float2(0.f, 0.f), coherent[2];
-    float magnitude = 0.f, energy[2];
+  for (int t0 = 0; t0 < p.nt; t0 += 64 * F) {
+    const int t = t0 + F * lane;
+    const int tt = t < p.nt ? t : p.nt - F;
+    float2 roots[F], coherent[2][F];
+    float magnitude[F], energy[2][F];
+#pragma unroll
+    for (int f = 0; f < F; ++f) {
+      roots[f] = make_float2(0.f, 0.f);
+      magnitude[f] = 0.f;
+    }
     int active = 0;
 #pragma unroll
     for (int conf = 0; conf < 2; ++conf) {
@@ -182,37 +229,50 @@ __global__ __launch_bounds__(256) void k_hybrid_finish(FinishArgs p) {
       for (int ch = 0; ch < p.nc; ++ch) {
         const float4 r = rx[ch];
         if (r.x < 0.f) continue;
-        const float2 val = rotate(table_value(p.tables[conf], static_cast<size_t>(u) * p.nc + ch, p.nq, p.nt, tt,
-                                              r.x * p.inverse_step, p.cubic != 0),
-                                  r.y, r.z);
-        const float m = sqrtf(val.x * val.x + val.y * val.y);
-        const float w = m > 0.f ? 1.f / sqrtf(m) : 0.f;
-        roots = add2(roots, scale2(val, w));
-        magnitude += m;
+        float2 val[F];
+        table_value<F>(p.tables[conf], static_cast<size_t>(u) * p.nc + ch, p.nq, p.nt, tt, r.x * p.inverse_step,
+                       p.cubic != 0, val);
+#pragma unroll
+        for (int f = 0; f < F; ++f) {
+          const float2 x = rotate(val[f], r.y, r.z);
+          const float m = sqrtf(x.x * x.x + x.y * x.y);
+          const float w = m > 0.f ? 1.f / sqrtf(m) : 0.f;
+          roots[f] = add2(roots[f], scale2(x, w));
+          magnitude[f] += m;
+        }
         ++active;
       }
-      float2 sum = make_float2(0.f, 0.f);
-      float e = 0.f;
+#pragma unroll
+      for (int f = 0; f < F; ++f) {
+        coherent[conf][f] = make_float2(0.f, 0.f);
+        energy[conf][f] = 0.f;
+      }
       for (int a = 0; a < p.na; ++a) {
         const float4 b = tx[a];
-        const float2 val = rotate(spline_value(p.planes[conf], p.second[conf], v, a, tt, p.na, p.nk, p.nt, b.x),
-                                  b.y, b.z);
-        sum = add2(sum, val);
-        e += val.x * val.x + val.y * val.y;
+        float2 val[F];
+        spline_value<F>(p.planes[conf], p.second[conf], v, a, tt, p.na, p.nk, p.nt, b.x, val);
+#pragma unroll
+        for (int f = 0; f < F; ++f) {
+          const float2 x = rotate(val[f], b.y, b.z);
+          coherent[conf][f] = add2(coherent[conf][f], x);
+          energy[conf][f] += x.x * x.x + x.y * x.y;
+        }
       }
-      coherent[conf] = sum;
-      energy[conf] = e;
     }
-    const float dm = roots.x * roots.x + roots.y * roots.y - magnitude;
-    float power = 0.f;
-    if (dm > 0.f && magnitude > 0.f && active > 0) {
-      const float r = (coherent[0].x * coherent[0].x + coherent[0].y * coherent[0].y) * inv_na * inv_na;
-      const float q = (coherent[1].x * coherent[1].x + coherent[1].y * coherent[1].y) * inv_na * inv_na;
-      const float var = fmaxf(0.f, energy[0] * inv_na - r) + fmaxf(0.f, energy[1] * inv_na - q);
-      const float num = sqrtf(r) * sqrtf(q), den = r + q + 0.5f * var - num;
-      if (den > 0.f) power = dm * (dm / static_cast<float>(active) / magnitude) * (num / den);
+#pragma unroll
+    for (int f = 0; f < F; ++f) {
+      const float dm = roots[f].x * roots[f].x + roots[f].y * roots[f].y - magnitude[f];
+      float power = 0.f;
+      if (dm > 0.f && magnitude[f] > 0.f && active > 0) {
+        const float2 c0 = coherent[0][f], c1 = coherent[1][f];
+        const float r = (c0.x * c0.x + c0.y * c0.y) * inv_na * inv_na;
+        const float q = (c1.x * c1.x + c1.y * c1.y) * inv_na * inv_na;
+        const float var = fmaxf(0.f, energy[0][f] * inv_na - r) + fmaxf(0.f, energy[1][f] * inv_na - q);
+        const float num = sqrtf(r) * sqrtf(q), den = r + q + 0.5f * var - num;
+        if (den > 0.f) power = dm * (dm / static_cast<float>(active) / magnitude[f]) * (num / den);
+      }
+      if (t + f < p.nt) total += power;
     }
-    if (t < p.nt) total += power;
   }
   total = wave_sum(total);
   if (lane == 0) p.power[(static_cast<size_t>(ix) * p.nz + z) * p.ny + iy] = total / static_cast<float>(p.nt);
@@ -301,6 +361,10 @@ extern "C" rcabeam_status rcabeam_hybrid_finish_device(const void* tables_rc, co
   p.nk = static_cast<int>(nk);
   p.inverse_step = inverse_step;
   p.cubic = cubic;
-  k_hybrid_finish<<<static_cast<unsigned>(blocks), 256>>>(p);
+  // Two frames per lane where rows and planes keep 16-byte alignment (even nt).
+  if (nt % 2 == 0)
+    k_hybrid_finish<2><<<static_cast<unsigned>(blocks), 256>>>(p);
+  else
+    k_hybrid_finish<1><<<static_cast<unsigned>(blocks), 256>>>(p);
   return launch_status();
 }
