@@ -45,13 +45,42 @@ __device__ __forceinline__ float2 lerp2(float2 a, float2 b, float w) {
 __device__ __forceinline__ float2 add2(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 scale2(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
 
-// Linear interpolation of a (ns, nt) trace at a wave-uniform position; zero
-// outside the closed window [0, ns - 1].
-__device__ __forceinline__ float2 trace_sample(const float2* trace, float pos, int ns, int nt, int t) {
-  if (!(pos >= 0.f) || pos > static_cast<float>(ns - 1)) return make_float2(0.f, 0.f);
+// F consecutive frames from p (F = 2: one 16-byte load; the caller keeps p
+// 16-byte aligned).
+template <int F>
+__device__ __forceinline__ void load_frames(const float2* p, float2 (&v)[F]) {
+  if constexpr (F == 1) {
+    v[0] = p[0];
+  } else {
+    const float4 q = *reinterpret_cast<const float4*>(p);
+    v[0] = make_float2(q.x, q.y);
+    v[1] = make_float2(q.z, q.w);
+  }
+}
+
+// Frames [t, t + F) of a (ns, nt) trace, linearly interpolated at a
+// wave-uniform position and rotated by (c, s), added to acc; nothing outside
+// the closed window [0, ns - 1].
+template <int F>
+__device__ __forceinline__ void add_sample(const float2* trace, float pos, int ns, int nt, int t, float c, float s,
+                                           float2 (&acc)[F]) {
+  if (!(pos >= 0.f) || pos > static_cast<float>(ns - 1)) return;
   const int i = static_cast<int>(pos);
-  return lerp2(trace[static_cast<size_t>(i) * nt + t], trace[static_cast<size_t>(min(i + 1, ns - 1)) * nt + t],
-               pos - static_cast<float>(i));
+  float2 a[F], b[F];
+  load_frames<F>(trace + static_cast<size_t>(i) * nt + t, a);
+  load_frames<F>(trace + static_cast<size_t>(min(i + 1, ns - 1)) * nt + t, b);
+#pragma unroll
+  for (int f = 0; f < F; ++f) acc[f] = add2(acc[f], rotate(lerp2(a[f], b[f], pos - static_cast<float>(i)), c, s));
+}
+
+// Store frames [t, t + F) that exist.
+template <int F>
+__device__ __forceinline__ void store_frames(float2* out, int t, int nt, const float2 (&v)[F]) {
+  if (t >= nt) return;
+  if constexpr (F == 1)
+    out[t] = v[0];
+  else
+    *reinterpret_cast<float4*>(out + t) = make_float4(v[0].x, v[0].y, v[1].x, v[1].y);
 }
 
 // Block b's logical index: each XCD (blocks b % 8) takes a contiguous run.
@@ -60,6 +89,7 @@ __device__ __forceinline__ long long xcd_block(long long blocks) {
   return static_cast<long long>(blockIdx.x % kXcdCount) * per + blockIdx.x / kXcdCount;
 }
 
+template <int F>
 __global__ __launch_bounds__(256) void k_hybrid_planes(const float2* __restrict__ raw, const float4* __restrict__ prx,
                                                        const float* __restrict__ ptx, float2* __restrict__ out,
                                                        long long entries, int nc, int na, int ns, int nt, int nk) {
@@ -70,20 +100,23 @@ __global__ __launch_bounds__(256) void k_hybrid_planes(const float2* __restrict_
   const int k = static_cast<int>((entry / na) % nk);
   const long long v = entry / na / nk;
   const float tx = ptx[k * na + a];
-  for (int t0 = 0; t0 < nt; t0 += 64) {
-    const int t = t0 + lane;
-    const int tt = t < nt ? t : nt - 1;
-    float2 sum = make_float2(0.f, 0.f);
+  for (int t0 = 0; t0 < nt; t0 += 64 * F) {
+    const int t = t0 + F * lane;
+    const int tt = t < nt ? t : nt - F;
+    float2 sum[F];
+#pragma unroll
+    for (int f = 0; f < F; ++f) sum[f] = make_float2(0.f, 0.f);
     for (int ch = 0; ch < nc; ++ch) {
       const float4 r = prx[(v * nc + ch) * nk + k];
       if (r.w == 0.f) continue;
       const float2* trace = raw + (static_cast<size_t>(ch) * na + a) * ns * static_cast<size_t>(nt);
-      sum = add2(sum, rotate(trace_sample(trace, r.x + tx, ns, nt, tt), r.y, r.z));
+      add_sample<F>(trace, r.x + tx, ns, nt, tt, r.y, r.z, sum);
     }
-    if (t < nt) out[entry * nt + t] = sum;
+    store_frames<F>(out + entry * nt, t, nt, sum);
   }
 }
 
+template <int F>
 __global__ __launch_bounds__(256) void k_hybrid_tables(const float2* __restrict__ raw,
                                                        const float4* __restrict__ anchors, float2* __restrict__ out,
                                                        long long entries, int ns, int nc, int na, int nt, int nu,
@@ -96,29 +129,18 @@ __global__ __launch_bounds__(256) void k_hybrid_tables(const float2* __restrict_
   const int u = static_cast<int>(entry / nq / nc);
   const float4* b = anchors + (static_cast<size_t>(slab) * nu + u) * na;
   const float shift = static_cast<float>(k) * step;
-  for (int t0 = 0; t0 < nt; t0 += 64) {
-    const int t = t0 + lane;
-    const int tt = t < nt ? t : nt - 1;
-    float2 sum = make_float2(0.f, 0.f);
+  for (int t0 = 0; t0 < nt; t0 += 64 * F) {
+    const int t = t0 + F * lane;
+    const int tt = t < nt ? t : nt - F;
+    float2 sum[F];
+#pragma unroll
+    for (int f = 0; f < F; ++f) sum[f] = make_float2(0.f, 0.f);
     for (int a = 0; a < na; ++a) {
       const float4 g = b[a];
       const float2* trace = raw + (static_cast<size_t>(ch) * na + a) * ns * static_cast<size_t>(nt);
-      sum = add2(sum, rotate(trace_sample(trace, g.x + shift, ns, nt, tt), g.y, g.z));
+      add_sample<F>(trace, g.x + shift, ns, nt, tt, g.y, g.z, sum);
     }
-    if (t < nt) out[entry * nt + t] = sum;
-  }
-}
-
-// F consecutive frames from p (F = 2: one 16-byte load; the caller keeps p
-// 16-byte aligned).
-template <int F>
-__device__ __forceinline__ void load_frames(const float2* p, float2 (&v)[F]) {
-  if constexpr (F == 1) {
-    v[0] = p[0];
-  } else {
-    const float4 q = *reinterpret_cast<const float4*>(p);
-    v[0] = make_float2(q.x, q.y);
-    v[1] = make_float2(q.z, q.w);
+    store_frames<F>(out + entry * nt, t, nt, sum);
   }
 }
 
@@ -280,6 +302,11 @@ __global__ __launch_bounds__(256) void k_hybrid_finish(FinishArgs p) {
 
 bool fits(size_t n) { return n <= 0x7fffffffu; }
 
+// Rows of nt complex64 frames at a and b stay 16-byte aligned.
+bool even_frames(size_t nt, const void* a, const void* b) {
+  return nt % 2 == 0 && reinterpret_cast<uintptr_t>(a) % 16 == 0 && reinterpret_cast<uintptr_t>(b) % 16 == 0;
+}
+
 }  // namespace
 }  // namespace rcb
 
@@ -295,7 +322,9 @@ extern "C" rcabeam_status rcabeam_hybrid_planes_device(const void* packed, const
   if (entries == 0) return RCABEAM_SUCCESS;
   const long long blocks = (entries + kHybridWaves - 1) / kHybridWaves;
   if (blocks > 0x7fffffffLL) return RCABEAM_ERROR_ARGUMENT;
-  k_hybrid_planes<<<static_cast<unsigned>(blocks), 256>>>(
+  // Two frames per lane where rows keep 16-byte alignment (even nt, aligned buffers).
+  auto kernel = even_frames(nt, packed, planes) ? k_hybrid_planes<2> : k_hybrid_planes<1>;
+  kernel<<<static_cast<unsigned>(blocks), 256>>>(
       static_cast<const float2*>(packed), reinterpret_cast<const float4*>(prx), ptx, static_cast<float2*>(planes),
       entries, static_cast<int>(nc), static_cast<int>(na), static_cast<int>(ns), static_cast<int>(nt),
       static_cast<int>(nk));
@@ -312,7 +341,8 @@ extern "C" rcabeam_status rcabeam_hybrid_tables_device(const void* packed, const
   if (entries == 0) return RCABEAM_SUCCESS;
   const long long blocks = (entries + kHybridWaves - 1) / kHybridWaves;
   if (blocks > 0x7fffffffLL) return RCABEAM_ERROR_ARGUMENT;
-  k_hybrid_tables<<<static_cast<unsigned>(blocks), 256>>>(
+  auto kernel = even_frames(nt, packed, tables) ? k_hybrid_tables<2> : k_hybrid_tables<1>;
+  kernel<<<static_cast<unsigned>(blocks), 256>>>(
       static_cast<const float2*>(packed), reinterpret_cast<const float4*>(anchors), static_cast<float2*>(tables),
       entries, static_cast<int>(ns), static_cast<int>(nc), static_cast<int>(na), static_cast<int>(nt),
       static_cast<int>(nu), static_cast<int>(slab), static_cast<int>(nq), step);
@@ -362,7 +392,8 @@ extern "C" rcabeam_status rcabeam_hybrid_finish_device(const void* tables_rc, co
   p.inverse_step = inverse_step;
   p.cubic = cubic;
   // Two frames per lane where rows and planes keep 16-byte alignment (even nt).
-  if (nt % 2 == 0)
+  if (even_frames(nt, tables_rc, tables_cr) && even_frames(nt, planes_rc, planes_cr) &&
+      even_frames(nt, second_rc, second_cr))
     k_hybrid_finish<2><<<static_cast<unsigned>(blocks), 256>>>(p);
   else
     k_hybrid_finish<1><<<static_cast<unsigned>(blocks), 256>>>(p);
