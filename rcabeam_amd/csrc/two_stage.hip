@@ -315,13 +315,34 @@ __device__ __forceinline__ float2 spline_value(const float2* p, const float2* s,
   return r;
 }
 
+// Spline values of frames f and f + 1 (f even, rows 16-byte aligned): one
+// 16-byte load per row.
+__device__ __forceinline__ void spline_pair(const float2* p, const float2* s, const Lookup& l, long long knot_stride,
+                                            int f, float2& r0, float2& r1) {
+  if (l.offset < 0) {
+    r0 = r1 = make_float2(0.f, 0.f);
+    return;
+  }
+  const float4 p0 = *reinterpret_cast<const float4*>(p + l.offset + f);
+  const float4 p1 = *reinterpret_cast<const float4*>(p + l.offset + knot_stride + f);
+  const float4 s0 = *reinterpret_cast<const float4*>(s + l.offset + f);
+  const float4 s1 = *reinterpret_cast<const float4*>(s + l.offset + knot_stride + f);
+  r0.x = fmaf(l.ca, p0.x, fmaf(l.cw, p1.x, fmaf(l.sa, s0.x, l.sw * s1.x)));
+  r0.y = fmaf(l.ca, p0.y, fmaf(l.cw, p1.y, fmaf(l.sa, s0.y, l.sw * s1.y)));
+  r1.x = fmaf(l.ca, p0.z, fmaf(l.cw, p1.z, fmaf(l.sa, s0.z, l.sw * s1.z)));
+  r1.y = fmaf(l.ca, p0.w, fmaf(l.cw, p1.w, fmaf(l.sa, s0.w, l.sw * s1.w)));
+}
+
 __device__ __forceinline__ float2 coherence(float2 r, float2 q, float var_r, float var_q) {
   const float2 n = cmulc(r, q);
   const float d = cnorm(r) + cnorm(q) + 0.5f * (var_r + var_q) - cabs(n);
   return d > 0.f ? make_float2(n.x / d, n.y / d) : make_float2(0.f, 0.f);
 }
 
-template <int G, bool STSW>
+// PAIRS (G even, even chunk): frame slot g of a lane is frame
+// f0 + (g / 2) 128 + 2 lane + g % 2, so each pair of slots is one 16-byte load
+// per plane row; otherwise slot g is frame f0 + 64 g + lane.
+template <int G, bool STSW, bool PAIRS>
 __global__ __launch_bounds__(256) void ts_expand(ExpandArgs a) {
   const int lane = threadIdx.x & 63;
   const long long task = static_cast<long long>(blockIdx.x) * kWaves + (threadIdx.x >> 6);
@@ -366,12 +387,28 @@ __global__ __launch_bounds__(256) void ts_expand(ExpandArgs a) {
         ph_q = cmul(axial, a.x_phase[static_cast<size_t>(ix) * a.na + angle]);
       }
       const int h = angle & 1;
+      float2 rv[G], qv[G];
+      if constexpr (PAIRS) {
+#pragma unroll
+        for (int g = 0; g < G; g += 2) {
+          const int f = f0 + (g / 2) * 128 + 2 * lane;
+          if (f >= a.count) continue;
+          spline_pair(a.rc, a.rc_second, lr, knot_stride, f, rv[g], rv[g + 1]);
+          spline_pair(a.cr, a.cr_second, lq, knot_stride, f, qv[g], qv[g + 1]);
+        }
+      }
 #pragma unroll
       for (int g = 0; g < G; ++g) {
-        const int f = f0 + g * 64 + lane;
+        const int f = PAIRS ? f0 + (g / 2) * 128 + 2 * lane + g % 2 : f0 + g * 64 + lane;
         if (f >= a.count) continue;
-        float2 r = spline_value(a.rc, a.rc_second, lr, knot_stride, f);
-        float2 q = spline_value(a.cr, a.cr_second, lq, knot_stride, f);
+        float2 r, q;
+        if constexpr (PAIRS) {
+          r = rv[g];
+          q = qv[g];
+        } else {
+          r = spline_value(a.rc, a.rc_second, lr, knot_stride, f);
+          q = spline_value(a.cr, a.cr_second, lq, knot_stride, f);
+        }
         if (baseband) {
           r = cmul(r, ph_r);
           q = cmul(q, ph_q);
@@ -396,7 +433,7 @@ __global__ __launch_bounds__(256) void ts_expand(ExpandArgs a) {
     }
 #pragma unroll
     for (int g = 0; g < G; ++g) {
-      const int f = f0 + g * 64 + lane;
+      const int f = PAIRS ? f0 + (g / 2) * 128 + 2 * lane + g % 2 : f0 + g * 64 + lane;
       if (f >= a.count) continue;
       if (!STSW) {
         static_cast<float2*>(a.out)[v * a.nt + a.first + f] = sum[g];
@@ -520,19 +557,43 @@ rcabeam_status launch_expand(const ExpandArgs& a, bool stsw) {
   if (voxels == 0 || a.count == 0) return RCABEAM_SUCCESS;
   const int G = frame_groups(a.count);
   const unsigned blocks = static_cast<unsigned>((voxels + kWaves - 1) / kWaves);
+  // Frame pairs: an even frame-group count, even chunk rows and 16-byte aligned planes.
+  auto aligned = [](const void* q) { return reinterpret_cast<uintptr_t>(q) % 16 == 0; };
+  const bool pairs = G % 2 == 0 && a.chunk % 2 == 0 && aligned(a.rc) && aligned(a.rc_second) && aligned(a.cr) &&
+                     aligned(a.cr_second);
   if (stsw) {
     switch (G) {
-      case 1: ts_expand<1, true><<<blocks, 256>>>(a); break;
-      case 2: ts_expand<2, true><<<blocks, 256>>>(a); break;
-      case 3: ts_expand<3, true><<<blocks, 256>>>(a); break;
-      default: ts_expand<4, true><<<blocks, 256>>>(a); break;
+      case 1: ts_expand<1, true, false><<<blocks, 256>>>(a); break;
+      case 2:
+        if (pairs)
+          ts_expand<2, true, true><<<blocks, 256>>>(a);
+        else
+          ts_expand<2, true, false><<<blocks, 256>>>(a);
+        break;
+      case 3: ts_expand<3, true, false><<<blocks, 256>>>(a); break;
+      default:
+        if (pairs)
+          ts_expand<4, true, true><<<blocks, 256>>>(a);
+        else
+          ts_expand<4, true, false><<<blocks, 256>>>(a);
+        break;
     }
   } else {
     switch (G) {
-      case 1: ts_expand<1, false><<<blocks, 256>>>(a); break;
-      case 2: ts_expand<2, false><<<blocks, 256>>>(a); break;
-      case 3: ts_expand<3, false><<<blocks, 256>>>(a); break;
-      default: ts_expand<4, false><<<blocks, 256>>>(a); break;
+      case 1: ts_expand<1, false, false><<<blocks, 256>>>(a); break;
+      case 2:
+        if (pairs)
+          ts_expand<2, false, true><<<blocks, 256>>>(a);
+        else
+          ts_expand<2, false, false><<<blocks, 256>>>(a);
+        break;
+      case 3: ts_expand<3, false, false><<<blocks, 256>>>(a); break;
+      default:
+        if (pairs)
+          ts_expand<4, false, true><<<blocks, 256>>>(a);
+        else
+          ts_expand<4, false, false><<<blocks, 256>>>(a);
+        break;
     }
   }
   return launch_status();
