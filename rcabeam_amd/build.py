@@ -1,6 +1,7 @@
 """Build the MI355X shared library `rcabeam_amd/lib/librcabeam_mi355x.so` with hipcc.
 
-Usage: python -m rcabeam_amd.build  (or `__graft_entry__.build()`).
+Usage: python -m rcabeam_amd.build [-v] [--incremental]  (or `__graft_entry__.build()`).
+`--incremental` recompiles only sources newer than their object (or than any header).
 
 Every HIP source is compiled for gfx950 only, with packed-FP32 instruction
 generation disabled (`-mno-packed-fp32` equivalent feature flag), and linked
@@ -45,8 +46,11 @@ def _hipcc() -> str:
     return str(candidate) if candidate.exists() else (shutil.which("hipcc") or "hipcc")
 
 
-def build(verbose: bool = False) -> Path:
-    """Compile all sources and link the shared library; return its path."""
+def build(verbose: bool = False, incremental: bool = False) -> Path:
+    """Compile all sources and link the shared library; return its path.
+
+    With `incremental`, a source whose object is newer than the source and every header is not
+    recompiled (the default rebuilds everything)."""
     OUT_DIR.mkdir(parents=True, exist_ok=True)
     obj_dir = ROOT / "_build"
     obj_dir.mkdir(exist_ok=True)
@@ -54,8 +58,13 @@ def build(verbose: bool = False) -> Path:
     includes = [f"-I{ROOT / 'include'}", f"-I{CSRC}"]
     sources = [CSRC / s for s in SOURCES if (CSRC / s).exists()]
 
+    headers = [*CSRC.glob("*.hpp"), *(ROOT / "include").glob("*.h")]
+    newest_header = max((h.stat().st_mtime for h in headers), default=0.0)
+
     def compile_one(src: Path) -> Path:
         obj = obj_dir / (src.stem + ".o")
+        if incremental and obj.exists() and obj.stat().st_mtime > max(src.stat().st_mtime, newest_header):
+            return obj
         cmd = [hipcc, *HIPCC_FLAGS, *includes, "-c", str(src), "-o", str(obj)]
         if verbose:
             print(" ".join(cmd), flush=True)
@@ -91,5 +100,5 @@ def build(verbose: bool = False) -> Path:
 
 
 if __name__ == "__main__":
-    path = build(verbose="-v" in sys.argv)
+    path = build(verbose="-v" in sys.argv, incremental="--incremental" in sys.argv)
     print(path)

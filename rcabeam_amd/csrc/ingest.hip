@@ -32,6 +32,7 @@ struct Staging {
   bool ready = false;
   void* buffer[kStagingBuffers] = {};
   hipEvent_t drained[kStagingBuffers] = {};  // the copy out of buffer b has finished
+  hipEvent_t ordered = nullptr;               // null-stream work enqueued before the call
   hipStream_t stream = nullptr;
 };
 Staging g_staging[kMaxDevices];
@@ -39,6 +40,7 @@ Staging g_staging[kMaxDevices];
 hipError_t prepare(Staging& s) {
   if (s.ready) return hipSuccess;
   hipError_t e = hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&s.ordered, hipEventDisableTiming);
   for (int b = 0; b < kStagingBuffers && e == hipSuccess; ++b) {
     e = hipHostMalloc(&s.buffer[b], kChunkBytes, hipHostMallocDefault);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&s.drained[b], hipEventDisableTiming);
@@ -75,7 +77,10 @@ extern "C" rcabeam_status rcabeam_upload_host(const void* host, void* device, si
   const int t = std::min(threads, kMaxThreads);
   const auto* src = static_cast<const unsigned char*>(host);
   auto* dst = static_cast<unsigned char*>(device);
-  hipError_t e = hipSuccess;
+  // The copies run on a non-blocking stream: order them after everything already enqueued on the
+  // null stream (where every other entry point launches), which may still read or write `device`.
+  hipError_t e = hipEventRecord(s.ordered, nullptr);
+  if (e == hipSuccess) e = hipStreamWaitEvent(s.stream, s.ordered, 0);
   for (size_t off = 0, k = 0; off < bytes && e == hipSuccess; off += kChunkBytes, ++k) {
     const size_t n = std::min(kChunkBytes, bytes - off);
     const int b = static_cast<int>(k % kStagingBuffers);

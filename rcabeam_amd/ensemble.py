@@ -182,6 +182,13 @@ def run_ensemble(packed_rc, packed_cr, g: _Geometry, shape4, method: int, k: int
             dev.ptr(packed_rc), dev.ptr(packed_cr), dev.ptr(scan), *common, dev.ptr(w_view), dev.ptr(workspace),
             dev.ptr(g.geometry_workspace), ns, nc, na, nt, count, c, fs, fd, fn, method, k,
         )
+    elif storage == "float16" and signal_storage == 0:
+        # The reference's FP16 entry point (ensemble.cu:798-809): float32 interpolation and accumulation.
+        _lib.call(
+            "rcabeam_opw_fp16_device",
+            dev.ptr(packed_rc), dev.ptr(packed_cr), dev.ptr(scan), *common, dev.ptr(g.geometry_workspace),
+            ns, nc, na, nt, count, c, fs, fd, fn,
+        )
     else:
         # 16-bit channels (OPW only): float32-accurate coefficients, float32 accumulation.
         _lib.call(

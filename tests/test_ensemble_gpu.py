@@ -403,3 +403,56 @@ def test_grid_entry_rejects_unsupported_shapes():
         status = lib.rcabeam_opw_grid_device(one * 16, one * 16, 0, 0, 16, 16, 16, 1, 1, 1, 16, 16, 16, 16, 16, 0,
                                              0, method, 8, 1, na, 4, 1540.0, 20e6, 15e6, 1.0)
         assert status == 2, (na, method)
+
+
+@pytest.mark.parametrize("frames", [1, 3, 8, 66, 200])
+def test_opw_fp16_entry_point_matches_quantized_fp32(frames):
+    """rcabeam_opw_fp16_device through the C ABI (reference include/rcabeam.h:222-235,
+    ensemble.cu:798-809; cases of reference tests/test_opw_fp16.py:12-44): FP16-packed channels give
+    the float32 reconstruction of the FP16-rounded channels, at scales 0, 1, 1e-4 and 1e-6 (the last
+    puts components into the FP16 subnormal range).  Samples are exact in float32 and the arithmetic is
+    float32 in both, so the bar is float32 rounding (rel L2 1e-6)."""
+    import torch
+
+    from rcabeam_amd import _device as dev
+    from rcabeam_amd import _lib
+    from rcabeam_amd.ensemble import _Geometry, pack_channels
+
+    lib = _lib.load()
+    rng = np.random.default_rng(111)
+    shape = (48, 3, 4, frames)
+    rc = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
+    cr = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
+    el = np.linspace(-0.001, 0.001, 3)
+    geom = rb.RCAGeometry(el, el, fs=2e6, f_demod=1e6)
+    angles = np.linspace(-0.1, 0.1, 4)
+    grid = (np.linspace(-0.001, 0.001, 3), np.array([0.005, 0.007]), np.linspace(-0.001, 0.003, 5))
+    g = _Geometry(angles, 1e-6, grid, geom)
+    c, fs, fd, fn = g.constants
+    ns, nc, na, nt = shape
+    for scale in (0.0, 1.0, 1e-4, 1e-6):
+        a, b = rc * scale, cr * scale
+        q = [(x.real.astype(np.float16).astype(np.float32) + 1j * x.imag.astype(np.float16).astype(np.float32))
+             .astype(np.complex64) for x in (a, b)]
+        packed = [pack_channels(dev.to_device(x, torch.complex64), "float16") for x in (a, b)]
+        pd = dev.empty((g.nv, 2), torch.float32)
+        signal = dev.empty((g.nv, nt), torch.complex64)
+        _lib.check(lib.rcabeam_opw_fp16_device(
+            dev.ptr(packed[0]), dev.ptr(packed[1]), dev.ptr(g.scan), dev.ptr(g.xe), dev.ptr(g.ye), dev.ptr(g.angles),
+            dev.ptr(g.starts), dev.ptr(pd), dev.ptr(signal), dev.ptr(g.geometry_workspace), ns, nc, na, nt, g.nv,
+            c, fs, fd, fn), "opw_fp16")
+        dev.synchronize()
+        expected = rb.beamform_ensemble(q[0], q[1], angles, 1e-6, grid, geom).reshape(g.nv, nt)
+        got = signal.cpu().numpy()
+        power = pd.cpu().numpy()
+        if scale == 0.0:
+            assert not got.any() and not power.any()
+            continue
+        assert rel_l2(got, expected) < 1e-6, scale
+        assert rel_l2(power[:, 0], np.mean(np.abs(expected.astype(np.complex128)) ** 2, axis=-1)) < 1e-6, scale
+        assert not power[:, 1].any()
+        # The geometry workspace is required (reference contract, ensemble.cu:674-675).
+        status = lib.rcabeam_opw_fp16_device(
+            dev.ptr(packed[0]), dev.ptr(packed[1]), dev.ptr(g.scan), dev.ptr(g.xe), dev.ptr(g.ye), dev.ptr(g.angles),
+            dev.ptr(g.starts), dev.ptr(pd), dev.ptr(signal), None, ns, nc, na, nt, g.nv, c, fs, fd, fn)
+        assert status == _lib.ERROR_ARGUMENT

@@ -60,3 +60,22 @@ def test_host_input_above_staging_threshold_matches_device_input():
     from_device = rb.beamform_ensemble(rc_d, cr_d, acq.angles, acq.t_start, acq.grid, geom).cpu().numpy()
     np.testing.assert_array_equal(from_host, from_device)
     assert rel_l2(from_host, from_device) == 0.0
+
+
+def test_staged_upload_waits_for_earlier_null_stream_work():
+    """The staged copies run on their own stream; work already enqueued on the null stream that writes the
+    destination must finish first, so the buffer ends up holding exactly the host data."""
+    import torch
+
+    from rcabeam_amd import _device as dev
+    from rcabeam_amd import _lib
+
+    count = 3 * (64 << 20) // 8 + 17
+    rng = np.random.default_rng(3)
+    host = (rng.standard_normal(count) + 1j * rng.standard_normal(count)).astype(np.complex64)
+    out = torch.empty(count, dtype=torch.complex64, device="cuda")
+    junk = torch.full_like(out, 7.0 - 3.0j)
+    for _ in range(200):  # ~tens of ms of null-stream writes into the destination
+        out.copy_(junk)
+    _lib.call("rcabeam_upload_host", host.ctypes.data, out.data_ptr(), host.nbytes, dev.upload_threads())
+    np.testing.assert_array_equal(out.cpu().numpy(), host)

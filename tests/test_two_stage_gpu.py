@@ -53,9 +53,14 @@ def _gpu(x):
     return torch.from_numpy(np.ascontiguousarray(x)).cuda()
 
 
-@pytest.mark.parametrize("frames,chunk", [(1, 1), (7, 3), (35, 32), (70, 64), (300, 300)])
+@pytest.mark.parametrize("frames,chunk", [(1, 1), (7, 3), (35, 32), (70, 64), (300, 300), (127, 128), (100, 100),
+                                          (301, 302)])
 def test_two_stage_matches_oracle(frames, chunk):
-    """Odd shapes, partial chunks and several frame groups (reference test_two_stage.py:74-86)."""
+    """Odd shapes, partial chunks and several frame groups (reference test_two_stage.py:74-86).  The
+    expansion reads frame pairs when a chunk has an even number of 64-frame groups and even rows:
+    (127, 128) and (100, 100) take the two-group pair kernel, (300, 300) the four-group one, and
+    (301, 302) an odd frame count in an even chunk, whose last pair reads the unwritten row at index
+    301 and must drop it."""
     from rcabeam_amd.two_stage import TwoStageOpwWorkspace
 
     rc, cr, angles, starts, grid, x_el, y_el, fn = _case(frames)
@@ -127,10 +132,11 @@ def test_aperture_uses_output_depth(baseband, n_planes, spw):
     assert _rel(out, expected) < BOUND
 
 
-@pytest.mark.parametrize("n_planes", [1, 2])
-def test_stsw_matches_oracle(n_planes):
+@pytest.mark.parametrize("n_planes,frames", [(1, 35), (2, 35), (1, 128), (2, 301)])
+def test_stsw_matches_oracle(n_planes, frames):
     """Subset assignment, remodulation, frame tails and global normalisation
-    (reference test_two_stage.py:227-259)."""
+    (reference test_two_stage.py:227-259).  128 frames take the two-group frame-pair expansion; 301
+    frames run a 256-frame chunk (four-group pairs) and a 45-frame tail."""
     import torch
 
     from rcabeam_amd.two_stage import TwoStageStswWorkspace
@@ -139,12 +145,12 @@ def test_stsw_matches_oracle(n_planes):
     el = np.array([-0.0003, 0.0001])
     angles = np.array([-0.1, -0.04, 0.03, 0.08, 0.13])
     grid = (np.array([-0.0005, 0.0002]), np.array([0.003, 0.004]), np.array([-0.0007, 0.0003]))
-    shape = (96, 2, len(angles), 35)
+    shape = (96, 2, len(angles), frames)
     raw = [(rng.normal(size=shape) + 1j * rng.normal(size=shape)).astype(np.complex64) for _ in range(2)]
     ew, ep = ots.two_stage_stsw(*raw, angles, 0, grid, el, el[::-1], C, 12e6, 5e6, 1.0, n_planes=n_planes)
     ws = TwoStageStswWorkspace(shape, angles, 0, grid, _geom(el, el[::-1], 1.0, fs=12e6, fd=5e6), n_planes=n_planes)
     w, p = ws.reconstruct(*map(_gpu, raw))
-    print(f"n_planes {n_planes}: weights {_rel(w.cpu().numpy(), ew):.2e} power {_rel(p.cpu().numpy(), ep):.2e}")
+    print(f"n_planes {n_planes} frames {frames}: weights {_rel(w.cpu().numpy(), ew):.2e} power {_rel(p.cpu().numpy(), ep):.2e}")
     assert _rel(w.cpu().numpy(), ew) < BOUND
     assert _rel(p.cpu().numpy(), ep) < BOUND
     for v in ws.reconstruct(*[torch.zeros(shape, dtype=torch.complex64, device="cuda") for _ in range(2)]):
