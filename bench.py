@@ -41,6 +41,8 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
+FP64_PEAK_TFLOPS = 78.6
+BF16_PEAK_TFLOPS = 2500.0  # dense bf16 / fp16 MFMA
 FLOPS_PER_TAP = 14  # 2 sub + 2 fma (interpolation) + 4 fma (rotate-accumulate), as counted in the reference kernel
 PROFILES = ROOT / "profiles"
 # The delay-and-sum kernel each precision's rcabeam_opw_grid_device launch runs on this workload.
@@ -76,7 +78,8 @@ def parse():
                    help="only time the two-stage approximation on the full-size scene (profiling runs)")
     p.add_argument("--combiners-only", action="store_true",
                    help="only time the combiners on the full-size scene (profiling runs)")
-    p.add_argument("--cpu-voxels", type=int, default=2400)
+    p.add_argument("--cpu-voxels", type=int, default=600,
+                   help="full-size voxels at which each form is compared with the float64 oracle (0: skip)")
     p.add_argument("--copy-calibration", action="store_true",
                    help="only run a plain 1 GiB device-to-device copy three times (HBM counter calibration)")
     return p.parse_args()
@@ -164,11 +167,20 @@ class Pipeline:
 
         _lib.check(getattr(self.lib, name)(*args), name)
 
-    def step(self, das_events=None):
-        """Enqueue one full step on the null stream; optional events bracket the DAS launch."""
+    def step(self, das_events=None, marks=None):
+        """Enqueue one full step on the null stream.  Optional `das_events` bracket the DAS launch;
+        `marks` (a list) receives (stage name, event) pairs recorded at the end of each stage."""
         from rcabeam_amd import _lib
 
         dev, nv = self.dev, self.g.nv
+
+        def mark(name):
+            if marks is not None:
+                ev = self.torch.cuda.Event(enable_timing=True)
+                ev.record()
+                marks.append((name, ev))
+
+        mark("start")
         c, fs, fd, fn = self.g.constants
         shape = (self.ns, self.nc, self.na, self.nt)
         channel_code = _lib.STORAGE[self.channel_storage]
@@ -187,6 +199,7 @@ class Pipeline:
             self._call("rcabeam_eigh_device", dev.ptr(self.c_gram), dev.ptr(self.c_values), dev.ptr(self.e_work),
                        dev.ptr(self.info), self.nt)
             self.basis.copy_(self.c_gram[self.nt - self.k:].flip(0))
+            mark("channel_basis")
             for raw, packed in zip((self.rc, self.cr), self.packed):
                 self._call("rcabeam_pack_ensemble_lowrank_device", dev.ptr(raw), dev.ptr(packed), channel_code,
                            dev.ptr(self.basis), self.k, dev.ptr(self.range_error), *shape)
@@ -198,6 +211,7 @@ class Pipeline:
                     pack = ("rcabeam_pack_ensemble_fp16_device" if self.storage == "float16"
                             else "rcabeam_pack_ensemble_bf16_device")
                     self._call(pack, dev.ptr(raw), dev.ptr(packed), dev.ptr(self.range_error), *shape)
+        mark("pack")
         if das_events is not None:
             das_events[0].record()
         # OPW on the Cartesian grid (8 x 8 lateral tiles).  float32: complex64 channels and IQ, VALU;
@@ -212,9 +226,11 @@ class Pipeline:
                    self.ntc, c, fs, fd, fn)
         if das_events is not None:
             das_events[1].record()
+        mark("das")
         if self.lowrank:
             self._call("rcabeam_lowrank_expand_device", dev.ptr(self.coded), dev.ptr(self.basis), self.k,
                        dev.ptr(self.signal), 0, dev.ptr(self.pd), nv, self.nt)
+            mark("expand")
         self._call("rcabeam_iq_absmax_device", dev.ptr(self.signal), self.code, nv * self.nt, dev.ptr(self.stats), 0)
         if self.exact_gram:
             self._call("rcabeam_temporal_gram_f64_device", dev.ptr(self.signal), nv, self.nt, dev.ptr(self.real),
@@ -222,14 +238,17 @@ class Pipeline:
         else:
             self._call("rcabeam_temporal_gram_device", dev.ptr(self.signal), self.code, nv, self.nt,
                        dev.ptr(self.stats), dev.ptr(self.real), dev.ptr(self.work), self.work_bytes, 0)
+        mark("gram")
         self._call("rcabeam_gram_hermitian_device", dev.ptr(self.real), self.nt, 1.0 / nv, dev.ptr(self.gram))
         self._call("rcabeam_eigh_device", dev.ptr(self.gram), dev.ptr(self.values), dev.ptr(self.e_work),
                    dev.ptr(self.info), self.nt)
         # Discarded vectors = eigenvectors of the rank_start largest eigenvalues (descending order):
         # rows nt-1, nt-2, ... of the row-major view of the column-major eigenvector matrix.
         self.vectors.copy_(self.gram[self.nt - self.rank_start:].flip(0))
+        mark("evd")
         self._call("rcabeam_filtered_power_device", dev.ptr(self.signal), self.code, nv, self.nt,
                    dev.ptr(self.vectors), self.rank_start, 1, dev.ptr(self.power))
+        mark("project")
 
 
 def _lib_coef_16bit() -> int:
@@ -238,31 +257,110 @@ def _lib_coef_16bit() -> int:
     return _lib.COEF_16BIT
 
 
-def cpu_baseline(scene, rc, cr, rank_start, pick):
-    """Time the float64 NumPy oracle (OPW + Gram/EVD clutter filter + PD) on a voxel sample `pick` of the same
-    workload: (its IQ there, the baseline record).  The IQ gives the float64 oracle values of the parity lines."""
-    from oracle import clutter as oc
-    from oracle import das
+_C1 = {}  # the 64 + 64-element acquisition, inherited by forked CPU-baseline workers
 
-    voxels = len(pick)
-    scan = das.scan_coords(scene.grid)
-    rc_h = rc.cpu().numpy()
-    cr_h = cr.cpu().numpy()
-    g = scene.geom
-    begin = time.perf_counter()
-    iq = das.opw_iq(rc_h, cr_h, scan[pick], g.x_el, g.y_el, scene.angles, scene.t_start, g.c, g.fs, g.f_demod, g.fnumber)
-    _, basis = oc.temporal_basis(iq, method="gram")
-    oc.filtered_power(iq, basis, rank_start=rank_start)
-    seconds = time.perf_counter() - begin
+
+def _c1_chunk(bounds):
+    """Unfiltered OPW power of voxels [lo, hi) of the 64 + 64-element workload in the float64 oracle."""
+    from oracle import combiners, das
+
+    lo, hi = bounds
+    acq, scan = _C1["acq"], _C1["scan"]
+    iq = das.opw_iq(acq.rc, acq.cr, scan[lo:hi], acq.x_el, acq.y_el, acq.angles, acq.t_start, acq.c, acq.fs, acq.fd,
+                    acq.fnumber)
+    return combiners.power_doppler(iq)
+
+
+def cpu_baseline(reps: int = 5) -> dict:
+    """The float64 NumPy restatement of the reference CPU path (`_delay_numpy` -> OPW -> `power_doppler`,
+    reference sim.py:344-393, opw.py:8-21, power.py:8-21; here oracle/das.py and oracle/combiners.py) on the
+    64 + 64-element workload: IQ 256 x 64 x 3 x 16 per aperture, grid 32 x 64 x 32, seed 42.  Voxel blocks are
+    spread over one forked worker per host core (NumPy runs each block on one thread); one warm-up run, then
+    the median of `reps` runs.  Runs before the GPU is initialised, so the workers fork a GPU-free process."""
+    from concurrent.futures import ProcessPoolExecutor
+    from multiprocessing import get_context
+
+    from oracle import das, scene
+
+    acq = scene.point_scene()
+    _C1["acq"], _C1["scan"] = acq, das.scan_coords(acq.grid)
+    nv, nt = len(_C1["scan"]), acq.rc.shape[-1]
     cores = len(os.sched_getaffinity(0))
-    return iq, {
-        "value": voxels * scene.frames / seconds,
+    workers = max(1, min(16, cores))
+    edges = np.linspace(0, nv, 4 * workers + 1).astype(int)
+    blocks = list(zip(edges[:-1], edges[1:]))
+    times = []
+    with ProcessPoolExecutor(workers, mp_context=get_context("fork")) as pool:
+        for rep in range(reps + 1):
+            begin = time.perf_counter()
+            power = np.concatenate(list(pool.map(_c1_chunk, blocks)))
+            if rep:
+                times.append(time.perf_counter() - begin)
+    seconds = float(np.median(times))
+    _C1["power"] = power
+    return {
+        "value": nv * nt / seconds,
         "unit": "voxel-frames/s",
-        "cores": 1,
+        "cores": workers,
         "kind": "port",
-        "sample": f"{voxels} random voxels of the same 128x256x128 workload, all 200 frames, float64 NumPy oracle "
-                  f"(DAS single-threaded; host has {cores} cores), {seconds:.1f} s",
+        "sample": f"the whole 64+64-element workload (IQ 256x64x3x16 per aperture, grid 32x64x32, seed 42), "
+                  f"unfiltered OPW + power Doppler in the float64 NumPy restatement of the reference CPU path; "
+                  f"{workers} forked workers on {cores} host cores, NumPy single-threaded per block; one warm-up, "
+                  f"median of {reps} runs = {seconds * 1e3:.0f} ms (runs {', '.join(f'{t * 1e3:.0f}' for t in times)} ms)",
     }
+
+
+def gpu_on_cpu_workload(reps: int = 5) -> dict:
+    """The CPU baseline's workload on the GPU: `ensemble_pd_from_channels` with device-resident channels
+    (pack + grid delay-and-sum + power, allocation included), median of `reps` after a warm-up, and its power
+    against the CPU baseline's float64 power."""
+    import torch
+
+    import rcabeam_amd as rb
+
+    acq = _C1["acq"]
+    geom = rb.RCAGeometry(acq.x_el, acq.y_el, c=acq.c, fs=acq.fs, f_demod=acq.fd, fnumber=acq.fnumber)
+    rc, cr = (torch.from_numpy(x).cuda() for x in (acq.rc, acq.cr))
+    times = []
+    for rep in range(reps + 1):
+        torch.cuda.synchronize()
+        begin = time.perf_counter()
+        pd = rb.ensemble_pd_from_channels(rc, cr, acq.angles, acq.t_start, acq.grid, geom)
+        torch.cuda.synchronize()
+        if rep:
+            times.append(time.perf_counter() - begin)
+    seconds = float(np.median(times))
+    nv, nt = pd.numel(), acq.rc.shape[-1]
+    got = pd.reshape(-1).cpu().numpy().astype(np.float64)
+    expected = _C1["power"]
+    return {"ms": seconds * 1e3, "value": nv * nt / seconds,
+            "rel_l2_power_vs_cpu_baseline": float(np.linalg.norm(got - expected) / np.linalg.norm(expected))}
+
+
+def host_to_host(pipe, reps: int = 3) -> dict:
+    """One float32 step from host channels to a host power volume, the reference's published boundary
+    (examples/_support/rca_benchmark.py:108; ROADMAP.md:45): pageable NumPy RC and CR -> pinned staged upload
+    (rcabeam_upload_host) into the resident channel buffers -> the device step -> float32 power to the host.
+    Best of `reps` after a warm-up."""
+    import torch
+
+    from rcabeam_amd import _device as dev
+    from rcabeam_amd import _lib
+
+    hosts = [pipe.rc.cpu().numpy(), pipe.cr.cpu().numpy()]
+    best = float("inf")
+    for _ in range(reps + 1):
+        torch.cuda.synchronize()
+        begin = time.perf_counter()
+        for h, d in zip(hosts, (pipe.rc, pipe.cr)):
+            _lib.call("rcabeam_upload_host", h.ctypes.data, d.data_ptr(), h.nbytes, dev.upload_threads())
+        pipe.step()
+        power = pipe.power.to(torch.float32).cpu().numpy()
+        best = min(best, time.perf_counter() - begin)
+    nv, nt = pipe.g.nv, pipe.nt
+    return {"ms": best * 1e3, "value": nv * nt / best, "unit": "voxel-frames/s",
+            "upload_bytes": sum(h.nbytes for h in hosts), "output": f"{power.nbytes} bytes float32 power",
+            "threads": dev.upload_threads()}
 
 
 def combiner_timing(scene, rc, cr) -> dict:
@@ -310,6 +408,36 @@ def combiner_timing(scene, rc, cr) -> dict:
     seconds = time.perf_counter() - begin
     out["dmas_hybrid"] = {"ms": seconds * 1e3, "voxel_frames_per_s": nx * nz * ny * frames / seconds,
                           "prepare_ms": prepare * 1e3}
+    return out
+
+
+def combiner_timing_64el(reps: int = 3) -> dict:
+    """Every combiner on the 64 + 64-element point-scatterer acquisition (IQ 256 x 64 x 3 x 16, seed 42)
+    over a 128 x 256 x 128 grid on the same +/-6.4 mm x 5-15 mm extent: one `ensemble_pd_from_channels`
+    call each, channels resident in HBM, median of `reps` after a warm-up."""
+    import torch
+
+    import rcabeam_amd as rb
+    from oracle import scene as oscene
+
+    acq = oscene.point_scene(grid_shape=(128, 256, 128))
+    geom = rb.RCAGeometry(acq.x_el, acq.y_el, c=acq.c, fs=acq.fs, f_demod=acq.fd, fnumber=acq.fnumber)
+    rc, cr = (torch.from_numpy(x).cuda() for x in (acq.rc, acq.cr))
+    nv = 128 * 256 * 128
+    frames = acq.rc.shape[-1]
+    out = {}
+    for method, k in (("opw", 2), ("xdoppler", 2), ("rc_fmas", 2), ("dmas", 2), ("st_sw", 2), ("st_sw", 3)):
+        key = method if method != "st_sw" else f"st_sw_k{k}"
+        times = []
+        for rep in range(reps + 1):
+            torch.cuda.synchronize()
+            begin = time.perf_counter()
+            rb.ensemble_pd_from_channels(rc, cr, acq.angles, acq.t_start, acq.grid, geom, method=method, k=k)
+            torch.cuda.synchronize()
+            if rep:
+                times.append(time.perf_counter() - begin)
+        seconds = float(np.median(times))
+        out[key] = {"ms": seconds * 1e3, "voxel_frames_per_s": nv * frames / seconds}
     return out
 
 
@@ -459,21 +587,80 @@ def adjoint_timing(reps: int = 3) -> dict:
 
 
 def time_pipeline(pipe, warmup: int, steps: int):
-    """W untimed steps, then K timed steps between synchronizations: (ms per step, mean delay-and-sum ms)."""
+    """W untimed steps, then K timed steps between synchronizations: (ms per step, mean delay-and-sum ms,
+    mean ms per stage).  Stage times come from HIP events recorded on the null stream between stages."""
     import torch
 
     for _ in range(warmup):
         pipe.step()
     torch.cuda.synchronize()
     events = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    marks = [[] for _ in range(steps)]
     log(f"bench: timing {steps} steps ({pipe.storage})")
     torch.cuda.synchronize()
     begin = time.perf_counter()
-    for ev in events:
-        pipe.step(ev)
+    for ev, mk in zip(events, marks):
+        pipe.step(ev, mk)
     torch.cuda.synchronize()
     wall = time.perf_counter() - begin
-    return wall * 1e3 / steps, float(np.mean([a.elapsed_time(b) for a, b in events]))
+    stages = {}
+    for mk in marks:
+        for (_, a), (name, b) in zip(mk[:-1], mk[1:]):
+            stages.setdefault(name, []).append(a.elapsed_time(b))
+    stage_ms = {name: float(np.mean(v)) for name, v in stages.items()}
+    return wall * 1e3 / steps, float(np.mean([a.elapsed_time(b) for a, b in events])), stage_ms
+
+
+def stage_roofs(pipe, stage_ms: dict, das_flops: float) -> dict:
+    """Each stage of one step against the roof that binds it: algorithmic bytes or flops per step (inputs
+    read once, outputs written once) over the stage's mean event time."""
+    nv, nt, ntc = pipe.g.nv, pipe.nt, pipe.ntc
+    raw = pipe.ns * pipe.nc * pipe.na * pipe.nt * 8  # complex64 channels of one aperture
+    sample = 8 if pipe.code == 0 else 4  # bytes per IQ sample
+    packed = pipe.ns * pipe.nc * pipe.na * ntc * (8 if pipe.storage == "float32" else 4)
+    hbm = lambda nbytes, ms: {"bound": "hbm", "bytes": nbytes, "achieved_GBps": nbytes / (ms * 1e6),  # noqa: E731
+                              "peak_GBps": HBM_PEAK_GBS, "frac": nbytes / (ms * 1e6) / HBM_PEAK_GBS}
+    out = {}
+    for name, ms in stage_ms.items():
+        entry = {"ms": ms}
+        if name == "channel_basis":
+            rows = pipe.c_rows * 2
+            entry.update(bound="mfma", note="every 4th channel row of both apertures: split-bfloat16 Gram "
+                         "(4 products) + float64 EVD of the 200 x 200 channel Gram",
+                         flops=8.0 * nt * nt * rows * 4)
+        elif name == "pack":
+            entry.update(hbm(2 * (raw + packed), ms))
+            entry["note"] = "both apertures: complex64 channels read once, packed layout written once"
+        elif name == "das":
+            mfma = pipe.storage != "float32"
+            peak = BF16_PEAK_TFLOPS if mfma else FP32_PEAK_TFLOPS
+            entry.update(bound="mfma" if mfma else "valu", flops=das_flops,
+                         achieved_TFLOPs=das_flops / (ms * 1e9), peak_TFLOPs=peak,
+                         frac=das_flops / (ms * 1e9) / peak,
+                         note="14 flop per active tap and frame (algorithmic, not the instructions issued)")
+        elif name == "expand":
+            entry.update(hbm(nv * ntc * 8 + nv * nt * 8, ms))
+            entry["note"] = "coded IQ read once, complex64 IQ written once"
+        elif name == "gram":
+            flops = 8.0 * nt * nt * nv
+            if pipe.exact_gram:
+                peak, form = FP64_PEAK_TFLOPS, "exact float64 MFMA (complex64 products exact in float64)"
+            elif pipe.code == 0:
+                peak, form = BF16_PEAK_TFLOPS, "split bfloat16 MFMA, 4 products per complex64 product"
+                flops *= 4
+            else:
+                peak, form = BF16_PEAK_TFLOPS, "one exact 16-bit MFMA product"
+            entry.update(bound="mfma", flops=flops, achieved_TFLOPs=flops / (ms * 1e9), peak_TFLOPs=peak,
+                         frac=flops / (ms * 1e9) / peak, bytes=nv * nt * sample,
+                         note=f"8 T flop per voxel-frame (full complex Gram); {form}; includes the |max| pass")
+        elif name == "evd":
+            entry.update(bound="latency", note="200 x 200 complex128 Hermitian EVD (rocSOLVER zheevd) and "
+                         "the T x T Hermitian assembly")
+        elif name == "project":
+            entry.update(hbm(nv * nt * sample + nv * 8, ms))
+            entry["note"] = "IQ read once, float64 filtered power written once (3 discarded vectors)"
+        out[name] = entry
+    return out
 
 
 def traffic_from_profiles(precision: str):
@@ -489,14 +676,29 @@ def traffic_from_profiles(precision: str):
         return None
 
 
+def sampled_oracle(scene, rc, cr, pick):
+    """Float64 oracle OPW IQ at the voxels `pick` of the full-size workload (parity lines of the record)."""
+    from oracle import das
+
+    scan = das.scan_coords(scene.grid)
+    g = scene.geom
+    return das.opw_iq(rc.cpu().numpy(), cr.cpu().numpy(), scan[pick], g.x_el, g.y_el, scene.angles, scene.t_start,
+                      g.c, g.fs, g.f_demod, g.fnumber)
+
+
 def main() -> int:
     args = parse()
     if args.copy_calibration:
         return copy_calibration()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); see the sharding note in DESIGN.md",
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); multi-GPU runs are not part of "
+              "this build (the path would shard by voxel slabs, with one all-reduce of the T x T Gram)",
               file=sys.stderr)
         return 2
+    baseline = None
+    if not (args.no_cpu_baseline or args.two_stage_only or args.combiners_only):
+        log("bench: CPU baseline (float64 NumPy restatement, 64+64-element workload)")
+        baseline = cpu_baseline()
     import torch
 
     from rcabeam_amd import _device as dev
@@ -525,9 +727,15 @@ def main() -> int:
     taps = int(counter.item())
     das_flops = FLOPS_PER_TAP * taps * nt
     log(f"bench: {taps} active taps ({taps / nv:.1f} per voxel), warmup {args.warmup}")
-    ms_per_step, das_ms = time_pipeline(pipe, args.warmup, args.steps)
+    ms_per_step, das_ms, stage_ms = time_pipeline(pipe, args.warmup, args.steps)
     value = nv * nt / (ms_per_step / 1e3)
     achieved = das_flops / (das_ms / 1e3) / 1e12
+    mfma_das = storage != "float32"
+    peak = BF16_PEAK_TFLOPS if mfma_das else FP32_PEAK_TFLOPS
+    # Clutter cut: singular values sigma_k = sqrt(lambda_k) of the Gram, descending; the retained interval
+    # starts at rank_start, so sigma_{rank_start} / sigma_{rank_start + 1} (one-based) is the margin of the cut.
+    lam = pipe.values.cpu().numpy()[::-1]
+    sigma = np.sqrt(np.maximum(lam, 0))
     record = {
         "metric": "voxel-frames/s OPW+SVD+PD",
         "value": value,
@@ -551,24 +759,26 @@ def main() -> int:
             "active_taps_per_voxel": taps / nv,
         },
         "roofline": {
-            "bound": "mfma",
-            "engine": "fp32 VALU (the float32 delay-and-sum; MI355X fp32 vector peak as the roof, of which the "
-                      "build's non-packed FP32 reaches half)",
+            "bound": "mfma" if mfma_das else "valu",
+            "engine": ("dense fp16/bf16 MFMA" if mfma_das else
+                       "fp32 VALU (MI355X fp32 vector peak; the build disables packed FP32, so half of it is reachable)"),
             "kernel": DAS_KERNEL[storage],
             "achieved": achieved,
-            "peak": FP32_PEAK_TFLOPS,
+            "peak": peak,
             "unit": "TFLOP/s",
-            "frac": achieved / FP32_PEAK_TFLOPS,
+            "frac": achieved / peak,
             "traffic": traffic_from_profiles(storage),
         },
+        "stages": stage_roofs(pipe, stage_ms, das_flops),
+        "clutter_cut": {"rank_start": rank_start, "sigma_1_to_6": sigma[:6].tolist(),
+                        "gap_sigma3_over_sigma4": float(sigma[rank_start - 1] / sigma[rank_start])},
     }
-    # Voxels at which every form's IQ is compared with the float64 oracle (the CPU baseline's sample).
+    # Voxels at which every form's IQ is compared with the float64 oracle.
     pick = np.sort(np.random.default_rng(0).choice(nv, size=args.cpu_voxels, replace=False))
     pick_d = torch.from_numpy(pick).cuda()
     sampled = {"float32": pipe.signal[pick_d].clone() if pipe.code == 0 else None}
     if storage == "float32" and not args.no_mixed:
-        # Mixed-precision forms on the same scene, with their full-size difference from the float32 pipeline
-        # (whose own error against the float64 oracle is ~1e-5 on filtered power; tests/test_mixed_precision_gpu.py).
+        # Mixed-precision forms on the same scene, with their full-size difference from the float32 pipeline.
         reference = {"iq": pipe.signal, "pd": pipe.pd[:, 0].clone(), "power": pipe.power.clone()}
 
         def rel(a, b):
@@ -588,7 +798,7 @@ def main() -> int:
         ):
             log(f"bench: mixed form {form}")
             mixed = Pipeline(scene, rc, cr, form, rank_start, args.lowrank_modes)
-            m_ms, m_das = time_pipeline(mixed, args.warmup, args.steps)
+            m_ms, m_das, m_stages = time_pipeline(mixed, args.warmup, args.steps)
             iq = mixed.signal if mixed.code == 0 else mixed.signal.view(torch.float16).to(torch.float32).view(
                 torch.complex64)
             record[key] = {
@@ -600,24 +810,32 @@ def main() -> int:
                 "speedup_vs_float32": ms_per_step / m_ms,
                 "rel_l2_vs_float32": {"iq": rel(iq, reference["iq"]), "power": rel(mixed.pd[:, 0], reference["pd"]),
                                       "filtered_power": rel(mixed.power, reference["power"])},
+                "stages_ms": m_stages,
             }
             sampled[key] = iq[pick_d].clone()
             del mixed, iq
             torch.cuda.empty_cache()
         del reference
+    log("bench: host to host (staged upload + step + power download)")
+    record["host_to_host"] = host_to_host(pipe)
     log("bench: adjoint / forward on the differentiable-OPW workload")
     record["adjoint"] = adjoint_timing()
     if not args.no_combiners:
         log("bench: combiners at full size")
         record["combiners"] = combiner_timing(scene, rc, cr)
+        log("bench: combiners on the 64+64-element acquisition at 128x256x128")
+        record["combiners_64el"] = combiner_timing_64el()
         log("bench: host ingest")
         record["host_ingest"] = ingest_timing(rc)
     if not args.no_two_stage:
         log("bench: two-stage approximation at full size")
         record["two_stage"] = two_stage_timing(scene, rc, cr)
-    if not args.no_cpu_baseline:
-        log("bench: CPU baseline (oracle) on a voxel sample")
-        oracle_iq, record["cpu_baseline"] = cpu_baseline(scene, rc, cr, rank_start, pick)
+    if baseline is not None:
+        baseline["gpu_same_workload"] = gpu_on_cpu_workload()
+        record["cpu_baseline"] = baseline
+    if args.cpu_voxels:
+        log(f"bench: float64 oracle at {args.cpu_voxels} sampled voxels")
+        oracle_iq = sampled_oracle(scene, rc, cr, pick)
         # Each form's IQ and unfiltered power at the sampled voxels against the float64 oracle.
         for key, iq in sampled.items():
             if iq is None:
