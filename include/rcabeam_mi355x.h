@@ -188,6 +188,7 @@ size_t rcabeam_gram_f64_workspace_bytes(size_t rows, size_t frames);
 rcabeam_status rcabeam_temporal_gram_f64_device(const void* iq, size_t rows, size_t frames, double* real_gram,
                                                 double* workspace, size_t workspace_bytes, int accumulate);
 
+
 /* Complex Hermitian Gram G = scale2 * X^H X (complex128, column-major
  * frames x frames) from the real Gram above (reference gpu_clutter.py:160-161). */
 rcabeam_status rcabeam_gram_hermitian_device(const double* real_gram, size_t frames, double scale2, void* gram);

@@ -217,35 +217,79 @@ __global__ void k_gram_form(const double* __restrict__ c, int nb, int frames, do
 // ---- exact Gram: float64 MFMA ----------------------------------------------
 // complex64 samples widen exactly to float64; v_mfma_f64_16x16x4_f64 forms the
 // products exactly and accumulates in float64, so C equals the complex128
-// reference Gram up to float64 summation order.  Blocks are 16x16; a pass
-// covers up to kG64Waves * kG64MaxBlocks upper-triangular blocks and the
-// row chunk staged in LDS is shared by every block of the pass.
+// reference Gram up to float64 summation order.  C is tiled into 16x16 blocks
+// and its upper-triangular blocks (row-major order) are cut into sets of up to
+// kG64BlocksPerSet; rows are cut into groups.  One launch covers every (group,
+// set) pair: workgroup (g, s) stages the rows of group g in LDS 16 at a time
+// and forms the blocks of set s.  The sets of one group are launched next to
+// each other on the same XCD (workgroup w runs on XCD w % 8), so a row chunk
+// is fetched from HBM once and served to the other sets from that XCD's L2.
+// Each thread prefetches its share of the next 16 rows into registers while
+// the waves run the MFMAs of the current ones.  Partials are [group][block]
+// [256] float64 and are summed over groups in a fixed order (deterministic).
 typedef double doublex4 __attribute__((ext_vector_type(4)));
 constexpr int kG64Threads = 1024;
 constexpr int kG64Waves = kG64Threads / kWave;
 constexpr int kG64MaxBlocks = 7;
-constexpr int kG64BlocksPerPass = kG64Waves * kG64MaxBlocks;  // 112
-constexpr int kG64Rows = 16;                                  // rows staged per step (4 MFMA K-steps)
-constexpr int kG64RowsPerGroup = 8192;
+constexpr int kG64BlocksPerSet = kG64Waves * kG64MaxBlocks;  // 112
+constexpr int kG64Rows = 16;                                 // rows staged per step (4 MFMA K-steps)
+constexpr int kG64MaxFrames = 640;
+constexpr int kG64Prefetch = (kG64Rows * kG64MaxFrames + kG64Threads - 1) / kG64Threads;  // float2 per thread
+constexpr long long kG64RowsPerGroup = 16384;                // default grouping (multiple of kG64Rows)
 
-struct Gram64Plan {
-  int nb;  // 16-column blocks of the real Gram (2T padded to 16)
-  int wave_start[kG64Waves];
-  int wave_count[kG64Waves];
-  unsigned char bi[kG64BlocksPerPass];
-  unsigned char bj[kG64BlocksPerPass];
-  int pass_blocks;
+struct Gram64Shape {
+  int nb;                   // 16-column blocks of the real Gram (2T padded to 16)
+  int total;                // upper-triangular blocks
+  int sets;                 // ceil(total / kG64BlocksPerSet)
+  int groups;               // row groups
+  long long rows_per_group; // multiple of kG64Rows
 };
 
+// (I, J) of upper-triangular block `idx` in row-major order of an nb x nb block grid.
+__device__ __forceinline__ int2 triangle_block(int idx, int nb) {
+  int i = 0;
+  while (i < nb - 1 && idx >= nb - i) {
+    idx -= nb - i;
+    ++i;
+  }
+  return make_int2(i, i + idx);
+}
+
 __global__ __launch_bounds__(kG64Threads) void k_gram64(const float2* __restrict__ x, long long rows, int frames,
-                                                       Gram64Plan plan, double* __restrict__ partials) {
+                                                       Gram64Shape shape, double* __restrict__ partials) {
   extern __shared__ __align__(16) double stage[];  // [kG64Rows][ld]
-  const int ld = plan.nb * 16;
+  // Workgroup -> (group, set): the sets of 8 consecutive groups occupy 8 * sets consecutive workgroups,
+  // set-major, so group g's sets all land on XCD g % 8 at about the same time.
+  const int per_chunk = 8 * shape.sets;
+  const int chunk = static_cast<int>(blockIdx.x) / per_chunk, rem = static_cast<int>(blockIdx.x) % per_chunk;
+  const int set = rem / 8;
+  const int group = chunk * 8 + rem % 8;
+  if (group >= shape.groups) return;
+  const int ld = shape.nb * 16;
   const int lane = threadIdx.x & (kWave - 1);
-  const int wave = threadIdx.x / kWave;
-  const long long row0 = static_cast<long long>(blockIdx.x) * kG64RowsPerGroup;
-  const long long row_end = row0 + kG64RowsPerGroup < rows ? row0 + kG64RowsPerGroup : rows;
+  const int wave = uniform(static_cast<int>(threadIdx.x) / kWave);
+  const long long row0 = static_cast<long long>(group) * shape.rows_per_group;
+  const long long row_end = row0 + shape.rows_per_group < rows ? row0 + shape.rows_per_group : rows;
   const int steps = static_cast<int>((row_end - row0 + kG64Rows - 1) / kG64Rows);
+  // This wave's run of the set's blocks.
+  const int first = set * kG64BlocksPerSet;
+  const int count = shape.total - first < kG64BlocksPerSet ? shape.total - first : kG64BlocksPerSet;
+  const int base = count / kG64Waves, extra = count % kG64Waves;
+  const int my_count = base + (wave < extra ? 1 : 0);
+  const int my_start = first + wave * base + (wave < extra ? wave : extra);
+  int bi[kG64MaxBlocks], bj[kG64MaxBlocks];
+  {
+    int2 ij = triangle_block(my_start, shape.nb);
+#pragma unroll
+    for (int b = 0; b < kG64MaxBlocks; ++b) {
+      bi[b] = ij.x;
+      bj[b] = ij.y;
+      if (++ij.y == shape.nb) {
+        ++ij.x;
+        ij.y = ij.x;
+      }
+    }
+  }
   for (int idx = threadIdx.x; idx < kG64Rows * (ld - 2 * frames); idx += blockDim.x) {
     const int k = idx / (ld - 2 * frames), c = idx % (ld - 2 * frames);
     stage[k * ld + 2 * frames + c] = 0.0;
@@ -253,19 +297,34 @@ __global__ __launch_bounds__(kG64Threads) void k_gram64(const float2* __restrict
   doublex4 acc[kG64MaxBlocks];
 #pragma unroll
   for (int b = 0; b < kG64MaxBlocks; ++b) acc[b] = doublex4{0.0, 0.0, 0.0, 0.0};
-  const int my_start = plan.wave_start[wave];
-  const int my_count = plan.wave_count[wave];
   const int per_step = kG64Rows * frames;
-  for (int s = 0; s < steps; ++s) {
+  // Prefetched share of the next step: element idx = u * kG64Threads + tid of the step's 16 x frames samples.
+  float2 next[kG64Prefetch];
+  auto fetch = [&](int s) {
     const long long rbase = row0 + static_cast<long long>(s) * kG64Rows;
+#pragma unroll
+    for (int u = 0; u < kG64Prefetch; ++u) {
+      const int idx = u * kG64Threads + static_cast<int>(threadIdx.x);
+      next[u] = make_float2(0.f, 0.f);
+      if (idx < per_step) {
+        const int k = idx / frames;
+        if (rbase + k < row_end) next[u] = x[rbase * frames + idx];
+      }
+    }
+  };
+  if (steps > 0) fetch(0);
+  for (int s = 0; s < steps; ++s) {
     __syncthreads();
-    for (int idx = threadIdx.x; idx < per_step; idx += blockDim.x) {
-      const int k = idx / frames, t = idx - k * frames;
-      const long long r = rbase + k;
-      const float2 v = r < row_end ? x[r * frames + t] : make_float2(0.f, 0.f);
-      *reinterpret_cast<double2*>(stage + k * ld + 2 * t) = make_double2(v.x, v.y);
+#pragma unroll
+    for (int u = 0; u < kG64Prefetch; ++u) {
+      const int idx = u * kG64Threads + static_cast<int>(threadIdx.x);
+      if (idx < per_step) {
+        const int k = idx / frames, t = idx - k * frames;
+        *reinterpret_cast<double2*>(stage + k * ld + 2 * t) = make_double2(next[u].x, next[u].y);
+      }
     }
     __syncthreads();
+    if (s + 1 < steps) fetch(s + 1);
 #pragma unroll
     for (int kk = 0; kk < kG64Rows; kk += 4) {
       const double* rowp = stage + (kk + lane / 16) * ld + (lane & 15);
@@ -274,39 +333,39 @@ __global__ __launch_bounds__(kG64Threads) void k_gram64(const float2* __restrict
 #pragma unroll
       for (int b = 0; b < kG64MaxBlocks; ++b) {
         if (b < my_count) {
-          const int I = plan.bi[my_start + b], J = plan.bj[my_start + b];
-          if (I != prev) {
-            a = rowp[16 * I];
-            prev = I;
+          if (bi[b] != prev) {
+            a = rowp[16 * bi[b]];
+            prev = bi[b];
           }
-          acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, rowp[16 * J], acc[b], 0, 0, 0);
+          acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, rowp[16 * bj[b]], acc[b], 0, 0, 0);
         }
       }
     }
   }
-  double* out = partials + static_cast<long long>(blockIdx.x) * plan.pass_blocks * 256;
+  double* out = partials + (static_cast<long long>(group) * shape.total + my_start) * 256;
 #pragma unroll
   for (int b = 0; b < kG64MaxBlocks; ++b)
     if (b < my_count) {
-      double* dst = out + static_cast<long long>(my_start + b) * 256 + lane * 4;
+      double* dst = out + static_cast<long long>(b) * 256 + lane * 4;
 #pragma unroll
       for (int e = 0; e < 4; ++e) dst[e] = acc[b][e];
     }
 }
 
-// Sum float64 partials into the real Gram C (16x16 blocks, ldc = 32 * ceil(2T / 32)).
-__global__ void k_gram64_reduce(const double* __restrict__ partials, int groups, Gram64Plan plan, int ldc,
-                                double* __restrict__ c, int accumulate) {
+// Sum float64 partials over groups (in group order) into the real Gram C (ldc = 32 * ceil(2T / 32)).
+__global__ void k_gram64_reduce(const double* __restrict__ partials, Gram64Shape shape, int ldc, double* __restrict__ c,
+                                int accumulate) {
   const long long idx = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (idx >= static_cast<long long>(plan.pass_blocks) * 256) return;
+  if (idx >= static_cast<long long>(shape.total) * 256) return;
   const int b = static_cast<int>(idx / 256), e = static_cast<int>(idx % 256);
   double sum = 0.0;
-  for (int g = 0; g < groups; ++g) sum += partials[(static_cast<long long>(g) * plan.pass_blocks + b) * 256 + e];
+  for (int g = 0; g < shape.groups; ++g) sum += partials[(static_cast<long long>(g) * shape.total + b) * 256 + e];
   const int l = e / 4, r = e % 4;
+  const int2 ij = triangle_block(b, shape.nb);
   // 16x16x4 float64 MFMA accumulator layout (measured on gfx950): lane l,
   // register r -> (row l/16 + 4 r, column l%16).
-  const int i = 16 * plan.bi[b] + (l / 16) + 4 * r;
-  const int j = 16 * plan.bj[b] + (l % 16);
+  const int i = 16 * ij.x + (l / 16) + 4 * r;
+  const int j = 16 * ij.y + (l % 16);
   double* dst = c + static_cast<long long>(i) * ldc + j;
   *dst = accumulate ? *dst + sum : sum;
 }
@@ -565,33 +624,44 @@ static hipError_t gram_launch(const void* x, long long rows, long long row_step,
   return hipSuccess;
 }
 
-static void make_plan64(int nb, int pass, Gram64Plan& plan) {
-  plan.nb = nb;
-  const long long total = triangle_blocks(nb);
-  const long long first = static_cast<long long>(pass) * kG64BlocksPerPass;
-  const int count = static_cast<int>(total - first < kG64BlocksPerPass ? total - first : kG64BlocksPerPass);
-  plan.pass_blocks = count;
-  long long idx = 0;
-  int filled = 0;
-  for (int I = 0; I < nb; ++I)
-    for (int J = I; J < nb; ++J, ++idx)
-      if (idx >= first && idx < first + count) {
-        plan.bi[filled] = static_cast<unsigned char>(I);
-        plan.bj[filled] = static_cast<unsigned char>(J);
-        ++filled;
-      }
-  const int base = count / kG64Waves, extra = count % kG64Waves;
-  int start = 0;
-  for (int w = 0; w < kG64Waves; ++w) {
-    const int n = base + (w < extra ? 1 : 0);
-    plan.wave_start[w] = start;
-    plan.wave_count[w] = n;
-    start += n;
-  }
-}
-
 static int column_blocks16(size_t frames) { return static_cast<int>((2 * frames + 15) / 16); }
 static size_t lds64_bytes(size_t frames) { return static_cast<size_t>(kG64Rows) * 16 * column_blocks16(frames) * sizeof(double); }
+
+// Shape of the exact Gram for `rows` x `frames` with groups of `rows_per_group` rows.
+static Gram64Shape gram64_shape(size_t rows, size_t frames, long long rows_per_group) {
+  Gram64Shape sh;
+  sh.nb = column_blocks16(frames);
+  sh.total = static_cast<int>(triangle_blocks(sh.nb));
+  sh.sets = (sh.total + kG64BlocksPerSet - 1) / kG64BlocksPerSet;
+  sh.rows_per_group = rows_per_group;
+  sh.groups = static_cast<int>((static_cast<long long>(rows) + rows_per_group - 1) / rows_per_group);
+  return sh;
+}
+
+static size_t gram64_partial_bytes(const Gram64Shape& sh) {
+  return static_cast<size_t>(sh.groups) * sh.total * 256 * sizeof(double);
+}
+
+static bool gram64_frames_ok(size_t frames) { return frames >= 1 && frames <= kG64MaxFrames; }
+
+// One launch over all (group, set) pairs, then the fixed-order reduction into C.
+static hipError_t gram64_launch(const float2* iq, size_t rows, size_t frames, const Gram64Shape& sh,
+                                double* partials, double* real_gram, int accumulate) {
+  const size_t lds = lds64_bytes(frames);
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_gram64),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+  if (e != hipSuccess) return e;
+  const long long chunks = (sh.groups + 7) / 8;
+  const long long grid = chunks * 8 * sh.sets;
+  if (grid > 0x7fffffffLL) return hipErrorInvalidValue;
+  k_gram64<<<dim3(static_cast<unsigned>(grid)), kG64Threads, lds>>>(iq, static_cast<long long>(rows),
+                                                                    static_cast<int>(frames), sh, partials);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  const long long n = static_cast<long long>(sh.total) * 256;
+  const int ldc = 32 * column_blocks(frames);
+  k_gram64_reduce<<<dim3(static_cast<unsigned>((n + 255) / 256)), 256>>>(partials, sh, ldc, real_gram, accumulate);
+  return hipGetLastError();
+}
 
 struct SolverState {
   std::mutex lock;
@@ -768,12 +838,8 @@ extern "C" rcabeam_status rcabeam_temporal_filter_device(const void* iq, int sto
 }
 
 extern "C" size_t rcabeam_gram_f64_workspace_bytes(size_t rows, size_t frames) {
-  if (!rows || !frames || lds64_bytes(frames) > 160 * 1024) return 0;
-  const long long nb = column_blocks16(frames);
-  const long long groups = (static_cast<long long>(rows) + kG64RowsPerGroup - 1) / kG64RowsPerGroup;
-  const long long per_pass = triangle_blocks(static_cast<int>(nb)) < kG64BlocksPerPass ? triangle_blocks(static_cast<int>(nb))
-                                                                                       : kG64BlocksPerPass;
-  return static_cast<size_t>(groups * per_pass * 256 * sizeof(double));
+  if (!rows || !gram64_frames_ok(frames)) return 0;
+  return gram64_partial_bytes(gram64_shape(rows, frames, kG64RowsPerGroup));
 }
 
 extern "C" rcabeam_status rcabeam_temporal_gram_f64_device(const void* iq, size_t rows, size_t frames,
@@ -782,25 +848,8 @@ extern "C" rcabeam_status rcabeam_temporal_gram_f64_device(const void* iq, size_
   if (!iq || !real_gram || !workspace || !rows || !frames) return RCABEAM_ERROR_ARGUMENT;
   const size_t need = rcabeam_gram_f64_workspace_bytes(rows, frames);
   if (!need || workspace_bytes < need) return RCABEAM_ERROR_ARGUMENT;
-  const int nb = column_blocks16(frames);
-  const long long groups = (static_cast<long long>(rows) + kG64RowsPerGroup - 1) / kG64RowsPerGroup;
-  const int passes = static_cast<int>((triangle_blocks(nb) + kG64BlocksPerPass - 1) / kG64BlocksPerPass);
-  const size_t lds = lds64_bytes(frames);
-  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_gram64),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
-  if (e != hipSuccess) return RCABEAM_ERROR_CUDA;
-  const int ldc = 32 * column_blocks(frames);
-  for (int pass = 0; pass < passes; ++pass) {
-    Gram64Plan plan;
-    make_plan64(nb, pass, plan);
-    k_gram64<<<dim3(static_cast<unsigned>(groups)), kG64Threads, lds>>>(static_cast<const float2*>(iq),
-                                                                         static_cast<long long>(rows),
-                                                                         static_cast<int>(frames), plan, workspace);
-    if (hipGetLastError() != hipSuccess) return RCABEAM_ERROR_CUDA;
-    const long long n = static_cast<long long>(plan.pass_blocks) * 256;
-    k_gram64_reduce<<<dim3(static_cast<unsigned>((n + 255) / 256)), 256>>>(workspace, static_cast<int>(groups), plan,
-                                                                            ldc, real_gram, accumulate);
-    if (hipGetLastError() != hipSuccess) return RCABEAM_ERROR_CUDA;
-  }
-  return RCABEAM_SUCCESS;
+  const Gram64Shape sh = gram64_shape(rows, frames, kG64RowsPerGroup);
+  return gram64_launch(static_cast<const float2*>(iq), rows, frames, sh, workspace, real_gram, accumulate) == hipSuccess
+             ? RCABEAM_SUCCESS
+             : RCABEAM_ERROR_CUDA;
 }
