@@ -188,6 +188,21 @@ size_t rcabeam_gram_f64_workspace_bytes(size_t rows, size_t frames);
 rcabeam_status rcabeam_temporal_gram_f64_device(const void* iq, size_t rows, size_t frames, double* real_gram,
                                                 double* workspace, size_t workspace_bytes, int accumulate);
 
+/* The same exact Gram without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The float64 partial sums live in a library-owned device array of
+ * RCABEAM_GRAM_F64_SCRATCH_BYTES on each GPU (held until the process ends)
+ * instead of a caller workspace; the call allocates nothing and frees nothing.
+ * Same result as rcabeam_temporal_gram_f64_device, bit for bit, while the
+ * default row groups fit the array (rows <= 16384 * floor(scratch / (blocks *
+ * 2 KiB)), 1,638,400 rows at frames = 200); larger inputs use larger row
+ * groups (float64 summation order differs).  frames > 640 returns
+ * RCABEAM_ERROR_ARGUMENT.  Calls that overlap on one GPU share the array and
+ * race; serialize them. */
+#define RCABEAM_GRAM_F64_SCRATCH_BYTES ((size_t)64 << 20)
+rcabeam_status rcabeam_temporal_gram_f64_noalloc_device(const void* iq, size_t rows, size_t frames,
+                                                        double* real_gram, int accumulate);
 
 /* Complex Hermitian Gram G = scale2 * X^H X (complex128, column-major
  * frames x frames) from the real Gram above (reference gpu_clutter.py:160-161). */

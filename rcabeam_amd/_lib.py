@@ -63,6 +63,7 @@ _SIGNATURES: dict[str, list] = {
                                              c_void_p, c_size_t, c_int],
     "rcabeam_gram_f64_workspace_bytes": [c_size_t, c_size_t],
     "rcabeam_temporal_gram_f64_device": [c_void_p, c_size_t, c_size_t, c_void_p, c_void_p, c_size_t, c_int],
+    "rcabeam_temporal_gram_f64_noalloc_device": [c_void_p, c_size_t, c_size_t, c_void_p, c_int],
     "rcabeam_simulate_lds_bytes": [c_size_t, c_int],
     "rcabeam_simulate_channels_device": [c_void_p, c_void_p, c_void_p, c_size_t, c_void_p, c_int, c_void_p, c_void_p,
                                          c_size_t, c_void_p, c_void_p, c_void_p] + [c_size_t] * 4

@@ -837,6 +837,39 @@ extern "C" rcabeam_status rcabeam_temporal_filter_device(const void* iq, int sto
   return project<true>(iq, storage, rows, frames, vectors, r, subtract, nullptr, out);
 }
 
+// ---- exact Gram without allocation -----------------------------------------
+// The partial sums live in this file-scope device array instead of a caller
+// workspace (each GPU the library runs on holds its own copy until the process
+// ends).  Host code never passes the array's name to a kernel: its device
+// address on the current GPU comes from hipGetSymbolAddress, looked up once per
+// device on the entry point's first call there.  Every partial a call reads is
+// written earlier in the same call, so nothing needs clearing; calls that
+// overlap on one GPU share the array and race.
+constexpr size_t kGramScratchBytes = RCABEAM_GRAM_F64_SCRATCH_BYTES;
+static __device__ __align__(256) double scratch[kGramScratchBytes / sizeof(double)];
+
+struct ScratchState {
+  std::mutex lock;
+  double* ptr[64] = {};
+};
+static ScratchState g_scratch;
+
+static hipError_t scratch_pointer(double** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= 64) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> guard(g_scratch.lock);
+  if (!g_scratch.ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch.ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch.ptr[dev];
+  return hipSuccess;
+}
+
 extern "C" size_t rcabeam_gram_f64_workspace_bytes(size_t rows, size_t frames) {
   if (!rows || !gram64_frames_ok(frames)) return 0;
   return gram64_partial_bytes(gram64_shape(rows, frames, kG64RowsPerGroup));
@@ -850,6 +883,27 @@ extern "C" rcabeam_status rcabeam_temporal_gram_f64_device(const void* iq, size_
   if (!need || workspace_bytes < need) return RCABEAM_ERROR_ARGUMENT;
   const Gram64Shape sh = gram64_shape(rows, frames, kG64RowsPerGroup);
   return gram64_launch(static_cast<const float2*>(iq), rows, frames, sh, workspace, real_gram, accumulate) == hipSuccess
+             ? RCABEAM_SUCCESS
+             : RCABEAM_ERROR_CUDA;
+}
+
+extern "C" rcabeam_status rcabeam_temporal_gram_f64_noalloc_device(const void* iq, size_t rows, size_t frames,
+                                                                   double* real_gram, int accumulate) {
+  if (!iq || !real_gram || !rows || !gram64_frames_ok(frames)) return RCABEAM_ERROR_ARGUMENT;
+  Gram64Shape sh = gram64_shape(rows, frames, kG64RowsPerGroup);
+  const size_t per_group = static_cast<size_t>(sh.total) * 256 * sizeof(double);
+  const long long max_groups = static_cast<long long>(kGramScratchBytes / per_group);
+  if (max_groups < 1) return RCABEAM_ERROR_ARGUMENT;
+  if (sh.groups > max_groups) {
+    // Larger row groups so that the partials fit (float64 summation order then differs from the
+    // workspace entry point's by regrouping only).
+    long long per = (static_cast<long long>(rows) + max_groups - 1) / max_groups;
+    per = (per + kG64Rows - 1) / kG64Rows * kG64Rows;
+    sh = gram64_shape(rows, frames, per);
+  }
+  double* partials = nullptr;
+  if (scratch_pointer(&partials) != hipSuccess) return RCABEAM_ERROR_CUDA;
+  return gram64_launch(static_cast<const float2*>(iq), rows, frames, sh, partials, real_gram, accumulate) == hipSuccess
              ? RCABEAM_SUCCESS
              : RCABEAM_ERROR_CUDA;
 }
