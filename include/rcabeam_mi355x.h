@@ -21,6 +21,8 @@ typedef enum rcabeam_storage {
     RCABEAM_STORAGE_C64 = 0,  /* complex64: two float32 */
     RCABEAM_STORAGE_FP16 = 1, /* two IEEE binary16 (real, imaginary) */
     RCABEAM_STORAGE_BF16 = 2, /* two bfloat16, real in the low half-word */
+    RCABEAM_STORAGE_FP16_SPLIT = 3, /* packed channels only: complex64 as two FP16 pairs (hi, lo), 8 bytes,
+                                       see rcabeam_pack_ensemble_split_device */
 } rcabeam_storage;
 
 /* Pack complex64 channels to complex BF16, same layout conversion and range
@@ -73,6 +75,33 @@ rcabeam_status rcabeam_opw_grid_device(const void* rc, const void* cr, int chann
                                        const float* starts, float* pd, void* signal, int signal_storage, int method,
                                        size_t ns, size_t nc, size_t na, size_t nt, float c, float fs, float fd,
                                        float fn);
+
+/* Float32-accurate channels for the MFMA delay-and-sum: complex64 (samples,
+ * channels, angles, frames) packed to (channels, angles, samples, frames) as
+ * word pairs (hi, lo) of FP16 complex values, hi = fp16(x 2^e) and
+ * lo = fp16(x 2^e - hi), 8 bytes per sample (as complex64).  One power-of-two
+ * scale 2^e per ensemble: absmax points at out[0] of
+ * rcabeam_iq_absmax_device over both apertures (the joint max |component| m),
+ * and e puts m 2^e in [2^14, 2^15) (e = 0 when m is 0 or not finite).
+ * Components within 2^17 of m keep 22 significant bits; the rest an absolute
+ * error below 2^-40 m.  No reference counterpart (it replaces complex64
+ * packing, ensemble.cu:119-128, for the MFMA path). */
+rcabeam_status rcabeam_pack_ensemble_split_device(const void* source, void* target, const unsigned* absmax,
+                                                  size_t ns, size_t nc, size_t na, size_t nt);
+
+/* OPW on a Cartesian grid from split FP16 channels (both apertures packed with
+ * the same absmax) on MFMA: hi and lo coefficients times hi and lo samples
+ * (three exact FP16 products per term, float32 accumulation; the dropped
+ * lo x lo product is below 2^-22 of the term), then the scale 2^-e undone
+ * exactly.  Float32-accurate: the same IQ and power as the complex64 grid
+ * kernel up to float32 rounding.  Arguments as rcabeam_opw_grid_device
+ * (method 0 only; nt <= 224, na <= 16, 8-byte aligned channels). */
+rcabeam_status rcabeam_opw_grid_split_device(const void* rc, const void* cr, const unsigned* absmax, const float* x,
+                                             const float* z, const float* y, size_t nx, size_t nz, size_t ny,
+                                             const float* xe, const float* ye, const float* angles,
+                                             const float* starts, float* pd, void* signal, int signal_storage,
+                                             size_t ns, size_t nc, size_t na, size_t nt, float c, float fs, float fd,
+                                             float fn);
 
 /* Inverse of rcabeam_pack_ensemble_device: complex64 (channels, angles,
  * samples, frames) back to (samples, channels, angles, frames).  Used to
@@ -227,6 +256,55 @@ rcabeam_status rcabeam_filtered_power_device(const void* iq, int storage, size_t
  * (reference gpu_clutter.py:212-241). */
 rcabeam_status rcabeam_temporal_filter_device(const void* iq, int storage, size_t rows, size_t frames,
                                               const void* vectors, size_t r, int subtract, void* out);
+
+/* ---- direct temporal SVD and complex128 filtering (reference
+ * src/rcabeam/gpu_clutter.py:43-109, 276-336; src/rcabeam/clutter.py:18-44,
+ * 74-109) -------------------------------------------------------------------
+ * Matrices are column-major; complex128 flags select complex128 (1) or
+ * complex64 (0) elements.  The rocBLAS / rocSOLVER handle is the one
+ * rcabeam_eigh_device creates per device. */
+
+/* Rows [offset, offset + rows) of a column-major matrix (leading dimension
+ * ld) from a row-major (rows, frames) Casorati array (complex64 or
+ * complex128), converting the element type; stacks RC/CR arrays by offset. */
+rcabeam_status rcabeam_casorati_colmajor_device(const void* iq, int complex128_in, size_t rows, size_t frames,
+                                                void* matrix, int complex128_out, size_t ld, size_t offset);
+
+/* SVD of the column-major rows x frames matrix (ld = rows) with rocSOLVER
+ * gesvd: singular values descending in values (float64 for complex128, float32
+ * for complex64; min(rows, frames) of them), V^H in vh (frames x frames,
+ * column-major).  want_u != 0 overwrites the matrix with U (its first
+ * min(rows, frames) columns), otherwise the matrix is destroyed.  e_work:
+ * frames reals; info: one device int (0 = converged).  reference
+ * gpu_clutter.py:84-103 (cuSOLVER gesvd, jobu 'O' / 'N', jobvt 'A'). */
+rcabeam_status rcabeam_svd_device(void* matrix, int complex128, size_t rows, size_t frames, int want_u, void* values,
+                                  void* vh, void* e_work, int* info);
+
+/* U[:, k] *= sigma_k for k in [start, stop), U[:, k] = 0 otherwise (rows x
+ * frames, leading dimension ld): the retained singular values of reference
+ * gpu_clutter.py:299-301 folded into U. */
+rcabeam_status rcabeam_scale_columns_device(void* u, int complex128, size_t rows, size_t ld, size_t frames,
+                                            const void* values, size_t start, size_t stop);
+
+/* out (rows x frames, row-major) = B A for column-major B (rows x frames,
+ * leading dimension ld) and A (frames x frames), in B's precision (rocBLAS
+ * gemm), written as complex128 or complex64 (out_complex128).  When the output
+ * type differs from B's, workspace holds chunk_rows x frames elements of B's
+ * type; chunk_rows also bounds each gemm.  The U Sigma V^H reconstruction
+ * (A = V^H) of reference gpu_clutter.py:302-312 and the retained projector
+ * (A = V_r V_r^H) of clutter.py:101. */
+rcabeam_status rcabeam_times_square_device(const void* b, int complex128, size_t rows, size_t ld, size_t frames,
+                                           const void* a, void* out, int out_complex128, void* workspace,
+                                           size_t chunk_rows);
+
+/* Complex128 Gram X^H X (upper triangle, frames x frames column-major) of a
+ * column-major complex128 matrix with rocBLAS herk: the Gram of reference
+ * clutter.py:29-33 for complex128 input. */
+rcabeam_status rcabeam_gram_colmajor_device(const void* matrix, size_t rows, size_t ld, size_t frames, void* gram);
+
+/* P = V_r V_r^H (frames x frames, column-major complex128) for the columns
+ * [start, stop) of a column-major complex128 basis. */
+rcabeam_status rcabeam_projector_device(const void* basis, size_t frames, size_t start, size_t stop, void* projector);
 
 /* ---- synthetic acquisitions (GPU counterpart of reference
  * src/rcabeam/sim.py:131-189, scaled to full-size scenes) ------------------

@@ -6,7 +6,8 @@ point calls the HIP library through the C ABI (include/rcabeam.h,
 include/rcabeam_mi355x.h); there is no CPU fallback.
 """
 
-from rcabeam_amd.clutter import svd_clutter_filter, temporal_basis
+from rcabeam_amd import clutter
+from rcabeam_amd.clutter_host import svd_clutter_filter, temporal_basis
 from rcabeam_amd.ensemble import (
     EnsembleTiming,
     beamform_apertures,
@@ -32,6 +33,7 @@ __all__ = [
     "RCAGeometry",
     "beamform_apertures",
     "beamform_ensemble",
+    "clutter",
     "delay_rca_channel_data",
     "delay_rca_channels",
     "depth_axis",

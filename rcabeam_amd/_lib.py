@@ -20,7 +20,7 @@ from pathlib import Path
 LIBRARY_PATH = Path(os.environ.get("RCABEAM_LIBRARY", Path(__file__).resolve().parent / "lib" / "librcabeam_mi355x.so"))
 
 SUCCESS, ERROR_DEVICE, ERROR_ARGUMENT = 0, 1, 2
-STORAGE = {"float32": 0, "complex64": 0, "float16": 1, "bfloat16": 2}
+STORAGE = {"float32": 0, "complex64": 0, "float16": 1, "bfloat16": 2, "float16-split": 3}
 # rcabeam_coefficients (include/rcabeam_mi355x.h)
 COEF_FLOAT32 = 0
 COEF_16BIT = 1
@@ -50,6 +50,9 @@ _SIGNATURES: dict[str, list] = {
     "rcabeam_opw_grid_device": [c_void_p, c_void_p, c_int, c_int] + [c_void_p] * 3 + [c_size_t] * 3 + [c_void_p] * 6
     + [c_int, c_int] + [c_size_t] * 4 + [c_float] * 4,
     "rcabeam_unpack_ensemble_device": [c_void_p, c_void_p] + [c_size_t] * 4,
+    "rcabeam_pack_ensemble_split_device": [c_void_p, c_void_p, c_void_p] + [c_size_t] * 4,
+    "rcabeam_opw_grid_split_device": [c_void_p] * 6 + [c_size_t] * 3 + [c_void_p] * 6 + [c_int] + [c_size_t] * 4
+    + [c_float] * 4,
     "rcabeam_opw_adjoint_grid_device": [c_void_p, c_void_p] + [c_void_p] * 3 + [c_size_t] * 3 + [c_void_p] * 8
     + [c_size_t] * 4 + [c_float] * 4,
     "rcabeam_pack_ensemble_lowrank_device": [c_void_p, c_void_p, c_int, c_void_p, c_size_t, c_void_p] + [c_size_t] * 4,
@@ -69,6 +72,13 @@ _SIGNATURES: dict[str, list] = {
                                          c_size_t, c_void_p, c_void_p, c_void_p] + [c_size_t] * 4
     + [c_int] + [c_float] * 6 + [c_ulonglong],
     "rcabeam_gram_hermitian_device": [c_void_p, c_size_t, c_double, c_void_p],
+    "rcabeam_casorati_colmajor_device": [c_void_p, c_int, c_size_t, c_size_t, c_void_p, c_int, c_size_t, c_size_t],
+    "rcabeam_svd_device": [c_void_p, c_int, c_size_t, c_size_t, c_int, c_void_p, c_void_p, c_void_p, c_void_p],
+    "rcabeam_scale_columns_device": [c_void_p, c_int, c_size_t, c_size_t, c_size_t, c_void_p, c_size_t, c_size_t],
+    "rcabeam_times_square_device": [c_void_p, c_int, c_size_t, c_size_t, c_size_t, c_void_p, c_void_p, c_int,
+                                    c_void_p, c_size_t],
+    "rcabeam_gram_colmajor_device": [c_void_p, c_size_t, c_size_t, c_size_t, c_void_p],
+    "rcabeam_projector_device": [c_void_p, c_size_t, c_size_t, c_size_t, c_void_p],
     "rcabeam_eigh_device": [c_void_p, c_void_p, c_void_p, c_void_p, c_size_t],
     "rcabeam_filtered_power_device": [c_void_p, c_int, c_size_t, c_size_t, c_void_p, c_size_t, c_int, c_void_p],
     "rcabeam_temporal_filter_device": [c_void_p, c_int, c_size_t, c_size_t, c_void_p, c_size_t, c_int, c_void_p],

@@ -1,12 +1,18 @@
 """Global temporal SVD clutter filter on MI355X (Gram on MFMA, float64 EVD on the GPU).
 
 Mirrors reference src/rcabeam/gpu_clutter.py: `temporal_basis` (:112-193,
-method "gram"), `apply_temporal_filter` (:219-241), `filtered_power`
-(:244-273) and `svd_clutter_filter` (:322-336), plus the rank-interval rules
-of src/rcabeam/clutter.py:65-71.  Inputs are torch GPU tensors (complex64, or
+methods "gram" and "svd"), the direct SVD (:43-109), `apply_temporal_filter`
+(:219-241), `filtered_power` (:244-273), `svd_reference` (:276-319) and
+`svd_clutter_filter` (:322-336), plus the rank-interval rules of
+src/rcabeam/clutter.py:65-71.  Inputs are torch GPU tensors (complex64, or
 16-bit packed IQ given with `storage=`), or NumPy arrays, which are uploaded
 and answered with NumPy.  A tuple of arrays estimates one shared basis from
 all rows (RC/CR).
+
+The direct SVD runs rocSOLVER gesvd on a column-major complex128 (or
+complex64) copy of the stacked Casorati rows, as the reference runs cuSOLVER
+gesvd; its filter reconstructs U Sigma V^H of the retained interval with a
+rocBLAS gemm.
 
 Numerics: precision "complex128" forms the Gram exactly on float64 MFMA
 (complex64 products are exact in float64), "complex64" uses split bfloat16
@@ -27,7 +33,9 @@ import numpy as np
 from rcabeam_amd import _device as dev
 from rcabeam_amd import _lib
 
-Solver = Literal["gram"]
+Solver = Literal["gram", "svd"]
+# Rows per gemm of the SVD reconstruction (bounds its complex128 staging block).
+_GEMM_ROWS = 1 << 16
 
 
 def rank_interval(frames: int, rank_start, rank_stop):
@@ -54,7 +62,7 @@ def _storage_code(x, storage: str | None) -> int:
     return _lib.STORAGE[storage]
 
 
-def _inputs(iq, storage=None):
+def _inputs(iq, storage=None, allow_complex128=False):
     arrays = iq if isinstance(iq, tuple) else (iq,)
     if not arrays:
         raise ValueError("At least one IQ array is required")
@@ -66,14 +74,17 @@ def _inputs(iq, storage=None):
             arr = np.asarray(a)
             if arr.dtype not in (np.complex64, np.complex128):
                 raise ValueError("IQ must contain complex64 or complex128 arrays")
-            if arr.dtype == np.complex128:
-                raise ValueError("complex128 IQ is not supported on the GPU path; pass complex64")
-            a = dev.to_device(arr, t.complex64)
+            if arr.dtype == np.complex128 and not allow_complex128:
+                raise ValueError("complex128 IQ is not supported on the GPU Gram path; pass complex64")
+            a = dev.to_device(arr, t.complex128 if arr.dtype == np.complex128 else t.complex64)
         elif not a.is_cuda:
             raise ValueError("IQ tensors must be on the GPU")
         if a.ndim == 0 or a.numel() == 0:
             raise ValueError("IQ must be nonempty with frames on the last axis")
-        _storage_code(a, storage)
+        if allow_complex128 and a.dtype == t.complex128 and storage is None:
+            pass
+        else:
+            _storage_code(a, storage)
         out.append(a.contiguous())
     if any(a.shape[-1] != out[0].shape[-1] for a in out):
         raise ValueError("All IQ arrays must share the frame count")
@@ -92,13 +103,94 @@ def _scale_and_check(arrays, code):
     return float(host[:1].view(np.float32)[0]), stats
 
 
-def temporal_basis(iq, *, method: Solver = "gram", precision: str = "complex128", timings=None, diagnostics=None,
-                   storage: str | None = None):
-    """Descending singular values `(T,)` float64 and column basis `(T, T)` (gpu_clutter.py:112-193)."""
-    if method != "gram":
-        raise ValueError("Method must be 'gram' (the direct SVD is not provided on the GPU path)")
+def _check_finite(arrays):
+    """IQ must be finite (reference gpu_clutter.py:38-39): one |max| pass per array, one 12-byte read."""
+    t = dev.torch()
+    stats = dev.empty((3,), t.int32)
+    for i, a in enumerate(arrays):
+        if a.dtype == t.complex128:
+            if not bool(t.isfinite(t.view_as_real(a)).all()):
+                raise ValueError("IQ must be finite")
+            continue
+        _lib.call("rcabeam_iq_absmax_device", dev.ptr(a), 0, a.numel(), dev.ptr(stats), int(i > 0))
+        if stats.cpu().numpy().view(np.uint32)[1]:
+            raise ValueError("IQ must be finite")
+
+
+def colmajor_matrix(arrays, complex128: bool):
+    """Column-major copy of the stacked Casorati rows of `arrays` (rows, frames): a `(frames, rows)` tensor
+    (gpu_clutter.py:57-65: one Fortran copy also joins RC/CR)."""
+    t = dev.torch()
+    frames = arrays[0].shape[-1]
+    rows = sum(a.numel() // frames for a in arrays)
+    matrix = dev.empty((frames, rows), t.complex128 if complex128 else t.complex64)
+    offset = 0
+    for a in arrays:
+        r = a.numel() // frames
+        _lib.call("rcabeam_casorati_colmajor_device", dev.ptr(a), int(a.dtype == t.complex128), r, frames,
+                  dev.ptr(matrix), int(complex128), rows, offset)
+        offset += r
+    return matrix, rows
+
+
+def _direct_svd(arrays, timings, *, spatial: bool = False, precision: str = "complex128"):
+    """Singular values `(T,)` (padded with zeros), basis V `(T, T)` and, with `spatial`, the column-major U
+    (gpu_clutter.py:43-109)."""
     if precision not in ("complex64", "complex128"):
         raise ValueError("Precision must be complex64 or complex128")
+    t = dev.torch()
+    single = precision == "complex64"
+    frames = arrays[0].shape[-1]
+    begin = perf_counter()
+    if single and any(a.dtype == t.complex128 for a in arrays):
+        for a in arrays:
+            if a.dtype == t.complex128 and not bool(t.isfinite(t.view_as_real(a.to(t.complex64))).all()):
+                raise np.linalg.LinAlgError("SVD input cannot be represented as finite complex64; rescale the input")
+    matrix, rows = colmajor_matrix(arrays, complex128=not single)
+    if timings is not None:
+        dev.synchronize()
+        timings["matrix_seconds"] = perf_counter() - begin
+    begin = perf_counter()
+    real = t.float32 if single else t.float64
+    values = t.zeros((frames,), dtype=real, device=matrix.device)
+    vh = dev.empty((frames, frames), t.complex64 if single else t.complex128)  # column-major V^H
+    e_work = dev.empty((frames,), real)
+    info = dev.empty((1,), t.int32)
+    _lib.call("rcabeam_svd_device", dev.ptr(matrix), int(not single), rows, frames, int(spatial), dev.ptr(values),
+              dev.ptr(vh), dev.ptr(e_work), dev.ptr(info))
+    status = int(info.item())
+    if status != 0:
+        raise np.linalg.LinAlgError(f"rocSOLVER SVD failed (info={status})")
+    if timings is not None:
+        timings["decomposition_seconds"] = perf_counter() - begin
+    # The column-major V^H read row-major is V^T, so V = conj of that view.
+    basis = vh.conj().resolve_conj()
+    return values, basis, (matrix, rows) if spatial else None
+
+
+def temporal_basis(iq, *, method: Solver = "gram", precision: str = "complex128", timings=None, diagnostics=None,
+                   storage: str | None = None):
+    """Descending singular values `(T,)` and column basis `(T, T)` (gpu_clutter.py:112-193).
+
+    method "gram": Gram on MFMA + float64 EVD (values float64); "svd": direct rocSOLVER gesvd in `precision`
+    (complex128 or complex64 input; values float64 or float32)."""
+    if precision not in ("complex64", "complex128"):
+        raise ValueError("Precision must be complex64 or complex128")
+    if method == "svd":
+        if storage not in (None, "complex64", "float32"):
+            raise ValueError("The direct SVD takes complex64 or complex128 IQ")
+        arrays, host = _inputs(iq, None, allow_complex128=True)
+        dev.require_default_stream("Clutter filtering requires the default stream")
+        _check_finite(arrays)
+        values, basis, _ = _direct_svd(arrays, timings, precision=precision)
+        t = dev.torch()
+        if not bool(t.isfinite(values).all()) or not bool(t.isfinite(t.view_as_real(basis)).all()):
+            raise np.linalg.LinAlgError("Nonfinite temporal decomposition")
+        if host:
+            return dev.host(values), dev.host(basis)
+        return values, basis
+    if method != "gram":
+        raise ValueError("Method must be 'svd' or 'gram'")
     arrays, host = _inputs(iq, storage)
     code = _storage_code(arrays[0], storage)
     t = dev.torch()
@@ -240,8 +332,58 @@ def apply_temporal_filter(iq, basis, *, rank_start, rank_stop=None):
     return tuple(result) if isinstance(iq, tuple) else result[0]
 
 
+def svd_reference(iq, *, rank_start, rank_stop=None, timings=None, precision: str = "complex128"):
+    """Full direct SVD, discarded singular values zeroed, U Sigma V^H reconstructed (gpu_clutter.py:276-319).
+
+    Returns `(filtered IQ, singular values, V)`; the filtered arrays keep their input dtype.  For a
+    tall matrix U overwrites the solver's column-major copy, as in the reference."""
+    arrays, host = _inputs(iq, None, allow_complex128=True)
+    dev.require_default_stream("Clutter filtering requires the default stream")
+    t = dev.torch()
+    frames = arrays[0].shape[-1]
+    start, stop = rank_interval(frames, rank_start, rank_stop)
+    _check_finite(arrays)
+    values, basis, (matrix, rows) = _direct_svd(arrays, timings, spatial=True, precision=precision)
+    if not bool(t.isfinite(values).all()) or not bool(t.isfinite(t.view_as_real(basis)).all()):
+        raise np.linalg.LinAlgError("Nonfinite SVD decomposition")
+    begin = perf_counter()
+    single = precision == "complex64"
+    # Retained singular values folded into U (columns past min(rows, frames) are zeroed), then U' V^H.
+    _lib.call("rcabeam_scale_columns_device", dev.ptr(matrix), int(not single), rows, rows, frames, dev.ptr(values),
+              start, stop)
+    vh = basis.conj().resolve_conj()  # column-major V^H
+    result = []
+    offset = 0
+    for a in arrays:
+        r = a.numel() // frames
+        out = dev.empty(tuple(a.shape), a.dtype)
+        out_c128 = a.dtype == t.complex128
+        chunk = min(_GEMM_ROWS, r)
+        work = None if out_c128 == (not single) else dev.empty((chunk, frames), t.complex64 if single else t.complex128)
+        u_rows = matrix.reshape(-1)[offset:]  # column-major U starting at row `offset`, leading dimension rows
+        _lib.call("rcabeam_times_square_device", dev.ptr(u_rows), int(not single), r, rows, frames, dev.ptr(vh),
+                  dev.ptr(out), int(out_c128), dev.ptr(work), chunk)
+        if not bool(t.isfinite(t.view_as_real(out)).all()):
+            raise np.linalg.LinAlgError("Nonfinite SVD reconstruction")
+        result.append(out)
+        offset += r
+    if timings is not None:
+        dev.synchronize()
+        timings["application_seconds"] = perf_counter() - begin
+    filtered = tuple(result) if isinstance(iq, tuple) else result[0]
+    if host:
+        filtered = tuple(dev.host(o) for o in filtered) if isinstance(filtered, tuple) else dev.host(filtered)
+        return filtered, dev.host(values), dev.host(basis)
+    return filtered, values, basis
+
+
 def svd_clutter_filter(iq, *, rank_start, rank_stop=None, method: Solver = "gram", precision: str = "complex128"):
-    """Retain the temporal singular-vector interval [rank_start, rank_stop) (gpu_clutter.py:322-336)."""
+    """Retain the temporal singular-vector interval [rank_start, rank_stop) (gpu_clutter.py:322-336): the direct
+    SVD reference for method "svd", the Gram basis and the smaller projector otherwise."""
+    if method == "svd":
+        return svd_reference(iq, rank_start=rank_start, rank_stop=rank_stop, precision=precision)[0]
+    if method != "gram":
+        raise ValueError("Method must be 'svd' or 'gram'")
     arrays, _ = _inputs(iq)
     rank_interval(arrays[0].shape[-1], rank_start, rank_stop)
     _, basis = temporal_basis(arrays if isinstance(iq, tuple) else arrays[0], method=method, precision=precision)

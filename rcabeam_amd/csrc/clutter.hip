@@ -750,20 +750,26 @@ extern "C" rcabeam_status rcabeam_gram_hermitian_device(const double* real_gram,
   return launch_status();
 }
 
+namespace rcb {
+// The rocBLAS / rocSOLVER handle of the current device, created on first use
+// and bound to the null stream (nullptr on failure).
+rocblas_handle solver_handle() {
+  int device = 0;
+  if (hipGetDevice(&device) != hipSuccess || device < 0 || device >= 64) return nullptr;
+  std::lock_guard<std::mutex> guard(g_solver.lock);
+  if (!g_solver.handle[device]) {
+    if (rocblas_create_handle(&g_solver.handle[device]) != rocblas_status_success) return nullptr;
+    rocblas_set_stream(g_solver.handle[device], nullptr);
+  }
+  return g_solver.handle[device];
+}
+}  // namespace rcb
+
 extern "C" rcabeam_status rcabeam_eigh_device(void* gram, double* values, double* workspace_e, int* info,
                                               size_t frames) {
   if (!gram || !values || !workspace_e || !info || !frames || frames > 0x7fffffff) return RCABEAM_ERROR_ARGUMENT;
-  int device = 0;
-  if (hipGetDevice(&device) != hipSuccess || device < 0 || device >= 64) return RCABEAM_ERROR_CUDA;
-  rocblas_handle handle;
-  {
-    std::lock_guard<std::mutex> guard(g_solver.lock);
-    if (!g_solver.handle[device]) {
-      if (rocblas_create_handle(&g_solver.handle[device]) != rocblas_status_success) return RCABEAM_ERROR_CUDA;
-      rocblas_set_stream(g_solver.handle[device], nullptr);
-    }
-    handle = g_solver.handle[device];
-  }
+  rocblas_handle handle = solver_handle();
+  if (!handle) return RCABEAM_ERROR_CUDA;
   const rocblas_int n = static_cast<rocblas_int>(frames);
   const rocblas_status s =
       rocsolver_zheevd(handle, rocblas_evect_original, rocblas_fill_upper, n,

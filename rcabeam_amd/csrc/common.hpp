@@ -155,6 +155,21 @@ __device__ __forceinline__ float wave_sum(float x) {
   return x;
 }
 
+// Split FP16 channel storage: each complex64 sample x is kept as two FP16
+// pairs, hi = fp16(x 2^e) and lo = fp16(x 2^e - hi), with one power-of-two
+// scale 2^e per ensemble chosen from the joint max |component| m so that
+// m 2^e lies in [2^14, 2^15): hi + lo then carries 22 significant bits for
+// every component within 2^17 of m and an absolute error below 2^-40 m for
+// the rest.  absmax points at the bits of m (rcabeam_iq_absmax_device's
+// out[0]); m = 0 or a non-finite m gives e = 0.
+__device__ __forceinline__ int split_exponent(const unsigned* absmax) {
+  const float m = __uint_as_float(absmax[0]);
+  if (!(m > 0.f) || !(m < __builtin_inff())) return 0;
+  int ex = 0;
+  frexpf(m, &ex);  // m = f 2^ex, f in [0.5, 1)
+  return 15 - ex;
+}
+
 __device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
 __device__ __forceinline__ float uniform(float x) {
   return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));

@@ -16,6 +16,10 @@
 // are exact in the float32 accumulator:
 //   kCoefSingle : one 16-bit coefficient per sample (1 MFMA per K-step and block)
 //   kCoefSplit  : hi + lo FP16 coefficients, float32-accurate (2 MFMAs)
+// SPLIT channels (complex64 kept as scaled FP16 hi + lo word pairs, 8 bytes
+// per sample, see split_exponent) run the float32-accurate form of complex64
+// channels: C_hi S_hi + C_lo S_hi + C_hi S_lo (3 MFMAs; C_lo S_lo is below
+// 2^-22 of the product), and the epilogue undoes the power-of-two scale.
 //
 // Work decomposition: one 256-thread workgroup (4 wave64s, two per CU) owns
 // an 8 x 8 lateral patch at one depth and all frames (tiles in XCD-aware
@@ -102,23 +106,24 @@ struct MLayout {  // byte offsets inside the dynamic LDS allocation
 };
 
 // Tile tables (grid_tile.hpp) at 0; psum: float[64]; win: two buffers of
-// max_rows rows x nt words, each followed by zero padding that K-steps at the
-// end of a stage and frame blocks past nt may read.
-__host__ __device__ inline MLayout mlayout(int nc, int nt, int nb, int max_rows) {
+// max_rows rows x nt samples (wps words each), each followed by zero padding
+// that K-steps at the end of a stage and frame blocks past nt may read.
+__host__ __device__ inline MLayout mlayout(int nc, int nt, int nb, int max_rows, int wps) {
   MLayout l;
   l.psum = tile_table_bytes(nc);
   l.win = (l.psum + kTileSlots * 4 + 15) & ~15;
-  l.buffer_words = ((max_rows + kKRows) * nt + kBlockFrames * nb + 3) & ~3;
+  l.buffer_words = (((max_rows + kKRows) * nt + kBlockFrames * nb) * wps + 3) & ~3;
   l.bytes = l.win + 2 * l.buffer_words * 4;
   return l;
 }
 
-template <typename T, int CP, int NB>
+template <typename T, int CP, int NB, bool SPLIT>
 __global__ __launch_bounds__(kMThreads) __attribute__((amdgpu_waves_per_eu(2))) void das_opw_mfma(GridArgs p) {
   constexpr int CW = CP == kCoefSplit ? 4 : 2;
+  constexpr int WPS = SPLIT ? 2 : 1;  // 32-bit words per staged sample
   using W = uint32_t;
   extern __shared__ __align__(16) unsigned char lds[];
-  const MLayout layout = mlayout(p.nc, p.nt, NB, p.max_rows);
+  const MLayout layout = mlayout(p.nc, p.nt, NB, p.max_rows, WPS);
   float* psum = reinterpret_cast<float*>(lds + layout.psum);  // [64]
   W* win = reinterpret_cast<W*>(lds + layout.win);            // [2][buffer_words]
   const int buffer = layout.buffer_words;
@@ -148,7 +153,7 @@ __global__ __launch_bounds__(kMThreads) __attribute__((amdgpu_waves_per_eu(2))) 
   auto next_live = [&](int q) { return tile.next_live(q); };
   auto plane_of = [&](const Channel& ch) {
     const W* data = static_cast<const W*>(ch.cfg == 0 ? p.rc : p.cr);
-    return data + static_cast<size_t>(ch.q - ch.cfg * p.nc) * p.na * p.ns * nt;
+    return data + static_cast<size_t>(ch.q - ch.cfg * p.nc) * p.na * p.ns * nt * WPS;
   };
   const size_t plane_size = static_cast<size_t>(p.ns) * nt;
 
@@ -171,10 +176,10 @@ __global__ __launch_bounds__(kMThreads) __attribute__((amdgpu_waves_per_eu(2))) 
         lo = b.x;
         rows = b.y - b.x + 1;
       }
-      const W* src = plane0 + a * plane_size + static_cast<size_t>(lo) * nt;
-      W* dst = buf + off * nt;
+      const W* src = plane0 + (a * plane_size + static_cast<size_t>(lo) * nt) * WPS;
+      W* dst = buf + off * nt * WPS;
       off += rows;
-      const int words = rows * nt;
+      const int words = rows * nt * WPS;
       if (p.vec) {
         const int chunks = (words + 255) >> 8;  // 256 words = 1 KiB
         for (int i = g; i < chunks; i += kMWaves) {
@@ -242,22 +247,48 @@ __global__ __launch_bounds__(kMThreads) __attribute__((amdgpu_waves_per_eu(2))) 
       for (int r = lo; r <= hi; r += kKRows) {
         const int rel = take ? tp.i0 - r - 4 * kg : 64;
         const u32x4 ahi = place(rel, c0, c1);
-        // 0 <= row < 2^24: a 24-bit multiply gives the word offset.
-        const W* b0 = base + __umul24(static_cast<uint32_t>(r - first + row_off + 4 * kg), static_cast<uint32_t>(nt)) + col;
-        u32x4 bv[NB];
-#pragma unroll
-        for (int j = 0; j < NB; ++j) {
-          bv[j][0] = b0[kBlockFrames * j];
-          bv[j][1] = b0[nt + kBlockFrames * j];
-          bv[j][2] = b0[2 * nt + kBlockFrames * j];
-          bv[j][3] = b0[3 * nt + kBlockFrames * j];
-        }
-#pragma unroll
-        for (int j = 0; j < NB; ++j) acc[j] = mfma<T>(ahi, bv[j], acc[j]);
-        if constexpr (CW == 4) {
+        // 0 <= row < 2^24: a 24-bit multiply gives the sample offset.
+        const uint32_t s0 = __umul24(static_cast<uint32_t>(r - first + row_off + 4 * kg), static_cast<uint32_t>(nt)) + col;
+        if constexpr (SPLIT) {
+          // One 8-byte read per (row, frame): the hi and lo words of the sample.  Block by block, so
+          // only one block's operands are live (the three MFMAs of a block cover the next block's reads).
+          const uint2* b0 = reinterpret_cast<const uint2*>(base) + s0;
           const u32x4 alo = place(rel, l0, l1);
+          uint2 w[4];
 #pragma unroll
-          for (int j = 0; j < NB; ++j) acc[j] = mfma<T>(alo, bv[j], acc[j]);
+          for (int k = 0; k < 4; ++k) w[k] = b0[k * nt];
+#pragma unroll
+          for (int j = 0; j < NB; ++j) {
+            const u32x4 bh = {w[0].x, w[1].x, w[2].x, w[3].x};
+            const u32x4 bl = {w[0].y, w[1].y, w[2].y, w[3].y};
+            if (j + 1 < NB) {
+#pragma unroll
+              for (int k = 0; k < 4; ++k) w[k] = b0[k * nt + kBlockFrames * (j + 1)];
+            }
+            acc[j] = mfma<T>(ahi, bh, acc[j]);
+            acc[j] = mfma<T>(alo, bh, acc[j]);
+            acc[j] = mfma<T>(ahi, bl, acc[j]);
+            // One block's reads in flight behind the current block's MFMAs; hoisting every block's reads
+            // would spill registers.
+            __builtin_amdgcn_sched_barrier(0);
+          }
+        } else {
+          const W* b0 = base + s0;
+          u32x4 bv[NB];
+#pragma unroll
+          for (int j = 0; j < NB; ++j) {
+            bv[j][0] = b0[kBlockFrames * j];
+            bv[j][1] = b0[nt + kBlockFrames * j];
+            bv[j][2] = b0[2 * nt + kBlockFrames * j];
+            bv[j][3] = b0[3 * nt + kBlockFrames * j];
+          }
+#pragma unroll
+          for (int j = 0; j < NB; ++j) acc[j] = mfma<T>(ahi, bv[j], acc[j]);
+          if constexpr (CW == 4) {
+            const u32x4 alo = place(rel, l0, l1);
+#pragma unroll
+            for (int j = 0; j < NB; ++j) acc[j] = mfma<T>(alo, bv[j], acc[j]);
+          }
         }
       }
     }
@@ -307,13 +338,16 @@ __global__ __launch_bounds__(kMThreads) __attribute__((amdgpu_waves_per_eu(2))) 
   float pw[8];
 #pragma unroll
   for (int k = 0; k < 8; ++k) pw[k] = 0.f;
+  // Split channels: undo the power-of-two channel scale (exact).
+  const int unscale = SPLIT ? -split_exponent(p.absmax) : 0;
 #pragma unroll
   for (int j = 0; j < NB; ++j) {
     const int t = kBlockFrames * j + col;
     if (t >= nt) continue;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-      const float2 s = make_float2(acc[j][2 * k], acc[j][2 * k + 1]);
+      float2 s = make_float2(acc[j][2 * k], acc[j][2 * k + 1]);
+      if constexpr (SPLIT) s = make_float2(ldexpf(s.x, unscale), ldexpf(s.y, unscale));
       pw[k] = fmaf(s.x, s.x, fmaf(s.y, s.y, pw[k]));
       if (p.signal && vid[k] >= 0) {
         const long long index = vid[k] * nt + t;
@@ -345,41 +379,49 @@ __global__ __launch_bounds__(kMThreads) __attribute__((amdgpu_waves_per_eu(2))) 
   }
 }
 
-template <typename T, int CP, int NB>
+template <typename T, int CP, int NB, bool SPLIT>
 hipError_t launch_shape(GridArgs a) {
-  const int fixed = mlayout(a.nc, a.nt, NB, 0).bytes;
-  a.max_rows = std::min(kMaxRowsM, (kLdsBytesM - fixed) / (2 * a.nt * 4));
+  constexpr int WPS = SPLIT ? 2 : 1;
+  const int fixed = mlayout(a.nc, a.nt, NB, 0, WPS).bytes;
+  a.max_rows = std::min(kMaxRowsM, (kLdsBytesM - fixed) / (2 * a.nt * 4 * WPS));
   if (a.max_rows < kMinRowsM) return hipErrorNotSupported;
-  const int bytes = mlayout(a.nc, a.nt, NB, a.max_rows).bytes;
+  const int bytes = mlayout(a.nc, a.nt, NB, a.max_rows, WPS).bytes;
   if (bytes > kLdsBytesM) return hipErrorNotSupported;
   const long long grid = (a.tiles + 7) / 8 * 8;
   if (grid > 0x7fffffffLL) return hipErrorInvalidValue;
-  a.vec = reinterpret_cast<uintptr_t>(a.rc) % 16 == 0 && reinterpret_cast<uintptr_t>(a.cr) % 16 == 0 && a.nt % 4 == 0;
-  auto kernel = das_opw_mfma<T, CP, NB>;
+  // 1 KiB copies need 16-byte aligned planes and rows of a multiple of 16 bytes.
+  a.vec = reinterpret_cast<uintptr_t>(a.rc) % 16 == 0 && reinterpret_cast<uintptr_t>(a.cr) % 16 == 0 &&
+          (a.nt * WPS) % 4 == 0;
+  auto kernel = das_opw_mfma<T, CP, NB, SPLIT>;
   const hipError_t e = allow_dynamic_lds(reinterpret_cast<const void*>(kernel), kLdsBytesM);
   if (e != hipSuccess) return e;
   kernel<<<dim3(static_cast<unsigned>(grid)), kMThreads, bytes>>>(a);
   return hipGetLastError();
 }
 
-template <typename T, int CP>
+template <typename T, int CP, bool SPLIT = false>
 hipError_t launch_blocks(const GridArgs& a) {
   const int blocks = (a.nt + kBlockFrames - 1) / kBlockFrames;
-  if (blocks <= 1) return launch_shape<T, CP, 1>(a);
-  if (blocks <= 2) return launch_shape<T, CP, 2>(a);
-  if (blocks <= 4) return launch_shape<T, CP, 4>(a);
-  return launch_shape<T, CP, kMaxBlocks>(a);
+  if (blocks <= 1) return launch_shape<T, CP, 1, SPLIT>(a);
+  if (blocks <= 2) return launch_shape<T, CP, 2, SPLIT>(a);
+  if (blocks <= 4) return launch_shape<T, CP, 4, SPLIT>(a);
+  return launch_shape<T, CP, kMaxBlocks, SPLIT>(a);
 }
 
 }  // namespace
 
 hipError_t launch_opw_grid_mfma(const EnsembleArgs& p, const float* x, const float* z, const float* y, int nx, int nz,
-                                int ny, int channel_storage, int coefficients, int signal_storage) {
+                                int ny, int channel_storage, int coefficients, int signal_storage,
+                                const unsigned* absmax) {
   if (p.na > kGridMaxAngles || p.nt > kBlockFrames * kMaxBlocks) return hipErrorNotSupported;
   if (static_cast<long long>(p.ns) * p.nt > 0x7fffffffLL) return hipErrorNotSupported;
   if (signal_storage < RCABEAM_STORAGE_C64 || signal_storage > RCABEAM_STORAGE_BF16) return hipErrorInvalidValue;
-  const GridArgs a = make_grid_args(p, x, z, y, nx, nz, ny, signal_storage);
+  GridArgs a = make_grid_args(p, x, z, y, nx, nz, ny, signal_storage);
+  a.absmax = absmax;
   switch (channel_storage) {
+    case RCABEAM_STORAGE_FP16_SPLIT:
+      if (!absmax) return hipErrorInvalidValue;
+      return launch_blocks<__half2, kCoefSplit, true>(a);
     case RCABEAM_STORAGE_FP16:
       if (coefficients == kCoefSingle) return launch_blocks<__half2, kCoefSingle>(a);
       if (coefficients == kCoefSplit) return launch_blocks<__half2, kCoefSplit>(a);

@@ -56,8 +56,11 @@ hipError_t launch_opw_grid(const EnsembleArgs& p, const float* x, const float* z
 // kCoefSingle (FP16 or BF16) or kCoefSplit (FP16).  Same outputs as
 // launch_opw_grid up to float32 summation order.  hipErrorNotSupported for
 // shapes or storages it does not handle (na > 16, nt > 224, complex64).
+// RCABEAM_STORAGE_FP16_SPLIT channels (scaled FP16 hi + lo pairs) take the
+// float32-accurate three-product form; absmax is then required (the scale).
 hipError_t launch_opw_grid_mfma(const EnsembleArgs& p, const float* x, const float* z, const float* y, int nx, int nz,
-                                int ny, int channel_storage, int coefficients, int signal_storage);
+                                int ny, int channel_storage, int coefficients, int signal_storage,
+                                const unsigned* absmax = nullptr);
 
 // OPW on a Cartesian grid for complex64 channels, float32 VALU
 // interpolation (das_grid.hip): analytic windows, live-channel skipping.

@@ -111,6 +111,7 @@ struct GridArgs {
   int ns, nc, na, nt;
   int max_rows, out_storage, vec, stride;
   double c, fs, fd, fn;
+  const unsigned* absmax;  // split FP16 channels: bits of the joint max |component| (scale exponent)
 };
 
 // Grid arguments from the ensemble arguments (pd, signal and channels as

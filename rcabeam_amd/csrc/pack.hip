@@ -66,6 +66,31 @@ __global__ __launch_bounds__(256) void pack_rows(const float2* __restrict__ src,
   if (range_error && __any(bad) && lane == 0) *range_error = 1;
 }
 
+// Split FP16 packing: one wave per target row; each sample becomes the word
+// pair (hi, lo) of FP16 complex values (see split_exponent in common.hpp).
+__global__ __launch_bounds__(256) void pack_rows_split(const float2* __restrict__ src, uint2* __restrict__ dst,
+                                                       const unsigned* __restrict__ absmax, int ns, int nc, int na,
+                                                       int nt, long long rows) {
+  const long long row = static_cast<long long>(blockIdx.x) * 4 + threadIdx.x / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
+  if (row >= rows) return;
+  const int e = split_exponent(absmax);
+  const long long s = row % ns;
+  const long long ca = row / ns;
+  const long long a = ca % na;
+  const long long ch = ca / na;
+  const float2* in = src + ((s * nc + ch) * na + a) * nt;
+  uint2* out = dst + row * nt;
+  for (int t = lane; t < nt; t += kWave) {
+    const float2 v = in[t];
+    const float xr = ldexpf(v.x, e), xi = ldexpf(v.y, e);
+    const __half2 hi = __floats2half2_rn(xr, xi);
+    const float2 hf = __half22float2(hi);
+    const __half2 lo = __floats2half2_rn(xr - hf.x, xi - hf.y);
+    out[t] = make_uint2(__builtin_bit_cast(uint32_t, hi), __builtin_bit_cast(uint32_t, lo));
+  }
+}
+
 // Inverse of pack_rows for complex64: one wave per packed (channel, angle,
 // sample) row, written back to (samples, channels, angles, frames).
 __global__ __launch_bounds__(256) void unpack_rows(const float2* __restrict__ src, float2* __restrict__ dst, int ns,
@@ -139,5 +164,20 @@ extern "C" rcabeam_status rcabeam_unpack_ensemble_device(const void* source, voi
                                                              static_cast<float2*>(target), static_cast<int>(ns),
                                                              static_cast<int>(nc), static_cast<int>(na),
                                                              static_cast<int>(nt), rows);
+  return launch_status();
+}
+
+extern "C" rcabeam_status rcabeam_pack_ensemble_split_device(const void* source, void* target, const unsigned* absmax,
+                                                             size_t ns, size_t nc, size_t na, size_t nt) {
+  if (!source || !target || !absmax || !valid_dims(ns, nc, na, nt) || reinterpret_cast<uintptr_t>(source) % 8 ||
+      reinterpret_cast<uintptr_t>(target) % 8 || reinterpret_cast<uintptr_t>(absmax) % 4)
+    return RCABEAM_ERROR_ARGUMENT;
+  const long long rows = static_cast<long long>(ns) * nc * na;
+  const long long blocks = (rows + 3) / 4;
+  if (blocks > 0x7fffffffLL) return RCABEAM_ERROR_ARGUMENT;
+  pack_rows_split<<<dim3(static_cast<unsigned>(blocks)), 256>>>(static_cast<const float2*>(source),
+                                                                 static_cast<uint2*>(target), absmax,
+                                                                 static_cast<int>(ns), static_cast<int>(nc),
+                                                                 static_cast<int>(na), static_cast<int>(nt), rows);
   return launch_status();
 }

@@ -154,7 +154,7 @@ def test_invalid_inputs():
     with pytest.raises(ValueError, match="frame count"):
         gc.temporal_basis((np.ones((3, 4), np.complex64), np.ones((3, 5), np.complex64)))
     with pytest.raises(ValueError, match="Method"):
-        gc.temporal_basis(np.ones((3, 4), np.complex64), method="svd")
+        gc.temporal_basis(np.ones((3, 4), np.complex64), method="qr")
     iq = np.ones((3, 4), np.complex64)
     for basis in (np.ones((4, 4), complex), np.eye(3, dtype=complex), np.full((4, 4), complex(np.nan, 0))):
         with pytest.raises(ValueError, match="Basis"):

@@ -56,7 +56,7 @@ def test_channel_domain_filter_then_beamform(rank_start, rank_stop):
     print(f"case: channel-domain filter rank [{rank_start}, {rank_stop})", flush=True)
     acq = scene.point_scene(grid_shape=(8, 12, 8), frames=20)
     geom = rb.RCAGeometry(acq.x_el, acq.y_el, c=acq.c, fs=acq.fs, f_demod=acq.fd, fnumber=acq.fnumber)
-    rc_f, cr_f = rb.svd_clutter_filter((acq.rc, acq.cr), rank_start=rank_start, rank_stop=rank_stop)
+    rc_f, cr_f = rb.clutter.svd_clutter_filter((acq.rc, acq.cr), rank_start=rank_start, rank_stop=rank_stop)
     assert rc_f.shape == acq.rc.shape and cr_f.dtype == np.complex64
     nt = acq.rc.shape[-1]
     rows = np.concatenate([acq.rc.reshape(-1, nt), acq.cr.reshape(-1, nt)])
