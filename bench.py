@@ -408,7 +408,40 @@ def combiner_timing(scene, rc, cr) -> dict:
     seconds = time.perf_counter() - begin
     out["dmas_hybrid"] = {"ms": seconds * 1e3, "voxel_frames_per_s": nx * nz * ny * frames / seconds,
                           "prepare_ms": prepare * 1e3}
+    del reconstruct
+    out["dmas_hybrid_2us"] = hybrid_accuracy(scene)
     return out
+
+
+def hybrid_accuracy(scene) -> dict:
+    """The hybrid DMAS approximation on the same mixture re-simulated with the 2 us acquisition start its
+    geometry assumes (examples/_support/dmas_delay.py:37-38, 98): its distance from the exact grid
+    DMAS-CCF-ACF (relative L2 of the power volume; the reference reports 4.50 % on its own workload,
+    ROADMAP.md:48) and both timings."""
+    import torch
+
+    from rcabeam_amd.dmas_delay import prepare_hybrid
+    from rcabeam_amd.ensemble import ensemble_pd_from_channels
+    from rcabeam_amd.scene import mixture_scene
+
+    grid_shape = tuple(len(a) for a in scene.grid)
+    s2, rc, cr = mixture_scene(grid_shape=grid_shape, t_start=2e-6)
+    torch.cuda.synchronize()
+    begin = time.perf_counter()
+    exact = ensemble_pd_from_channels(rc, cr, s2.angles, s2.t_start, s2.grid, s2.geom, method="dmas")
+    torch.cuda.synchronize()
+    exact_s = time.perf_counter() - begin
+    reconstruct = prepare_hybrid(tuple(rc.shape), s2.grid, s2.geom, s2.angles, 0.5, 16, cubic=True)
+    reconstruct([rc, cr])
+    torch.cuda.synchronize()
+    begin = time.perf_counter()
+    approx = reconstruct([rc, cr])
+    torch.cuda.synchronize()
+    hybrid_s = time.perf_counter() - begin
+    a = approx.to(torch.float64).reshape(-1)
+    e = exact.to(torch.float64).reshape(-1)
+    return {"t_start_s": 2e-6, "exact_ms": exact_s * 1e3, "hybrid_ms": hybrid_s * 1e3,
+            "rel_l2_vs_exact_dmas": float(torch.linalg.vector_norm(a - e) / torch.linalg.vector_norm(e))}
 
 
 def combiner_timing_64el(reps: int = 3) -> dict:

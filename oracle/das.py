@@ -70,6 +70,7 @@ def delay_volumes(
     *,
     channels_out: bool = False,
     aperture_depth=None,
+    float32_delays: bool = False,
 ):
     """Delay one aperture of channel IQ onto voxels (sim.py:344-393).
 
@@ -90,6 +91,12 @@ def delay_volumes(
     aperture_depth
         Depth that sizes the receive aperture of every voxel (sim.py:372-373,
         used by two_stage.py:163-165); None: each voxel's own depth.
+    float32_delays
+        Form receive, transmit and sample positions in float32 with the
+        reference's operation order.  The reference's NumPy delay runs in
+        float32 whenever its grid axes are float32 arrays, as its two-stage
+        reference planes are (two_stage.py:37, 163-165); this reproduces that
+        rounding.  Default: float64.
 
     Returns
     -------
@@ -125,10 +132,21 @@ def delay_volumes(
     channels = np.arange(nc)
     block = max(1, _BLOCK_ELEMENTS // (nc * nt))
     cos_t, sin_t = np.cos(th), np.sin(th)
+    if float32_delays:
+        f32 = np.float32
+        coords32 = np.asarray(scan, dtype=np.float32)
+        el32, th32, t032 = el.astype(f32), th.astype(f32), t0.astype(f32)
+        c32, fs32 = f32(c), f32(fs)
     for first in range(0, nv, block):
         sl = slice(first, min(first + block, nv))
         depth = z[sl, None]
         distance = rx_coord[sl, None] - el[None, :]
+        if float32_delays:
+            x32, z32, y32 = coords32[sl, 0], coords32[sl, 1], coords32[sl, 2]
+            rx32, tx32 = (x32, y32) if config == "RC" else (y32, x32)
+            depth32 = z32[:, None]
+            distance32 = rx32[:, None] - el32[None, :]
+            receive32 = np.sqrt(depth32**2 + distance32**2) / c32
         receive = np.sqrt(depth**2 + distance**2) / c
         if full:
             active = np.ones_like(distance, dtype=bool)
@@ -145,13 +163,22 @@ def delay_volumes(
             transmit = (z[sl] * cos_t[m] + tx_coord[sl] * sin_t[m]) / c
             tau = transmit[:, None] + receive
             sample = (tau - t0[m]) * fs
+            phase = 2 * np.pi * fd * tau
+            if float32_delays:
+                transmit32 = (depth32[:, 0] * np.cos(th32[m]) + tx32 * np.sin(th32[m])) / c32
+                tau32 = transmit32[:, None] + receive32
+                tau = tau32.astype(np.float64)
+                sample = ((tau32 - t032[m]) * fs32).astype(np.float64)
+                # The reference's exp(2j pi fd tau) with float32 fd and tau evaluates its argument in complex64:
+                # float32(float32(2 pi) fd) times tau, rounded to float32.
+                phase = ((np.float32(2 * np.pi) * np.float32(fd)) * tau32).astype(np.float64)
             valid = (sample >= 0) & (sample <= ns - 1)
             sample = np.clip(sample, 0, ns - 1)
             lower = np.floor(sample).astype(np.intp)
             upper = np.minimum(lower + 1, ns - 1)
             w = (sample - lower)[..., None]
             delayed = (1 - w) * data[lower, channels, m] + w * data[upper, channels, m]
-            delayed *= (np.exp(2j * np.pi * fd * tau) * (active & valid))[..., None]
+            delayed *= (np.exp(1j * phase) * (active & valid))[..., None]
             per_angle[sl, m] = delayed.sum(axis=1)
             if per_channel is not None:
                 per_channel[sl] += delayed

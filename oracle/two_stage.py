@@ -157,8 +157,13 @@ def _wavenumbers(angles, c, fd):
 
 
 def two_stage_per_angle(rc, cr, angles, t_start, grid, x_el, y_el, c, fs, fd, fnumber, *,
-                        samples_per_wavelength=8, n_planes=1, baseband=False):
-    """Per-angle expanded signals `(rc_part, cr_part)`, each `(nx, nz, ny, na, nt)` complex128."""
+                        samples_per_wavelength=8, n_planes=1, baseband=False, float32_delays=False):
+    """Per-angle expanded signals `(rc_part, cr_part)`, each `(nx, nz, ny, na, nt)` complex128.
+
+    `float32_delays` reproduces the reference's float32 geometry: plane delays in float32 (its plane
+    grids are float32 arrays, so `_delay_numpy` runs in float32 and evaluates exp(2j pi fd tau) in
+    complex64, two_stage.py:163-165) and float32 knot positions (two_stage.py:178).  The default is
+    the float64 answer, which the GPU kernels follow."""
     rc = np.asarray(rc)
     cr = np.asarray(cr)
     if rc.shape != cr.shape:
@@ -182,7 +187,7 @@ def two_stage_per_angle(rc, cr, angles, t_start, grid, x_el, y_el, c, fs, fd, fn
             plane_grid = (lateral, knots, centers) if config == 0 else (centers, knots, lateral)
             per_angle, _, _ = delay_volumes(raw, scan_coords(plane_grid), elements, p["angles"], p["starts"],
                                             c32, fs32, fd32, fnum, "RC" if config == 0 else "CR",
-                                            aperture_depth=z[band.start])
+                                            aperture_depth=z[band.start], float32_delays=float32_delays)
             if config == 0:
                 planes = per_angle.reshape(len(lateral), len(knots), len(centers), na, nt)
             else:
@@ -193,8 +198,12 @@ def two_stage_per_angle(rc, cr, angles, t_start, grid, x_el, y_el, c, fs, fd, fn
                 planes = planes * np.exp(-1j * phase)[None, ..., None]
             second = second_derivatives(planes, axis=1)
             for m in range(na):
-                pos = (zb[:, None] + offset.astype(np.float64)[None, :] * float(p["slopes"][m])
-                       - float(knots[0])) / dz
+                if float32_delays:
+                    # float32 knot position, as the reference forms it (two_stage.py:178)
+                    pos = ((z[band, None] + offset[None, :] * p["slopes"][m] - knots[0]) / p["dz"]).astype(np.float64)
+                else:
+                    pos = (zb[:, None] + offset.astype(np.float64)[None, :] * float(p["slopes"][m])
+                           - float(knots[0])) / dz
                 i = np.floor(pos).astype(np.int64)
                 if i.min() < 0 or i.max() >= len(knots) - 1:
                     raise AssertionError("voxel mapped outside its knot slice")

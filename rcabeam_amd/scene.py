@@ -111,7 +111,7 @@ def _tube_blood(rng, n_blood, frames, prf_hz, lo, hi, tubes=3, radius=0.5e-3, sp
 
 def mixture_scene(*, elements=128, pitch=200e-6, samples=1024, angles_deg=10.0, angle_count=11, frames=200,
                   grid_shape=(128, 256, 128), lateral=12.8e-3, depth=(4e-3, 20e-3), n_tissue=20000, n_blood=2000,
-                  blood_db=-40.0, noise_db=-60.0, max_drift=0.05, prf_hz=500.0, seed=42, fnumber=1.0):
+                  blood_db=-40.0, noise_db=-60.0, max_drift=0.05, prf_hz=500.0, seed=42, fnumber=1.0, t_start=0.0):
     """Scene description plus RC and CR complex64 device tensors `(samples, elements, angles, frames)`.
 
     Tissue: `n_tissue` static Rayleigh scatterers (E|a|^2 = 1) split into three
@@ -121,6 +121,7 @@ def mixture_scene(*, elements=128, pitch=200e-6, samples=1024, angles_deg=10.0, 
     rank three.  Blood: `n_blood` Rayleigh scatterers at `blood_db` moving at
     1-20 mm/s along three seeded tube segments (frame rate `prf_hz`).  Noise:
     circular complex Gaussian with E|n|^2 = 10^(noise_db / 10) per sample.
+    `t_start` is the time of the first sample (the same for every angle).
     """
     rng = np.random.default_rng(seed)
     el = ((np.arange(elements) - (elements - 1) / 2) * pitch).astype(np.float32)
@@ -144,7 +145,7 @@ def mixture_scene(*, elements=128, pitch=200e-6, samples=1024, angles_deg=10.0, 
     noise_std = float(np.sqrt(10 ** (noise_db / 10) / 2))
     out = []
     for config in ("RC", "CR"):
-        out.append(_simulate(geom, angles, config, samples, frames, 0.0, static=(tissue, tissue_amp, tissue_group),
+        out.append(_simulate(geom, angles, config, samples, frames, t_start, static=(tissue, tissue_amp, tissue_group),
                              groups=factors, moving=(blood, blood_amp), noise_std=noise_std, seed=seed))
-    scene = MixtureScene(geom, angles, grid, 0.0, samples, frames, seed, tissue, blood, prf_hz)
+    scene = MixtureScene(geom, angles, grid, float(t_start), samples, frames, seed, tissue, blood, prf_hz)
     return scene, out[0], out[1]

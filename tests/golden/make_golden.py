@@ -3,7 +3,8 @@
 Run once, in a container where the reference repository is mounted at
 /root/reference (it never travels with this repository):
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py            # every fixture
+    python tests/golden/make_golden.py two_stage  # only two_stage.npz
 
 Only functions that no closed form, known-answer test or invariant pins
 exactly are evaluated here (the reference's exact complex64 delay-and-sum,
@@ -93,5 +94,39 @@ def main() -> None:
     print("wrote", sorted(p.name for p in HERE.glob("*.npz")))
 
 
+def two_stage() -> None:
+    """Reference `two_stage_opw_numpy` (src/rcabeam/two_stage.py:146-192) at steered angles: one and two
+    reference planes, RF and baseband planes, an f-number of 8 and the full aperture, on a small random
+    ensemble (its own seed, so the other fixtures are unaffected).  Written to two_stage.npz."""
+    sys.path.insert(0, str(REFERENCE_SRC))
+    from rcabeam import sim, two_stage as ts  # noqa: PLC0415
+
+    rng = np.random.default_rng(77)
+    ns, nc, na, nt = 80, 4, 3, 3
+    rc = (rng.normal(size=(ns, nc, na, nt)) + 1j * rng.normal(size=(ns, nc, na, nt))).astype(np.complex64)
+    cr = (rng.normal(size=(ns, nc, na, nt)) + 1j * rng.normal(size=(ns, nc, na, nt))).astype(np.complex64)
+    x_el = f32((np.arange(nc) - (nc - 1) / 2) * 200e-6)
+    y_el = f32((np.arange(nc) - (nc - 1) / 2) * 180e-6 + 20e-6)
+    angles = f32([-0.12, 0.03, 0.1])
+    starts = f32([4.9e-6, 5.0e-6, 5.1e-6])
+    grid = (f32(np.linspace(-0.3e-3, 0.3e-3, 3)), f32(np.linspace(4.0e-3, 4.4e-3, 6)), f32(np.linspace(-0.25e-3, 0.35e-3, 4)))
+    fix = {"rc": rc, "cr": cr, "x_el": x_el, "y_el": y_el, "angles": angles, "starts": starts,
+           "grid_x": grid[0], "grid_z": grid[1], "grid_y": grid[2]}
+    # F = 8 gives a 0.25 mm receive radius at 4 mm: the aperture bands cut through the element row.
+    for fn_name, fnumber in (("f8", 8.0), ("full", None)):
+        geom = sim.RCAGeometry(x_el, y_el, c=1540.0, fs=20e6, f_demod=15e6, fnumber=fnumber)
+        for n_planes in (1, 2):
+            for baseband in (False, True):
+                key = f"{fn_name}_p{n_planes}_{'bb' if baseband else 'rf'}"
+                fix[key] = ts.two_stage_opw_numpy(rc, cr, angles, starts, grid, geom, samples_per_wavelength=8,
+                                                  n_planes=n_planes, baseband=baseband)
+    np.savez_compressed(HERE / "two_stage.npz", **fix)
+    print("wrote two_stage.npz")
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["two_stage"]:
+        two_stage()
+    else:
+        main()
+        two_stage()

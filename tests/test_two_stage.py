@@ -273,3 +273,58 @@ def test_per_frame_functions_refuse_a_cpu_path():
             f(rc[..., 0], cr[..., 0], angles, starts, grid, geom, use_cuda=False)
     with pytest.raises(ValueError, match="no CPU path"):
         rb.opw(np.zeros((2, 3), np.complex64), use_cuda=False)
+
+
+GOLDEN_TS = np.load(__import__("pathlib").Path(__file__).parent / "golden" / "two_stage.npz")
+GOLDEN_KEYS = [f"{fn}_p{p}_{b}" for fn in ("f8", "full") for p in (1, 2) for b in ("rf", "bb")]
+
+
+def golden_two_stage_case(key):
+    """Inputs of one `tests/golden/two_stage.npz` case (written by the reference `two_stage_opw_numpy`):
+    (rc, cr, angles, starts, grid, x_el, y_el, fnumber, options, expected)."""
+    g = GOLDEN_TS
+    fn = 8.0 if key.startswith("f8") else None
+    options = {"samples_per_wavelength": 8, "n_planes": int(key.split("_p")[1][0]), "baseband": key.endswith("bb")}
+    grid = (g["grid_x"], g["grid_z"], g["grid_y"])
+    return g["rc"], g["cr"], g["angles"], g["starts"], grid, g["x_el"], g["y_el"], fn, options, g[key]
+
+
+@pytest.mark.parametrize("key", GOLDEN_KEYS)
+def test_oracle_matches_reference_golden_steered(key):
+    """The steered slab mapping z + o tan(theta / 2), the aperture bands (f-number 8 cuts the element row),
+    one and two planes, RF and baseband planes, against the reference's own output.
+
+    With the reference's float32 plane delays (its plane grids are float32 arrays, so its NumPy delay
+    runs in float32) the restatement agrees to the fixture's complex64 rounding: rel L2 < 1e-6.  The
+    float64 delays the GPU kernels use differ from the reference by that float32 rounding alone: tau
+    ~5 us carries ~2^-24 relative (3e-13 s), i.e. ~3e-5 rad of carrier phase at 15 MHz and ~1e-5 samples
+    at 20 MHz per tap, a few 1e-5 relative on the result (bar 1e-4)."""
+    rc, cr, angles, starts, grid, x_el, y_el, fn, options, expected = golden_two_stage_case(key)
+    got32 = ots.two_stage_opw(rc, cr, angles, starts, grid, x_el, y_el, C, FS, FD, fn, float32_delays=True,
+                              **options)
+    err32 = np.linalg.norm(got32 - expected) / np.linalg.norm(expected)
+    got = ots.two_stage_opw(rc, cr, angles, starts, grid, x_el, y_el, C, FS, FD, fn, **options)
+    err = np.linalg.norm(got - expected) / np.linalg.norm(expected)
+    assert err32 < 1e-6, err32
+    assert err < 1e-4, err
+
+
+def test_golden_detects_a_dropped_slab_mapping():
+    """A planted mistake in the oracle (the elevation-offset term o tan(theta / 2) of the steered mapping
+    dropped, i.e. every voxel read at its own depth) fails the golden comparison: the fixture pins the
+    steered mapping, which the zero-steering and closest-element checks leave open."""
+    rc, cr, angles, starts, grid, x_el, y_el, fn, options, expected = golden_two_stage_case("full_p2_rf")
+    original = ots.plan
+
+    def wrong_plan(*args, **kwargs):
+        p = original(*args, **kwargs)
+        p["slopes"] = np.zeros_like(p["slopes"])
+        return p
+
+    try:
+        ots.plan = wrong_plan
+        got = ots.two_stage_opw(rc, cr, angles, starts, grid, x_el, y_el, C, FS, FD, fn, float32_delays=True,
+                                **options)
+    finally:
+        ots.plan = original
+    assert np.linalg.norm(got - expected) / np.linalg.norm(expected) > 1e-3
