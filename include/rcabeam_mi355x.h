@@ -306,6 +306,23 @@ rcabeam_status rcabeam_gram_colmajor_device(const void* matrix, size_t rows, siz
  * [start, stop) of a column-major complex128 basis. */
 rcabeam_status rcabeam_projector_device(const void* basis, size_t frames, size_t start, size_t stop, void* projector);
 
+/* ---- RF -> IQ demodulation (no reference counterpart: reference
+ * ROADMAP.md:58 lists RF-to-IQ processing as open work) ---------------------
+ * Real RF rf (ns_rf, nc, na, nt) of rf_type 0 (float32) or 1 (int16) at fs_rf
+ * Hz, first sample at t_start, to complex64 IQ at fs_rf / decimation:
+ *   iq[m] = sum_k 2 taps[k] rf[m decimation + k - (ntaps - 1) / 2]
+ *           exp(-2j pi f_demod t_{m decimation + k - (ntaps - 1) / 2}),
+ * t_n = t_start + n / fs_rf, RF outside [0, ns_rf) zero, so iq sample m sits
+ * at t_start + m decimation / fs_rf and an echo delayed by tau comes out as
+ * env(t - tau) exp(-2j pi f_demod tau), the convention of the reference
+ * simulator and delay-and-sum (src/rcabeam/sim.py:188, 385).  taps: ntaps
+ * (odd, <= 255) real low-pass coefficients (device).  ns_iq <= ceil(ns_rf /
+ * decimation).  packed == 0: iq in (ns_iq, nc, na, nt); packed != 0: iq in the
+ * frame-inner (nc, na, ns_iq, nt) layout of rcabeam_pack_ensemble_device. */
+rcabeam_status rcabeam_rf_to_iq_device(const void* rf, int rf_type, size_t ns_rf, size_t nc, size_t na, size_t nt,
+                                       const float* taps, size_t ntaps, size_t decimation, float fs_rf,
+                                       float f_demod, float t_start, void* iq, size_t ns_iq, int packed);
+
 /* ---- synthetic acquisitions (GPU counterpart of reference
  * src/rcabeam/sim.py:131-189, scaled to full-size scenes) ------------------
  * Gaussian-envelope echoes exp(-((t - tau)/sigma_t)^2) exp(-2j pi fd tau) of

@@ -78,6 +78,8 @@ _SIGNATURES: dict[str, list] = {
     "rcabeam_times_square_device": [c_void_p, c_int, c_size_t, c_size_t, c_size_t, c_void_p, c_void_p, c_int,
                                     c_void_p, c_size_t],
     "rcabeam_gram_colmajor_device": [c_void_p, c_size_t, c_size_t, c_size_t, c_void_p],
+    "rcabeam_rf_to_iq_device": [c_void_p, c_int] + [c_size_t] * 4 + [c_void_p, c_size_t, c_size_t] + [c_float] * 3
+    + [c_void_p, c_size_t, c_int],
     "rcabeam_projector_device": [c_void_p, c_size_t, c_size_t, c_size_t, c_void_p],
     "rcabeam_eigh_device": [c_void_p, c_void_p, c_void_p, c_void_p, c_size_t],
     "rcabeam_filtered_power_device": [c_void_p, c_int, c_size_t, c_size_t, c_void_p, c_size_t, c_int, c_void_p],

@@ -24,7 +24,7 @@ OUT_DIR = PACKAGE / "lib"
 LIBRARY = OUT_DIR / "librcabeam_mi355x.so"
 ARCH = "gfx950"
 
-SOURCES = ["pack.hip", "das.hip", "das_tile.hip", "das_mfma.hip", "das_grid.hip", "lowrank.hip", "ingest.hip", "two_stage.hip", "combine.hip", "hybrid.hip", "frame.hip", "clutter.hip", "svd.hip", "scene.hip"]
+SOURCES = ["pack.hip", "das.hip", "das_tile.hip", "das_mfma.hip", "das_grid.hip", "lowrank.hip", "ingest.hip", "two_stage.hip", "combine.hip", "hybrid.hip", "frame.hip", "clutter.hip", "svd.hip", "demod.hip", "scene.hip"]
 
 HIPCC_FLAGS = [
     "-O3",
