@@ -167,9 +167,10 @@ class Pipeline:
 
         _lib.check(getattr(self.lib, name)(*args), name)
 
-    def step(self, das_events=None, marks=None):
+    def step(self, das_events=None, marks=None, hosts=None):
         """Enqueue one full step on the null stream.  Optional `das_events` bracket the DAS launch;
-        `marks` (a list) receives (stage name, event) pairs recorded at the end of each stage."""
+        `marks` (a list) receives (stage name, event) pairs recorded at the end of each stage.  `hosts`
+        (float32 form only): pageable RC / CR arrays uploaded inside the step, CR overlapping the RC pass."""
         from rcabeam_amd import _lib
 
         dev, nv = self.dev, self.g.nv
@@ -203,7 +204,7 @@ class Pipeline:
             for raw, packed in zip((self.rc, self.cr), self.packed):
                 self._call("rcabeam_pack_ensemble_lowrank_device", dev.ptr(raw), dev.ptr(packed), channel_code,
                            dev.ptr(self.basis), self.k, dev.ptr(self.range_error), *shape)
-        else:
+        elif hosts is None:
             for raw, packed in zip((self.rc, self.cr), self.packed):
                 if self.storage == "float32":
                     self._call("rcabeam_pack_ensemble_device", dev.ptr(raw), dev.ptr(packed), *shape)
@@ -211,21 +212,35 @@ class Pipeline:
                     pack = ("rcabeam_pack_ensemble_fp16_device" if self.storage == "float16"
                             else "rcabeam_pack_ensemble_bf16_device")
                     self._call(pack, dev.ptr(raw), dev.ptr(packed), dev.ptr(self.range_error), *shape)
-        mark("pack")
-        if das_events is not None:
-            das_events[0].record()
-        # OPW on the Cartesian grid (8 x 8 lateral tiles).  float32: complex64 channels and IQ, VALU;
-        # 16-bit: MFMA with one 16-bit coefficient per sample (exact products, float32 accumulation).
         nx, nz, ny = self.g.shape
-        coef = 0 if self.storage == "float32" else _lib_coef_16bit()
-        out = self.coded if self.lowrank else self.signal
-        self._call("rcabeam_opw_grid_device", dev.ptr(self.packed[0]), dev.ptr(self.packed[1]), channel_code, coef,
-                   dev.ptr(self.g.axes[0]), dev.ptr(self.g.axes[1]), dev.ptr(self.g.axes[2]), nx, nz, ny,
-                   dev.ptr(self.g.xe), dev.ptr(self.g.ye), dev.ptr(self.g.angles), dev.ptr(self.g.starts),
-                   dev.ptr(self.pd), dev.ptr(out), 0 if self.lowrank else self.code, 0, self.ns, self.nc, self.na,
-                   self.ntc, c, fs, fd, fn)
-        if das_events is not None:
-            das_events[1].record()
+        if hosts is not None:
+            # Host channels: RC uploaded, packed and summed; CR uploaded on the copy stream while that pass runs,
+            # then packed and added (rcabeam_opw_grid_aperture_device), complex64 only.
+            threads = dev.upload_threads()
+            for aperture, (h, raw, packed) in enumerate(zip(hosts, (self.rc, self.cr), self.packed)):
+                name = "rcabeam_upload_host" if aperture == 0 else "rcabeam_upload_host_concurrent"
+                self._call(name, h.ctypes.data, dev.ptr(raw), h.nbytes, threads)
+                self._call("rcabeam_pack_ensemble_device", dev.ptr(raw), dev.ptr(packed), *shape)
+                self._call("rcabeam_opw_grid_aperture_device", dev.ptr(self.packed[0]), dev.ptr(self.packed[1]),
+                           dev.ptr(self.g.axes[0]), dev.ptr(self.g.axes[1]), dev.ptr(self.g.axes[2]), nx, nz, ny,
+                           dev.ptr(self.g.xe), dev.ptr(self.g.ye), dev.ptr(self.g.angles), dev.ptr(self.g.starts),
+                           dev.ptr(self.pd), dev.ptr(self.signal), aperture, aperture, self.ns, self.nc, self.na,
+                           self.nt, c, fs, fd, fn)
+        else:
+            mark("pack")
+            if das_events is not None:
+                das_events[0].record()
+            # OPW on the Cartesian grid (8 x 8 lateral tiles).  float32: complex64 channels and IQ, VALU;
+            # 16-bit: MFMA with one 16-bit coefficient per sample (exact products, float32 accumulation).
+            coef = 0 if self.storage == "float32" else _lib_coef_16bit()
+            out = self.coded if self.lowrank else self.signal
+            self._call("rcabeam_opw_grid_device", dev.ptr(self.packed[0]), dev.ptr(self.packed[1]), channel_code,
+                       coef, dev.ptr(self.g.axes[0]), dev.ptr(self.g.axes[1]), dev.ptr(self.g.axes[2]), nx, nz, ny,
+                       dev.ptr(self.g.xe), dev.ptr(self.g.ye), dev.ptr(self.g.angles), dev.ptr(self.g.starts),
+                       dev.ptr(self.pd), dev.ptr(out), 0 if self.lowrank else self.code, 0, self.ns, self.nc,
+                       self.na, self.ntc, c, fs, fd, fn)
+            if das_events is not None:
+                das_events[1].record()
         mark("das")
         if self.lowrank:
             self._call("rcabeam_lowrank_expand_device", dev.ptr(self.coded), dev.ptr(self.basis), self.k,
@@ -341,26 +356,36 @@ def host_to_host(pipe, reps: int = 3) -> dict:
     """One float32 step from host channels to a host power volume, the reference's published boundary
     (examples/_support/rca_benchmark.py:108; ROADMAP.md:45): pageable NumPy RC and CR -> pinned staged upload
     (rcabeam_upload_host) into the resident channel buffers -> the device step -> float32 power to the host.
-    Best of `reps` after a warm-up."""
+    Timed sequentially and with the CR upload overlapping the RC pass; best of `reps` after a warm-up."""
     import torch
 
     from rcabeam_amd import _device as dev
     from rcabeam_amd import _lib
 
     hosts = [pipe.rc.cpu().numpy(), pipe.cr.cpu().numpy()]
-    best = float("inf")
-    for _ in range(reps + 1):
-        torch.cuda.synchronize()
-        begin = time.perf_counter()
-        for h, d in zip(hosts, (pipe.rc, pipe.cr)):
-            _lib.call("rcabeam_upload_host", h.ctypes.data, d.data_ptr(), h.nbytes, dev.upload_threads())
-        pipe.step()
-        power = pipe.power.to(torch.float32).cpu().numpy()
-        best = min(best, time.perf_counter() - begin)
+    out = {}
+    for key, overlapped in (("sequential", False), ("overlapped", True)):
+        best = float("inf")
+        for _ in range(reps + 1):
+            torch.cuda.synchronize()
+            begin = time.perf_counter()
+            if overlapped:
+                pipe.step(hosts=hosts)
+            else:
+                for h, d in zip(hosts, (pipe.rc, pipe.cr)):
+                    _lib.call("rcabeam_upload_host", h.ctypes.data, d.data_ptr(), h.nbytes, dev.upload_threads())
+                pipe.step()
+            power = pipe.power.to(torch.float32).cpu().numpy()
+            best = min(best, time.perf_counter() - begin)
+        out[key] = best
     nv, nt = pipe.g.nv, pipe.nt
+    best = min(out.values())
     return {"ms": best * 1e3, "value": nv * nt / best, "unit": "voxel-frames/s",
+            "sequential_ms": out["sequential"] * 1e3, "overlapped_ms": out["overlapped"] * 1e3,
             "upload_bytes": sum(h.nbytes for h in hosts), "output": f"{power.nbytes} bytes float32 power",
-            "threads": dev.upload_threads()}
+            "threads": dev.upload_threads(),
+            "note": "sequential: both staged uploads, then the device step; overlapped: the CR upload runs while "
+                    "the RC half of the delay-and-sum executes (two aperture passes)"}
 
 
 def combiner_timing(scene, rc, cr) -> dict:

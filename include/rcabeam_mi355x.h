@@ -103,6 +103,19 @@ rcabeam_status rcabeam_opw_grid_split_device(const void* rc, const void* cr, con
                                              size_t ns, size_t nc, size_t na, size_t nt, float c, float fs, float fd,
                                              float fn);
 
+/* Grid OPW (method 0, complex64 channels, the VALU grid kernel) over one
+ * aperture only: aperture 0 sums the RC channels, 1 the CR channels.  With
+ * accumulate != 0 the sums are added to the complex64 signal already there
+ * (the other aperture's) and pd is the power of the total; otherwise signal is
+ * overwritten.  RC then CR with accumulate gives rcabeam_opw_grid_device's IQ
+ * and power (up to float32 rounding of the one extra addition), so the upload
+ * of CR can overlap the RC pass.  signal: (nv, nt) complex64, required. */
+rcabeam_status rcabeam_opw_grid_aperture_device(const void* rc, const void* cr, const float* x, const float* z,
+                                                const float* y, size_t nx, size_t nz, size_t ny, const float* xe,
+                                                const float* ye, const float* angles, const float* starts, float* pd,
+                                                void* signal, int aperture, int accumulate, size_t ns, size_t nc,
+                                                size_t na, size_t nt, float c, float fs, float fd, float fn);
+
 /* Inverse of rcabeam_pack_ensemble_device: complex64 (channels, angles,
  * samples, frames) back to (samples, channels, angles, frames).  Used to
  * return channel gradients in the caller's layout. */
@@ -118,6 +131,13 @@ rcabeam_status rcabeam_unpack_ensemble_device(const void* source, void* target, 
  * until the process ends (128 MiB of pinned host memory per device); calls on
  * one device are serialized. */
 rcabeam_status rcabeam_upload_host(const void* host, void* device, size_t bytes, int threads);
+
+/* As rcabeam_upload_host, ordered after work already enqueued on the null
+ * stream (which may still read or write `device`).  The _concurrent form
+ * does not wait for that work, so the copy overlaps kernels already in flight
+ * (e.g. the delay-and-sum of the other aperture); the caller guarantees that
+ * no queued work touches `device`. */
+rcabeam_status rcabeam_upload_host_concurrent(const void* host, void* device, size_t bytes, int threads);
 
 /* XDoppler power from separate-aperture IQ: apertures (nv, 2, nt) complex64
  * (RC then CR per voxel, as rcabeam_opw_grid_device method 5 writes them);

@@ -47,6 +47,9 @@ _SIGNATURES: dict[str, list] = {
     "rcabeam_opw_mixed_device": [c_void_p] * 9 + [c_int, c_int, c_int] + [c_size_t] * 5 + [c_float] * 4,
     "rcabeam_xdoppler_from_apertures_device": [c_void_p, c_size_t, c_size_t, c_void_p, c_void_p],
     "rcabeam_upload_host": [c_void_p, c_void_p, c_size_t, c_int],
+    "rcabeam_upload_host_concurrent": [c_void_p, c_void_p, c_size_t, c_int],
+    "rcabeam_opw_grid_aperture_device": [c_void_p, c_void_p] + [c_void_p] * 3 + [c_size_t] * 3 + [c_void_p] * 6
+    + [c_int, c_int] + [c_size_t] * 4 + [c_float] * 4,
     "rcabeam_opw_grid_device": [c_void_p, c_void_p, c_int, c_int] + [c_void_p] * 3 + [c_size_t] * 3 + [c_void_p] * 6
     + [c_int, c_int] + [c_size_t] * 4 + [c_float] * 4,
     "rcabeam_unpack_ensemble_device": [c_void_p, c_void_p] + [c_size_t] * 4,

@@ -776,6 +776,32 @@ extern "C" rcabeam_status rcabeam_opw_grid_split_device(const void* rc, const vo
   return to_status(e);
 }
 
+extern "C" rcabeam_status rcabeam_opw_grid_aperture_device(const void* rc, const void* cr, const float* x,
+                                                           const float* z, const float* y, size_t nx, size_t nz,
+                                                           size_t ny, const float* xe, const float* ye,
+                                                           const float* angles, const float* starts, float* pd,
+                                                           void* signal, int aperture, int accumulate, size_t ns,
+                                                           size_t nc, size_t na, size_t nt, float c, float fs,
+                                                           float fd, float fn) {
+  if (!x || !z || !y || !nx || !nz || !ny || nx > 0x7fffffff || nz > 0x7fffffff || ny > 0x7fffffff || !signal ||
+      (aperture != 0 && aperture != 1))
+    return RCABEAM_ERROR_ARGUMENT;
+  if (reinterpret_cast<uintptr_t>(rc) % 8 || reinterpret_cast<uintptr_t>(cr) % 8 ||
+      reinterpret_cast<uintptr_t>(signal) % 8)
+    return RCABEAM_ERROR_ARGUMENT;
+  const size_t nv = nx * nz * ny;
+  if (!valid_ensemble(rc, cr, x, xe, ye, angles, starts, pd, ns, nc, na, nt, nv, c, fs, fd, fn))
+    return RCABEAM_ERROR_ARGUMENT;
+  EnsembleArgs p = make_args(rc, cr, nullptr, xe, ye, angles, starts, pd, signal, nullptr, nullptr, ns, nc, na, nt, nv,
+                             c, fs, fd, fn);
+  const int q0 = aperture == 0 ? 0 : static_cast<int>(nc);
+  const hipError_t e = launch_opw_grid_valu(p, x, z, y, static_cast<int>(nx), static_cast<int>(nz),
+                                            static_cast<int>(ny), RCABEAM_STORAGE_C64, false, q0,
+                                            q0 + static_cast<int>(nc), accumulate != 0);
+  if (e == hipErrorNotSupported) return RCABEAM_ERROR_ARGUMENT;
+  return to_status(e);
+}
+
 namespace rcb {
 namespace {
 

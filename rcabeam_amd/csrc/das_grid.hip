@@ -381,7 +381,11 @@ __global__ __launch_bounds__(kVThreads) __attribute__((amdgpu_waves_per_eu(4))) 
 #pragma unroll
     for (int e = 0; e < 2; ++e) {
       if (t + e < nt) {
-        const float2 s = acc[2 * j + e];
+        float2 s = acc[2 * j + e];
+        if (p.accumulate && v >= 0) {  // complex64 signal holding the sums of other channels
+          const float2 prior = reinterpret_cast<const float2*>(p.signal)[v * nt + t + e];
+          s = make_float2(s.x + prior.x, s.y + prior.y);
+        }
         power = fmaf(s.x, s.x, fmaf(s.y, s.y, power));
         if (p.signal && v >= 0) {
           const long long index = v * nt + t + e;
@@ -439,12 +443,19 @@ hipError_t launch_frames(const GridArgs& a) {
 }  // namespace
 
 hipError_t launch_opw_grid_valu(const EnsembleArgs& p, const float* x, const float* z, const float* y, int nx,
-                                int nz, int ny, int signal_storage, bool apertures) {
+                                int nz, int ny, int signal_storage, bool apertures, int q_begin, int q_end,
+                                bool accumulate) {
   if (p.na > kGridMaxAngles || p.nt > 16 * kMaxPairs) return hipErrorNotSupported;
   if (static_cast<long long>(p.ns) * p.nt > 0x7fffffffLL) return hipErrorNotSupported;
   if (signal_storage < RCABEAM_STORAGE_C64 || signal_storage > RCABEAM_STORAGE_BF16) return hipErrorInvalidValue;
   if (apertures && !p.signal) return hipErrorInvalidValue;  // apertures: complex64 planes whatever signal_storage
-  const GridArgs a = make_grid_args(p, x, z, y, nx, nz, ny, signal_storage);
+  if (accumulate && (apertures || !p.signal || signal_storage != RCABEAM_STORAGE_C64)) return hipErrorInvalidValue;
+  GridArgs a = make_grid_args(p, x, z, y, nx, nz, ny, signal_storage);
+  if (q_end >= 0) {
+    a.q_begin = q_begin;
+    a.q_end = q_end;
+  }
+  a.accumulate = accumulate ? 1 : 0;
   return apertures ? launch_frames<true>(a) : launch_frames<false>(a);
 }
 

@@ -67,8 +67,12 @@ hipError_t launch_opw_grid_mfma(const EnsembleArgs& p, const float* x, const flo
 // Same results as launch_opw_grid up to float32 summation order;
 // hipErrorNotSupported for na > 16, nt > 208, odd nt or unaligned channels.
 // apertures: separate RC and CR IQ into signal (voxels, 2, nt) complex64.
+// q_begin / q_end (q_end >= 0) restrict the sums to channels [q_begin, q_end)
+// of the 2 nc (RC then CR); accumulate adds them to the complex64 signal
+// already there (the power is that of the total).
 hipError_t launch_opw_grid_valu(const EnsembleArgs& p, const float* x, const float* z, const float* y, int nx,
-                                int nz, int ny, int signal_storage, bool apertures);
+                                int nz, int ny, int signal_storage, bool apertures, int q_begin = 0, int q_end = -1,
+                                bool accumulate = false);
 
 // Adjoint of the grid OPW for complex64 channels (das_tile.hip): accumulates
 // the packed channel gradients grc / gcr and dL/dc (*gc) for the IQ gradient

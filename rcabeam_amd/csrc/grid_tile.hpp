@@ -112,6 +112,8 @@ struct GridArgs {
   int max_rows, out_storage, vec, stride;
   double c, fs, fd, fn;
   const unsigned* absmax;  // split FP16 channels: bits of the joint max |component| (scale exponent)
+  int q_begin, q_end;      // channels [q_begin, q_end) of the 2 nc (RC then CR) take part
+  int accumulate;          // VALU kernel, complex64 signal: add the sums to the signal already there
 };
 
 // Grid arguments from the ensemble arguments (pd, signal and channels as
@@ -144,6 +146,9 @@ inline GridArgs make_grid_args(const EnsembleArgs& e, const float* x, const floa
   a.na = e.na;
   a.nt = e.nt;
   a.out_storage = out_storage;
+  a.q_begin = 0;
+  a.q_end = 2 * e.nc;
+  a.accumulate = 0;
   a.c = e.c;
   a.fs = e.fs;
   a.fd = e.fd;
@@ -238,6 +243,7 @@ struct GridTile {
           const double dv = coord(cfg == 0, pos) - e;  // same test as receive_term
           on = on || !(p.fn > 0.0) || fabs(dv) <= zv / (2.0 * p.fn);
         }
+        on = on && q >= p.q_begin && q < p.q_end;
       }
       const uint64_t mask = __ballot(on);
       if (lane == 0 && q0 + kWave * wave < nq) live[(q0 >> 6) + wave] = mask;

@@ -66,7 +66,17 @@ void parallel_copy(unsigned char* dst, const unsigned char* src, size_t n, int t
 
 using namespace rcb;
 
+static rcabeam_status upload(const void* host, void* device, size_t bytes, int threads, bool ordered);
+
 extern "C" rcabeam_status rcabeam_upload_host(const void* host, void* device, size_t bytes, int threads) {
+  return upload(host, device, bytes, threads, true);
+}
+
+extern "C" rcabeam_status rcabeam_upload_host_concurrent(const void* host, void* device, size_t bytes, int threads) {
+  return upload(host, device, bytes, threads, false);
+}
+
+static rcabeam_status upload(const void* host, void* device, size_t bytes, int threads, bool ordered) {
   if (!bytes) return RCABEAM_SUCCESS;
   if (!host || !device || threads < 1) return RCABEAM_ERROR_ARGUMENT;
   int dev = 0;
@@ -79,8 +89,11 @@ extern "C" rcabeam_status rcabeam_upload_host(const void* host, void* device, si
   auto* dst = static_cast<unsigned char*>(device);
   // The copies run on a non-blocking stream: order them after everything already enqueued on the
   // null stream (where every other entry point launches), which may still read or write `device`.
-  hipError_t e = hipEventRecord(s.ordered, nullptr);
-  if (e == hipSuccess) e = hipStreamWaitEvent(s.stream, s.ordered, 0);
+  hipError_t e = hipSuccess;
+  if (ordered) {
+    e = hipEventRecord(s.ordered, nullptr);
+    if (e == hipSuccess) e = hipStreamWaitEvent(s.stream, s.ordered, 0);
+  }
   for (size_t off = 0, k = 0; off < bytes && e == hipSuccess; off += kChunkBytes, ++k) {
     const size_t n = std::min(kChunkBytes, bytes - off);
     const int b = static_cast<int>(k % kStagingBuffers);

@@ -333,20 +333,25 @@ def _run(rc, cr, angles, t_start, grid, geom, method, k, return_signal, timings,
     weights = dev.empty((nv,), t.float32) if method == "st_sw" else None
     planes = 2 if method == "apertures" else 1
     signal = dev.empty((nv * planes, nt), t.complex64) if return_signal else None
-    dev.synchronize()
-    begin = perf_counter()
-    raw_rc = dev.to_device(rc, t.complex64)
-    raw_cr = dev.to_device(cr, t.complex64)
-    dev.synchronize()
-    upload = perf_counter() - begin if host_input else 0.0
-    if _per_angle_ok(method, iq_storage, na, nt, k):
-        run_per_angle(raw_rc, raw_cr, g, (ns, nc, na, nt), code, k, pd, signal, weights)
-        del raw_rc, raw_cr
-    elif method == "dmas" and iq_storage == "float32" and na <= GRID_MAX_ANGLES and nt <= GRID_MAX_FRAMES:
-        run_dmas_grid(raw_rc, raw_cr, g, (ns, nc, na, nt), pd)
-        del raw_rc, raw_cr
+    if (host_input and method == "opw" and iq_storage == "float32" and na <= GRID_MAX_ANGLES
+            and nt <= GRID_MAX_FRAMES and nt % 2 == 0):
+        opw_signal = signal if signal is not None else dev.empty((nv, nt), t.complex64)
+        upload = _opw_host_overlapped(rc, cr, g, (ns, nc, na, nt), pd, opw_signal)
     else:
-        _run_packed(raw_rc, raw_cr, g, (ns, nc, na, nt), method, code, k, iq_storage, pd, signal, weights)
+        dev.synchronize()
+        begin = perf_counter()
+        raw_rc = dev.to_device(rc, t.complex64)
+        raw_cr = dev.to_device(cr, t.complex64)
+        dev.synchronize()
+        upload = perf_counter() - begin if host_input else 0.0
+        if _per_angle_ok(method, iq_storage, na, nt, k):
+            run_per_angle(raw_rc, raw_cr, g, (ns, nc, na, nt), code, k, pd, signal, weights)
+            del raw_rc, raw_cr
+        elif method == "dmas" and iq_storage == "float32" and na <= GRID_MAX_ANGLES and nt <= GRID_MAX_FRAMES:
+            run_dmas_grid(raw_rc, raw_cr, g, (ns, nc, na, nt), pd)
+            del raw_rc, raw_cr
+        else:
+            _run_packed(raw_rc, raw_cr, g, (ns, nc, na, nt), method, code, k, iq_storage, pd, signal, weights)
     dev.synchronize()
     if timings is not None:
         timings["raw_upload_seconds"] = upload
@@ -359,6 +364,37 @@ def _run(rc, cr, angles, t_start, grid, geom, method, k, return_signal, timings,
         peak = weights.max()
         power = power * (weights / peak) if float(peak) > 0 else power * 0
     return to_host(power.reshape(shape)), None
+
+
+def _opw_host_overlapped(rc, cr, g: _Geometry, shape4, pd, signal) -> float:
+    """OPW of host channels with the CR upload overlapping the RC delay-and-sum (no reference counterpart:
+    the reference uploads both apertures before reconstructing, bindings/python.cpp:527-531).
+
+    RC: staged upload, pack, grid pass over the RC channels into `signal`; CR: staged upload on the
+    library's copy stream while that pass runs, pack, grid pass over the CR channels added to `signal`
+    with the power of the total into `pd`.  Returns the host wall time of the two uploads."""
+    t = dev.torch()
+    ns, nc, na, nt = shape4
+    hosts = [np.ascontiguousarray(x, dtype=np.complex64) for x in (rc, cr)]
+    raw = [dev.empty(shape4, t.complex64) for _ in range(2)]
+    packed = [dev.empty((nc, na, ns, nt), t.complex64) for _ in range(2)]
+    threads = dev.upload_threads()
+    nx, nz, ny = g.shape
+    c, fs, fd, fn = g.constants
+    dev.synchronize()  # from here on no queued work touches raw[1]
+    seconds = 0.0
+    for aperture in range(2):
+        begin = perf_counter()
+        name = "rcabeam_upload_host" if aperture == 0 else "rcabeam_upload_host_concurrent"
+        _lib.call(name, hosts[aperture].ctypes.data, dev.ptr(raw[aperture]), hosts[aperture].nbytes, threads)
+        seconds += perf_counter() - begin
+        pack_channels(raw[aperture], "float32", packed=packed[aperture])
+        _lib.call(
+            "rcabeam_opw_grid_aperture_device", dev.ptr(packed[0]), dev.ptr(packed[1]), dev.ptr(g.axes[0]),
+            dev.ptr(g.axes[1]), dev.ptr(g.axes[2]), nx, nz, ny, dev.ptr(g.xe), dev.ptr(g.ye), dev.ptr(g.angles),
+            dev.ptr(g.starts), dev.ptr(pd), dev.ptr(signal), aperture, aperture, ns, nc, na, nt, c, fs, fd, fn,
+        )
+    return seconds
 
 
 def _run_packed(raw_rc, raw_cr, g, shape4, method, code, k, iq_storage, pd, signal, weights):
