@@ -617,14 +617,22 @@ def adjoint_timing(reps: int = 3) -> dict:
                                                *common, dev.ptr(pd), dev.ptr(iq), 0, 0, ns, nc, na, nt,
                                                c, fs, fd, fn), "forward")
 
-    def adjoint():
+    def adjoint_atomic():
         _lib.check(lib.rcabeam_opw_adjoint_grid_device(dev.ptr(packed[0]), dev.ptr(packed[1]), *axes, nx, nz, ny,
                                                        *common, dev.ptr(grad_iq), dev.ptr(grads[0]),
                                                        dev.ptr(grads[1]), dev.ptr(grad_c), ns, nc, na, nt,
                                                        c, fs, fd, fn), "adjoint")
 
+    nbytes = int(lib.rcabeam_opw_adjoint_workspace_bytes(nx, nz, ny, ns, nc, na, nt))
+    work = dev.empty(((nbytes + 7) // 8,), torch.float64)
+
+    def adjoint():
+        _lib.check(lib.rcabeam_opw_adjoint_grid_deterministic_device(
+            dev.ptr(packed[0]), dev.ptr(packed[1]), *axes, nx, nz, ny, *common, dev.ptr(grad_iq), dev.ptr(grads[0]),
+            dev.ptr(grads[1]), dev.ptr(grad_c), dev.ptr(work), nbytes, ns, nc, na, nt, c, fs, fd, fn), "adjoint")
+
     times = {}
-    for name, fn_ in (("forward", forward), ("adjoint", adjoint)):
+    for name, fn_ in (("forward", forward), ("adjoint", adjoint), ("adjoint_atomic", adjoint_atomic)):
         fn_()
         torch.cuda.synchronize()
         ms = []
@@ -641,6 +649,8 @@ def adjoint_timing(reps: int = 3) -> dict:
         "forward_ms": times["forward"],
         "adjoint_ms": times["adjoint"],
         "ratio": times["adjoint"] / times["forward"],
+        "adjoint": "deterministic (fixed-point gradient sums; the autograd backward)",
+        "adjoint_float_atomics_ms": times["adjoint_atomic"],
     }
 
 

@@ -163,6 +163,22 @@ rcabeam_status rcabeam_opw_adjoint_grid_device(const void* rc, const void* cr, c
                                                size_t ns, size_t nc, size_t na, size_t nt, float c, float fs,
                                                float fd, float fn);
 
+/* The same adjoint, bit-identical from run to run: each workgroup's gradient
+ * sums are added as 64-bit fixed-point integers (multiples of 2^-e, e from
+ * max |grad_iq| so that no sample's total can overflow; integer addition does
+ * not depend on the order of the workgroups), converted to float32 and added
+ * to grad_rc / grad_cr at the end, and dL/dc is summed from per-workgroup
+ * partials in a fixed order.  workspace: 256-byte aligned, at least
+ * rcabeam_opw_adjoint_workspace_bytes(nx, nz, ny, ns, nc, na, nt) bytes (the
+ * fixed-point planes: 32 bytes per packed channel sample of both apertures). */
+size_t rcabeam_opw_adjoint_workspace_bytes(size_t nx, size_t nz, size_t ny, size_t ns, size_t nc, size_t na,
+                                           size_t nt);
+rcabeam_status rcabeam_opw_adjoint_grid_deterministic_device(
+    const void* rc, const void* cr, const float* x, const float* z, const float* y, size_t nx, size_t nz, size_t ny,
+    const float* xe, const float* ye, const float* angles, const float* starts, const void* grad_iq, void* grad_rc,
+    void* grad_cr, double* grad_c, void* workspace, size_t workspace_bytes, size_t ns, size_t nc, size_t na,
+    size_t nt, float c, float fs, float fd, float fn);
+
 /* Adds to *out (a device counter the caller zeroes) the number of
  * (voxel, aperture, channel, angle) taps inside the dynamic aperture and the
  * closed sample window: the work the ensemble kernels perform.  Used for

@@ -58,6 +58,9 @@ _SIGNATURES: dict[str, list] = {
     + [c_float] * 4,
     "rcabeam_opw_adjoint_grid_device": [c_void_p, c_void_p] + [c_void_p] * 3 + [c_size_t] * 3 + [c_void_p] * 8
     + [c_size_t] * 4 + [c_float] * 4,
+    "rcabeam_opw_adjoint_workspace_bytes": [c_size_t] * 7,
+    "rcabeam_opw_adjoint_grid_deterministic_device": [c_void_p, c_void_p] + [c_void_p] * 3 + [c_size_t] * 3
+    + [c_void_p] * 8 + [c_void_p, c_size_t] + [c_size_t] * 4 + [c_float] * 4,
     "rcabeam_pack_ensemble_lowrank_device": [c_void_p, c_void_p, c_int, c_void_p, c_size_t, c_void_p] + [c_size_t] * 4,
     "rcabeam_lowrank_expand_device": [c_void_p, c_void_p, c_size_t, c_void_p, c_int, c_void_p, c_size_t, c_size_t],
     "rcabeam_count_active_taps_device": [c_void_p] * 5 + [c_size_t] * 4 + [c_float] * 4 + [c_void_p],
@@ -112,6 +115,7 @@ _RESTYPE = {
     "rcabeam_gram_workspace_bytes": c_size_t,
     "rcabeam_gram_real_elements": c_size_t,
     "rcabeam_gram_f64_workspace_bytes": c_size_t,
+    "rcabeam_opw_adjoint_workspace_bytes": c_size_t,
     "rcabeam_simulate_lds_bytes": c_size_t,
     "rcabeam_two_stage_buffer_elements": c_size_t,
     "rcabeam_angle_accumulator_fields": c_size_t,

@@ -2,8 +2,9 @@
 
 `opw_iq(rc, cr, angles, t_start, grid, geom, sound_speed=...)` returns the
 OPW IQ volume `(nx, nz, ny, frames)` (complex64, on the GPU) as an autograd
-node.  Its backward pass is `rcabeam_opw_adjoint_grid_device`: the transpose
-of the delay-and-sum applied to the IQ gradient, giving the gradients of a
+node.  Its backward pass is `rcabeam_opw_adjoint_grid_deterministic_device`:
+the transpose of the delay-and-sum applied to the IQ gradient (fixed-point
+accumulation, bit-identical from run to run), giving the gradients of a
 real loss with respect to the complex64 RC/CR channel IQ and the scalar sound
 speed.  Tap weights and phases are recomputed in the backward pass from the
 geometry (nothing but the packed channels is kept between the passes), frame
@@ -61,11 +62,14 @@ def _function():
             grad = grad_iq.to(t.complex64).contiguous().view(g.nv, nt)
             g_packed = [t.zeros((nc, na, ns, nt), dtype=t.complex64, device=grad.device) for _ in range(2)]
             g_c = t.zeros((1,), dtype=t.float64, device=grad.device)
+            lib = _lib.load()
+            nbytes = int(lib.rcabeam_opw_adjoint_workspace_bytes(nx, nz, ny, ns, nc, na, nt))
+            work = dev.empty(((nbytes + 7) // 8,), t.float64)
             _lib.call(
-                "rcabeam_opw_adjoint_grid_device", dev.ptr(packed_rc), dev.ptr(packed_cr), dev.ptr(g.axes[0]),
-                dev.ptr(g.axes[1]), dev.ptr(g.axes[2]), nx, nz, ny, dev.ptr(g.xe), dev.ptr(g.ye), dev.ptr(g.angles),
-                dev.ptr(g.starts), dev.ptr(grad), dev.ptr(g_packed[0]), dev.ptr(g_packed[1]), dev.ptr(g_c),
-                ns, nc, na, nt, ctx.c, fs, fd, fn,
+                "rcabeam_opw_adjoint_grid_deterministic_device", dev.ptr(packed_rc), dev.ptr(packed_cr),
+                dev.ptr(g.axes[0]), dev.ptr(g.axes[1]), dev.ptr(g.axes[2]), nx, nz, ny, dev.ptr(g.xe), dev.ptr(g.ye),
+                dev.ptr(g.angles), dev.ptr(g.starts), dev.ptr(grad), dev.ptr(g_packed[0]), dev.ptr(g_packed[1]),
+                dev.ptr(g_c), dev.ptr(work), nbytes, ns, nc, na, nt, ctx.c, fs, fd, fn,
             )
             out = []
             for gp in g_packed:

@@ -865,3 +865,39 @@ extern "C" rcabeam_status rcabeam_opw_adjoint_grid_device(const void* rc, const 
   if (e == hipErrorNotSupported) return RCABEAM_ERROR_ARGUMENT;
   return to_status(e);
 }
+
+extern "C" size_t rcabeam_opw_adjoint_workspace_bytes(size_t nx, size_t nz, size_t ny, size_t ns, size_t nc,
+                                                      size_t na, size_t nt) {
+  if (!nx || !nz || !ny || !ns || !nc || !na || !nt || nx > 0x7fffffff || nz > 0x7fffffff || ny > 0x7fffffff)
+    return 0;
+  EnsembleArgs p{};
+  p.ns = static_cast<int>(ns);
+  p.nc = static_cast<int>(nc);
+  p.na = static_cast<int>(na);
+  p.nt = static_cast<int>(nt);
+  return adjoint_workspace_bytes(p, static_cast<int>(nx), static_cast<int>(nz), static_cast<int>(ny));
+}
+
+extern "C" rcabeam_status rcabeam_opw_adjoint_grid_deterministic_device(
+    const void* rc, const void* cr, const float* x, const float* z, const float* y, size_t nx, size_t nz, size_t ny,
+    const float* xe, const float* ye, const float* angles, const float* starts, const void* grad_iq, void* grad_rc,
+    void* grad_cr, double* grad_c, void* workspace, size_t workspace_bytes, size_t ns, size_t nc, size_t na,
+    size_t nt, float c, float fs, float fd, float fn) {
+  if (!x || !z || !y || !nx || !nz || !ny || nx > 0x7fffffff || nz > 0x7fffffff || ny > 0x7fffffff)
+    return RCABEAM_ERROR_ARGUMENT;
+  if (!grad_iq || !grad_rc || !grad_cr || !grad_c || !workspace || reinterpret_cast<uintptr_t>(workspace) % 256)
+    return RCABEAM_ERROR_ARGUMENT;
+  const size_t nv = nx * nz * ny;
+  float dummy_pd = 0.f;  // validation only: pd is not written by the adjoint
+  if (!valid_ensemble(rc, cr, x, xe, ye, angles, starts, &dummy_pd, ns, nc, na, nt, nv, c, fs, fd, fn))
+    return RCABEAM_ERROR_ARGUMENT;
+  if (workspace_bytes < rcabeam_opw_adjoint_workspace_bytes(nx, nz, ny, ns, nc, na, nt)) return RCABEAM_ERROR_ARGUMENT;
+  EnsembleArgs p = make_args(rc, cr, nullptr, xe, ye, angles, starts, nullptr, nullptr, nullptr, nullptr, ns, nc, na,
+                             nt, nv, c, fs, fd, fn);
+  const hipError_t e = launch_opw_adjoint_grid(p, x, z, y, static_cast<int>(nx), static_cast<int>(nz),
+                                               static_cast<int>(ny), static_cast<const float2*>(grad_iq),
+                                               static_cast<float2*>(grad_rc), static_cast<float2*>(grad_cr), grad_c,
+                                               workspace);
+  if (e == hipErrorNotSupported) return RCABEAM_ERROR_ARGUMENT;
+  return to_status(e);
+}

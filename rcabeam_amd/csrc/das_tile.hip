@@ -463,7 +463,13 @@ hipError_t launch_any(const EnsembleArgs& p, const TileMap& map, long long tiles
 // gradient window: it takes its 4 voxels one after another, every lane adding
 // to its own frame column with a plain read-modify-write (no LDS atomics, no
 // address conflicts).  Per (channel, angle) chunk the 16 windows are summed
-// and added to the packed gradient planes with global float atomics.
+// and added to the packed gradient planes with global float atomics, or, in
+// the deterministic form, as 64-bit fixed-point integers: integer addition is
+// associative, so the order in which workgroups add does not matter.  Each
+// window sum is rounded to a multiple of 2^-e, e chosen from max |grad IQ| so
+// that no sample's total reaches 2^63, and a final pass converts the integers
+// to float32 and adds them to the gradients.  dL/dc then goes to one partial
+// per workgroup, summed in a fixed order.
 struct AdjointArgs {
   EnsembleArgs p;     // rc, cr: packed complex64 channels of the forward pass
   const float2* giq;  // (nv, nt) gradient of the loss w.r.t. IQ
@@ -471,7 +477,23 @@ struct AdjointArgs {
   float2* gcr;
   double* gc;         // accumulated dL/dc
   int f0, nf;         // frame chunk [f0, f0 + nf) of this launch, nf <= 32
+  unsigned long long* fixed;  // deterministic form: (re, im) int64 per gradient sample, RC planes then CR
+  const unsigned* absmax;     // deterministic form: bits of max |component| of giq
+  double* dc_partials;        // deterministic form: one dL/dc partial per workgroup of this launch
 };
+
+// Fixed-point exponent of the deterministic adjoint: every per-sample total is
+// a sum of at most nv contributions of modulus <= |g| <= sqrt(2) m, so with
+// m < 2^ex and nv <= 2^ln, 2^e nv sqrt(2) m < 2^62 for e = 61 - ln - ex.
+__device__ __forceinline__ int fixed_exponent(const unsigned* absmax, long long nv) {
+  const float m = __uint_as_float(absmax[0]);
+  if (!(m > 0.f) || !(m < __builtin_inff())) return 0;
+  int ex = 0;
+  frexpf(m, &ex);
+  int ln = 0;
+  while ((1ll << ln) < nv) ++ln;
+  return 61 - ln - ex;
+}
 
 constexpr int kHalf = kWave / 2;                      // frame columns of a half-wave
 constexpr int kGroups = kThreads / kHalf;             // private gradient windows per workgroup
@@ -513,6 +535,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
   }
   for (int i = tid; i < kGroups * max_rows * kHalf; i += kThreads) win_g[i] = make_float2(0.f, 0.f);
   float2* my_g = win_g + group * max_rows * kHalf + t;
+  const int fe = args.fixed ? fixed_exponent(args.absmax, p.nv) : 0;
+  const size_t aperture_size = static_cast<size_t>(p.nc) * p.na * p.ns * p.nt;
   const float fs = static_cast<float>(p.fs);
   const float omega = static_cast<float>(2.0 * 3.14159265358979323846 * p.fd);
   double dc = 0.0;
@@ -593,6 +617,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
         __syncthreads();
         // Sum the 16 private windows, add to HBM, clear for the next chunk.
         float2* gdst = gplane0 + a * plane_size + static_cast<size_t>(r0) * p.nt + args.f0;
+        const size_t fixed_base =
+            cfg * aperture_size + (static_cast<size_t>(ch) * p.na + a) * plane_size + static_cast<size_t>(r0) * p.nt +
+            args.f0;
         for (int i = tid; i < rows * kHalf; i += kThreads) {
           const int row = i / kHalf;
           const int col = i - row * kHalf;
@@ -606,8 +633,16 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
             *cell = make_float2(0.f, 0.f);
           }
           if (col < args.nf) {
-            if (sum.x != 0.f) unsafeAtomicAdd(&gdst[row * p.nt + col].x, sum.x);
-            if (sum.y != 0.f) unsafeAtomicAdd(&gdst[row * p.nt + col].y, sum.y);
+            if (args.fixed) {
+              unsigned long long* cell = args.fixed + 2 * (fixed_base + static_cast<size_t>(row) * p.nt + col);
+              if (sum.x != 0.f)
+                atomicAdd(cell, static_cast<unsigned long long>(llrint(ldexp(static_cast<double>(sum.x), fe))));
+              if (sum.y != 0.f)
+                atomicAdd(cell + 1, static_cast<unsigned long long>(llrint(ldexp(static_cast<double>(sum.y), fe))));
+            } else {
+              if (sum.x != 0.f) unsafeAtomicAdd(&gdst[row * p.nt + col].x, sum.x);
+              if (sum.y != 0.f) unsafeAtomicAdd(&gdst[row * p.nt + col].y, sum.y);
+            }
           }
         }
         if (last) break;
@@ -623,8 +658,41 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
   if (tid == 0) {
     double sum = 0.0;
     for (int i = 0; i < kWaves; ++i) sum += partial[i];
-    unsafeAtomicAdd(args.gc, sum);
+    if (args.dc_partials)
+      args.dc_partials[blockIdx.x] = sum;
+    else
+      unsafeAtomicAdd(args.gc, sum);
   }
+}
+
+// Zero n 64-bit words (fixed-point planes and dL/dc partials) before a deterministic adjoint.
+__global__ __launch_bounds__(256) void k_zero_words(unsigned long long* __restrict__ w, long long n) {
+  const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) w[i] = 0ull;
+}
+
+// grad (RC planes then CR planes, float2) += fixed-point totals * 2^-e.
+__global__ __launch_bounds__(256) void k_fixed_finalize(const unsigned long long* __restrict__ fixed,
+                                                        const unsigned* __restrict__ absmax, long long nv,
+                                                        long long per_aperture, float2* __restrict__ grc,
+                                                        float2* __restrict__ gcr) {
+  const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= 2 * per_aperture) return;
+  const int fe = fixed_exponent(absmax, nv);
+  const double re = ldexp(static_cast<double>(static_cast<long long>(fixed[2 * i])), -fe);
+  const double im = ldexp(static_cast<double>(static_cast<long long>(fixed[2 * i + 1])), -fe);
+  float2* g = i < per_aperture ? grc + i : gcr + (i - per_aperture);
+  g->x += static_cast<float>(re);
+  g->y += static_cast<float>(im);
+}
+
+// gc += sum of the dL/dc partials, in index order (one wave: fixed lane order, then a fixed-order butterfly).
+__global__ __launch_bounds__(64) void k_dc_reduce(const double* __restrict__ partials, long long n, double* gc) {
+  double s = 0.0;
+  for (long long i = threadIdx.x; i < n; i += 64) s += partials[i];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, kWave);
+  if (threadIdx.x == 0) *gc += s;
 }
 
 hipError_t launch_adjoint(const AdjointArgs& a, const TileMap& map, long long tiles) {
@@ -661,19 +729,57 @@ hipError_t launch_opw_grid(const EnsembleArgs& p, const float* x, const float* z
 
 namespace rcb {
 
+size_t adjoint_workspace_bytes(const EnsembleArgs& p, int nx, int nz, int ny) {
+  const int nbx = (nx + kPatch - 1) / kPatch, nby = (ny + kPatch - 1) / kPatch;
+  const long long tiles = static_cast<long long>(nbx) * nby * nz;
+  const long long grid = (tiles + 7) / 8 * 8;
+  const long long chunks = (p.nt + kHalf - 1) / kHalf;
+  const long long per_aperture = static_cast<long long>(p.nc) * p.na * p.ns * p.nt;
+  return static_cast<size_t>(256 + (4 * per_aperture + chunks * grid) * 8);
+}
+
 hipError_t launch_opw_adjoint_grid(const EnsembleArgs& p, const float* x, const float* z, const float* y, int nx,
-                                   int nz, int ny, const float2* giq, float2* grc, float2* gcr, double* gc) {
+                                   int nz, int ny, const float2* giq, float2* grc, float2* gcr, double* gc,
+                                   void* workspace) {
   if (p.na > kMaxAngles) return hipErrorNotSupported;
   if (static_cast<long long>(p.ns) * p.nt > 0x7fffffffLL) return hipErrorNotSupported;
   const int nbx = (nx + kPatch - 1) / kPatch, nby = (ny + kPatch - 1) / kPatch;
   const TileMap map{x, z, y, nx, nz, ny, nby, true};
   const long long tiles = static_cast<long long>(nbx) * nby * nz;
+  const long long grid = (tiles + 7) / 8 * 8;
+  const long long per_aperture = static_cast<long long>(p.nc) * p.na * p.ns * p.nt;
+  // Deterministic form: workspace = [absmax (3 words, padded to 256 bytes)][fixed planes][dL/dc partials].
+  unsigned* absmax = nullptr;
+  unsigned long long* fixed = nullptr;
+  double* partials = nullptr;
+  const long long chunks = (p.nt + kHalf - 1) / kHalf;
+  if (workspace) {
+    absmax = static_cast<unsigned*>(workspace);
+    fixed = reinterpret_cast<unsigned long long*>(static_cast<unsigned char*>(workspace) + 256);
+    partials = reinterpret_cast<double*>(fixed + 4 * per_aperture);
+    if (rcabeam_iq_absmax_device(giq, RCABEAM_STORAGE_C64, static_cast<size_t>(p.nv) * p.nt, absmax, 0) !=
+        RCABEAM_SUCCESS)
+      return hipErrorLaunchFailure;
+    const long long words = 4 * per_aperture + chunks * grid;
+    k_zero_words<<<static_cast<unsigned>((words + 255) / 256), 256>>>(fixed, words);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
   // Frame chunks of at most 32: each chunk recomputes the taps (nothing is
   // stored between chunks or passes).
-  for (int f0 = 0; f0 < p.nt; f0 += kHalf) {
-    const AdjointArgs a{p, giq, grc, gcr, gc, f0, std::min(kHalf, p.nt - f0)};
+  for (int f0 = 0, k = 0; f0 < p.nt; f0 += kHalf, ++k) {
+    const AdjointArgs a{p, giq, grc, gcr, gc, f0, std::min(kHalf, p.nt - f0), fixed, absmax,
+                        partials ? partials + k * grid : nullptr};
     const hipError_t e = launch_adjoint(a, map, tiles);
     if (e != hipSuccess) return e;
+  }
+  if (workspace) {
+    k_fixed_finalize<<<static_cast<unsigned>((2 * per_aperture + 255) / 256), 256>>>(fixed, absmax, p.nv,
+                                                                                     per_aperture, grc, gcr);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    k_dc_reduce<<<1, 64>>>(partials, chunks * grid, gc);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
   }
   return hipSuccess;
 }

@@ -77,7 +77,11 @@ hipError_t launch_opw_grid_valu(const EnsembleArgs& p, const float* x, const flo
 // Adjoint of the grid OPW for complex64 channels (das_tile.hip): accumulates
 // the packed channel gradients grc / gcr and dL/dc (*gc) for the IQ gradient
 // giq (nv, nt); p.rc / p.cr hold the forward's packed channels.
+// workspace (optional, adjoint_workspace_bytes): the deterministic form
+// (fixed-point gradient sums, fixed-order dL/dc), bit-identical run to run.
 hipError_t launch_opw_adjoint_grid(const EnsembleArgs& p, const float* x, const float* z, const float* y, int nx,
-                                   int nz, int ny, const float2* giq, float2* grc, float2* gcr, double* gc);
+                                   int nz, int ny, const float2* giq, float2* grc, float2* gcr, double* gc,
+                                   void* workspace = nullptr);
+size_t adjoint_workspace_bytes(const EnsembleArgs& p, int nx, int nz, int ny);
 
 }  // namespace rcb

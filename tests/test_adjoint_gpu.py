@@ -146,3 +146,40 @@ def test_central_finite_differences():
     assert abs(fd_dir - analytic) <= 1e-4 * abs(analytic)
     assert abs(fd_c[1e-7] - dldc) <= 1e-3 * abs(dldc)
     assert np.sign(fd_c[1e-3]) == np.sign(dldc) and abs(fd_c[1e-3] - dldc) <= 0.5 * abs(dldc)
+
+
+def test_adjoint_is_deterministic_and_matches_the_atomic_form():
+    """The autograd backward (fixed-point accumulation, fixed-order dL/dc) gives bit-identical gradients on
+    repeated calls, and agrees with the float-atomic adjoint entry point to float32 summation order
+    (rel L2 1e-6)."""
+    from rcabeam_amd import _device as dev
+    from rcabeam_amd import _lib
+    from rcabeam_amd.autograd import opw_iq
+    from rcabeam_amd.ensemble import _Geometry, pack_channels
+
+    acq, geom = _case(grid_shape=(16, 24, 16), frames=40)
+    rng = np.random.default_rng(8)
+    xr = torch.from_numpy(acq.rc).cuda().requires_grad_(True)
+    xc = torch.from_numpy(acq.cr).cuda().requires_grad_(True)
+    c = torch.tensor(1540.0, dtype=torch.float64, device="cuda", requires_grad=True)
+    iq = opw_iq(xr, xc, acq.angles, acq.t_start, acq.grid, geom, sound_speed=c)
+    y = torch.from_numpy((rng.normal(size=iq.shape) + 1j * rng.normal(size=iq.shape)).astype(np.complex64)).cuda()
+    runs = [torch.autograd.grad(iq, (xr, xc, c), grad_outputs=y, retain_graph=True) for _ in range(3)]
+    for other in runs[1:]:
+        for a, b in zip(runs[0], other):
+            assert torch.equal(a, b)
+    # The float-atomic entry point on the same packed channels (accumulation order differs run to run).
+    g = _Geometry(acq.angles, acq.t_start, acq.grid, geom)
+    ns, nc, na, nt = acq.rc.shape
+    packed = [pack_channels(x.detach(), "float32") for x in (xr, xc)]
+    grads = [torch.zeros((nc, na, ns, nt), dtype=torch.complex64, device="cuda") for _ in range(2)]
+    gc = torch.zeros((1,), dtype=torch.float64, device="cuda")
+    nx, nz, ny = g.shape
+    _, fs, fd, fn = g.constants
+    _lib.call("rcabeam_opw_adjoint_grid_device", dev.ptr(packed[0]), dev.ptr(packed[1]), dev.ptr(g.axes[0]),
+              dev.ptr(g.axes[1]), dev.ptr(g.axes[2]), nx, nz, ny, dev.ptr(g.xe), dev.ptr(g.ye), dev.ptr(g.angles),
+              dev.ptr(g.starts), dev.ptr(y.reshape(g.nv, nt).contiguous()), dev.ptr(grads[0]), dev.ptr(grads[1]),
+              dev.ptr(gc), ns, nc, na, nt, 1540.0, fs, fd, fn)
+    unpacked = grads[0].permute(2, 0, 1, 3).contiguous()  # (nc, na, ns, nt) -> (ns, nc, na, nt)
+    assert rel_l2(runs[0][0].cpu().numpy(), unpacked.cpu().numpy()) < 1e-6
+    assert abs(float(runs[0][2]) - float(gc[0])) <= 1e-6 * abs(float(gc[0]))
