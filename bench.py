@@ -553,6 +553,52 @@ def two_stage_timing(scene, rc, cr, reps: int = 2) -> dict:
     return out
 
 
+def rf_timing(reps: int = 3) -> dict:
+    """RF -> packed IQ demodulation of one aperture at the full-size acquisition: int16 RF (3072 samples at
+    60 MHz = 3 x the 20 MHz IQ rate, 128 channels, 11 angles, 200 frames, seeded), 63-tap low-pass, carrier
+    15 MHz, straight into the packed (channels, angles, samples, frames) complex64 layout the delay-and-sum
+    reads.  HBM roof: the RF read once plus the packed IQ written once."""
+    import torch
+
+    from rcabeam_amd import _device as dev
+    from rcabeam_amd import _lib
+    from rcabeam_amd.demod import lowpass_taps
+
+    ns_rf, nc, na, nt, dec = 3072, 128, 11, 200, 3
+    gen = torch.Generator(device="cuda").manual_seed(42)
+    rf = torch.randint(-2000, 2000, (ns_rf, nc, na, nt), dtype=torch.int16, device="cuda", generator=gen)
+    taps = dev.to_device(lowpass_taps(63, 8e6, 60e6), torch.float32)
+    ns_iq = ns_rf // dec
+    out = dev.empty((nc, na, ns_iq, nt), torch.complex64)
+
+    def run():
+        _lib.call("rcabeam_rf_to_iq_device", dev.ptr(rf), 1, ns_rf, nc, na, nt, dev.ptr(taps), 63, dec, 60e6, 15e6,
+                  0.0, dev.ptr(out), ns_iq, 1)
+
+    run()
+    torch.cuda.synchronize()
+    ms = []
+    for _ in range(reps):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        run()
+        b.record()
+        torch.cuda.synchronize()
+        ms.append(a.elapsed_time(b))
+    t = float(np.median(ms))
+    nbytes = rf.numel() * 2 + out.numel() * 8
+    host = rf.cpu().numpy()
+    torch.cuda.synchronize()
+    begin = time.perf_counter()
+    dev.upload(host, torch.int16)
+    torch.cuda.synchronize()
+    upload = time.perf_counter() - begin
+    return {"workload": "one aperture: int16 RF 3072x128x11x200 at 60 MHz -> packed complex64 IQ 1024 samples, "
+                        "63 taps, decimation 3", "kernel_ms": t, "bytes": nbytes, "achieved_GBps": nbytes / t / 1e6,
+            "peak_GBps": HBM_PEAK_GBS, "frac": nbytes / t / 1e6 / HBM_PEAK_GBS,
+            "host_rf_upload_GBps": host.nbytes / upload / 1e9, "host_rf_bytes": host.nbytes}
+
+
 def ingest_timing(rc) -> dict:
     """Host-to-device upload of one aperture's complex64 channels (pageable NumPy array): torch's
     pageable copy against the library's pinned staging (rcabeam_upload_host), best of 3 each."""
@@ -895,6 +941,8 @@ def main() -> int:
         record["combiners_64el"] = combiner_timing_64el()
         log("bench: host ingest")
         record["host_ingest"] = ingest_timing(rc)
+    log("bench: RF to IQ demodulation")
+    record["rf_to_iq"] = rf_timing()
     if not args.no_two_stage:
         log("bench: two-stage approximation at full size")
         record["two_stage"] = two_stage_timing(scene, rc, cr)
