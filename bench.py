@@ -741,7 +741,7 @@ def stage_roofs(pipe, stage_ms: dict, das_flops: float) -> dict:
             rows = pipe.c_rows * 2
             entry.update(bound="mfma", note="every 4th channel row of both apertures: split-bfloat16 Gram "
                          "(4 products) + float64 EVD of the 200 x 200 channel Gram",
-                         flops=8.0 * nt * nt * rows * 4)
+                         flops=4.0 * nt * (nt + 1) * rows * 4)
         elif name == "pack":
             entry.update(hbm(2 * (raw + packed), ms))
             entry["note"] = "both apertures: complex64 channels read once, packed layout written once"
@@ -756,7 +756,8 @@ def stage_roofs(pipe, stage_ms: dict, das_flops: float) -> dict:
             entry.update(hbm(nv * ntc * 8 + nv * nt * 8, ms))
             entry["note"] = "coded IQ read once, complex64 IQ written once"
         elif name == "gram":
-            flops = 8.0 * nt * nt * nv
+            # Hermitian minimum: T (T + 1) / 2 complex multiply-adds (8 flop) per Casorati row.
+            flops = 4.0 * nt * (nt + 1) * nv
             if pipe.exact_gram:
                 peak, form = FP64_PEAK_TFLOPS, "exact float64 MFMA (complex64 products exact in float64)"
             elif pipe.code == 0:
@@ -766,7 +767,8 @@ def stage_roofs(pipe, stage_ms: dict, das_flops: float) -> dict:
                 peak, form = BF16_PEAK_TFLOPS, "one exact 16-bit MFMA product"
             entry.update(bound="mfma", flops=flops, achieved_TFLOPs=flops / (ms * 1e9), peak_TFLOPs=peak,
                          frac=flops / (ms * 1e9) / peak, bytes=nv * nt * sample,
-                         note=f"8 T flop per voxel-frame (full complex Gram); {form}; includes the |max| pass")
+                         note=f"4 (T + 1) flop per voxel-frame (upper triangle of the complex Gram); {form}; "
+                         "includes the |max| pass")
         elif name == "evd":
             entry.update(bound="latency", note="200 x 200 complex128 Hermitian EVD (rocSOLVER zheevd) and "
                          "the T x T Hermitian assembly")
