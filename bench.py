@@ -78,6 +78,8 @@ def parse():
                    help="only time the two-stage approximation on the full-size scene (profiling runs)")
     p.add_argument("--combiners-only", action="store_true",
                    help="only time the combiners on the full-size scene (profiling runs)")
+    p.add_argument("--aperture-passes-only", action="store_true",
+                   help="only time the float32 delay-and-sum as its two per-aperture passes (profiling runs)")
     p.add_argument("--cpu-voxels", type=int, default=600,
                    help="full-size voxels at which each form is compared with the float64 oracle (0: skip)")
     p.add_argument("--copy-calibration", action="store_true",
@@ -386,6 +388,42 @@ def host_to_host(pipe, reps: int = 3) -> dict:
             "threads": dev.upload_threads(),
             "note": "sequential: both staged uploads, then the device step; overlapped: the CR upload runs while "
                     "the RC half of the delay-and-sum executes (two aperture passes)"}
+
+
+def aperture_passes(pipe, reps: int = 2) -> dict:
+    """The float32 delay-and-sum split into its two per-aperture grid passes (rcabeam_opw_grid_aperture_device:
+    RC channels, then CR channels added to the RC sums), each timed alone with HIP events on the null stream
+    (best of `reps`), beside one whole-grid launch.  Channels are the step's packed buffers."""
+    import torch
+
+    from rcabeam_amd import _device as dev
+
+    pipe.step()  # packs RC and CR into pipe.packed
+    nx, nz, ny = pipe.g.shape
+    c, fs, fd, fn = pipe.g.constants
+    g = pipe.g
+
+    def one(aperture):
+        pipe._call("rcabeam_opw_grid_aperture_device", dev.ptr(pipe.packed[0]), dev.ptr(pipe.packed[1]),
+                   dev.ptr(g.axes[0]), dev.ptr(g.axes[1]), dev.ptr(g.axes[2]), nx, nz, ny, dev.ptr(g.xe),
+                   dev.ptr(g.ye), dev.ptr(g.angles), dev.ptr(g.starts), dev.ptr(pipe.pd), dev.ptr(pipe.signal),
+                   aperture, aperture, pipe.ns, pipe.nc, pipe.na, pipe.nt, c, fs, fd, fn)
+
+    out = {}
+    for key, aperture in (("rc_pass_ms", 0), ("cr_pass_ms", 1)):
+        best = float("inf")
+        for _ in range(reps):
+            start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            torch.cuda.synchronize()
+            start.record()
+            one(aperture)
+            end.record()
+            torch.cuda.synchronize()
+            best = min(best, start.elapsed_time(end))
+        out[key] = best
+    out["note"] = ("RC pass: the RC channels of every tile (wave-uniform aperture masks); CR pass: the CR channels "
+                   "added to the RC sums, with the power")
+    return out
 
 
 def combiner_timing(scene, rc, cr) -> dict:
@@ -812,7 +850,7 @@ def main() -> int:
               file=sys.stderr)
         return 2
     baseline = None
-    if not (args.no_cpu_baseline or args.two_stage_only or args.combiners_only):
+    if not (args.no_cpu_baseline or args.two_stage_only or args.combiners_only or args.aperture_passes_only):
         log("bench: CPU baseline (float64 NumPy restatement, 64+64-element workload)")
         baseline = cpu_baseline()
     import torch
@@ -833,6 +871,12 @@ def main() -> int:
         return 0
     storage = args.precision
     pipe = Pipeline(scene, rc, cr, storage, rank_start, args.lowrank_modes)
+    if args.aperture_passes_only:
+        if storage != "float32":
+            log("bench: --aperture-passes-only times the float32 form")
+            return 2
+        print(json.dumps({"das_aperture_passes": aperture_passes(pipe)}), flush=True)
+        return 0
     nv, nt = pipe.g.nv, pipe.nt
     counter = dev.empty((1,), torch.int64)
     counter.zero_()
@@ -932,6 +976,9 @@ def main() -> int:
             del mixed, iq
             torch.cuda.empty_cache()
         del reference
+    if storage == "float32":
+        log("bench: delay-and-sum as two per-aperture passes")
+        record["das_aperture_passes"] = aperture_passes(pipe)
     log("bench: host to host (staged upload + step + power download)")
     record["host_to_host"] = host_to_host(pipe)
     log("bench: adjoint / forward on the differentiable-OPW workload")
