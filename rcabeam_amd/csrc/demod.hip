@@ -20,8 +20,20 @@
 // store of a wave is 64 consecutive columns of one sample row.  The packed
 // output is the frame-inner (channels, angles, samples, frames) layout the
 // delay-and-sum kernels read, so demodulation also does the packing.  The
-// kernel is HBM-bound: each RF row is fetched from HBM once and re-read K / D
-// times from cache by the output rows of a block.
+// kernel is HBM-bound in principle (0.5 ms per aperture of the full-size
+// workload); its arithmetic, 2 K FMA per output, is close behind (~0.9 ms).
+//
+// Polyphase form (k_rf_to_iq_poly): with k = p D + q, output m is
+//   sum_q sum_p g[p D + q] y_q[m + p],   y_q[n] = rf[n D + q - c],
+// so a thread that computes R consecutive outputs of its column holds, per
+// phase q, the R + P_q - 1 samples y_q[m0 .. m0 + R + P_q - 2] in registers
+// (P_q = taps of phase q) and uses each of them for up to R outputs: about
+// (R + K / D) D / R loads per output instead of K.  A workgroup is 256
+// consecutive columns and R output rows; work items are dealt to XCDs in
+// contiguous runs, output-row blocks fastest, so the blocks that share RF rows
+// run on one XCD at about the same time and read them from its L2.  The output
+// phasor depends on m only and is formed once per workgroup.  Filters with
+// more than kPolyMaxTaps taps per phase use the direct kernel (k_rf_to_iq).
 #include "common.hpp"
 #include "rcabeam_mi355x.h"
 
@@ -29,8 +41,10 @@ namespace rcb {
 namespace {
 
 constexpr int kDemodThreads = 256;
-constexpr int kDemodRows = 32;  // output rows per workgroup
+constexpr int kDemodRows = 32;  // output rows per workgroup (direct kernel)
 constexpr int kMaxTaps = 255;
+constexpr int kPolyRows = 16;      // consecutive outputs per thread (polyphase kernel)
+constexpr int kPolyMaxTaps = 32;   // most taps per phase the polyphase kernel takes
 
 struct DemodArgs {
   const void* rf;
@@ -102,6 +116,118 @@ __global__ __launch_bounds__(kDemodThreads) void k_rf_to_iq(DemodArgs a) {
   }
 }
 
+// Complex taps g[k] = 2 h[k] exp(-2j pi fd (k - c) / fs) into LDS (cycles reduced in float64).
+__device__ __forceinline__ void form_taps(const DemodArgs& a, float2* g) {
+  const int c = (a.ntaps - 1) / 2;
+  for (int k = threadIdx.x; k < a.ntaps; k += kDemodThreads) {
+    const double cycles = a.fd * static_cast<double>(k - c) / a.fs;
+    const double r = cycles - rint(cycles);
+    float sn, cs;
+    sincospif(static_cast<float>(-2.0 * r), &sn, &cs);
+    const float h2 = 2.0f * a.taps[k];
+    g[k] = make_float2(h2 * cs, h2 * sn);
+  }
+}
+
+// PMAX: taps per phase unrolled (>= every phase's count).  The window loads
+// of a phase are issued together, unconditionally (rows outside the record or
+// past the phase's window read a valid row and are zeroed afterwards; the
+// test is wave-uniform), so a wave keeps R + PMAX - 1 loads in flight.
+template <typename T, int PMAX>
+__global__ __launch_bounds__(kDemodThreads) void k_rf_to_iq_poly(DemodArgs a, long long row_blocks,
+                                                                 long long items) {
+  __shared__ float2 g[kMaxTaps];
+  __shared__ float2 phasor[kPolyRows];
+  constexpr int W = kPolyRows + PMAX - 1;
+  // Work item: (column block, output-row block), row blocks fastest, contiguous per XCD.
+  const long long per_xcd = (items + 7) / 8;
+  const long long item = static_cast<long long>(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+  if (item >= items) return;  // whole workgroup: uniform
+  const long long m0 = (item % row_blocks) * kPolyRows;
+  const long long col = (item / row_blocks) * kDemodThreads + threadIdx.x;
+  form_taps(a, g);
+  if (threadIdx.x < kPolyRows) {
+    // exp(-2j pi fd t_m), t_m = t0 + m D / fs
+    const long long m = m0 + threadIdx.x;
+    const double cycles = a.fd * (a.t0 + static_cast<double>(m * a.dec) / a.fs);
+    const double r = cycles - rint(cycles);
+    float sn, cs;
+    sincospif(static_cast<float>(-2.0 * r), &sn, &cs);
+    phasor[threadIdx.x] = make_float2(cs, sn);
+  }
+  __syncthreads();
+  if (col >= a.cols) return;
+  const T* base = static_cast<const T*>(a.rf) + col;
+  const int c = (a.ntaps - 1) / 2;
+  float2 acc[kPolyRows];
+#pragma unroll
+  for (int r = 0; r < kPolyRows; ++r) acc[r] = make_float2(0.f, 0.f);
+  const long long first = m0 * a.dec - c;  // row of window element 0 of phase 0
+  for (int q = 0; q < a.dec && q < a.ntaps; ++q) {
+    const int taps_q = (a.ntaps - q + a.dec - 1) / a.dec;  // p D + q < K
+    const int n_win = kPolyRows + taps_q - 1;
+    T raw[W];
+    uint64_t valid = 0;  // wave-uniform
+    const long long n0 = first + q;
+    if (n0 >= 0 && n0 + static_cast<long long>(W - 1) * a.dec < a.ns_rf) {
+      // All W rows inside the record: a pointer stepped by D rows.
+      const T* ptr = base + n0 * a.cols;
+      const long long step = static_cast<long long>(a.dec) * a.cols;
+#pragma unroll
+      for (int i = 0; i < W; ++i) {
+        raw[i] = *ptr;
+        ptr += step;
+      }
+      valid = n_win >= 64 ? ~0ull : (1ull << n_win) - 1;
+    } else {
+#pragma unroll
+      for (int i = 0; i < W; ++i) {
+        const long long n = n0 + static_cast<long long>(i) * a.dec;
+        const bool ok = i < n_win && n >= 0 && n < a.ns_rf;
+        valid |= ok ? (1ull << i) : 0ull;
+        raw[i] = base[(ok ? n : 0ll) * a.cols];
+      }
+    }
+    float win[W];
+#pragma unroll
+    for (int i = 0; i < W; ++i) win[i] = (valid >> i) & 1ull ? static_cast<float>(raw[i]) : 0.f;
+#pragma unroll
+    for (int p = 0; p < PMAX; ++p) {
+      if (p < taps_q) {
+        const float2 h = g[p * a.dec + q];
+#pragma unroll
+        for (int r = 0; r < kPolyRows; ++r) {
+          acc[r].x = fmaf(h.x, win[r + p], acc[r].x);
+          acc[r].y = fmaf(h.y, win[r + p], acc[r].y);
+        }
+      }
+    }
+  }
+  const long long plane = col / a.nt, frame = col - plane * a.nt;
+#pragma unroll
+  for (int r = 0; r < kPolyRows; ++r) {
+    const long long m = m0 + r;
+    if (m >= a.ns_iq) break;
+    const float2 e = phasor[r];
+    const float2 v = make_float2(e.x * acc[r].x - e.y * acc[r].y, e.x * acc[r].y + e.y * acc[r].x);
+    const long long index = a.packed ? (plane * a.ns_iq + m) * a.nt + frame : m * a.cols + col;
+    a.out[index] = v;
+  }
+}
+
+template <typename T>
+void launch_poly(const DemodArgs& a, int taps_per_phase, long long row_blocks, long long items) {
+  const unsigned grid = static_cast<unsigned>((items + 7) / 8 * 8);
+  if (taps_per_phase <= 8)
+    k_rf_to_iq_poly<T, 8><<<grid, kDemodThreads>>>(a, row_blocks, items);
+  else if (taps_per_phase <= 16)
+    k_rf_to_iq_poly<T, 16><<<grid, kDemodThreads>>>(a, row_blocks, items);
+  else if (taps_per_phase <= 24)
+    k_rf_to_iq_poly<T, 24><<<grid, kDemodThreads>>>(a, row_blocks, items);
+  else
+    k_rf_to_iq_poly<T, kPolyMaxTaps><<<grid, kDemodThreads>>>(a, row_blocks, items);
+}
+
 }  // namespace
 }  // namespace rcb
 
@@ -122,7 +248,10 @@ extern "C" rcabeam_status rcabeam_rf_to_iq_device(const void* rf, int rf_type, s
   const long long cols = static_cast<long long>(nc) * static_cast<long long>(na) * static_cast<long long>(nt);
   const long long gx = (cols + kDemodThreads - 1) / kDemodThreads;
   const long long gy = (static_cast<long long>(ns_iq) + kDemodRows - 1) / kDemodRows;
-  if (gx > 0x7fffffffLL || gy > 65535) return RCABEAM_ERROR_ARGUMENT;
+  const bool poly = (ntaps + decimation - 1) / decimation <= static_cast<size_t>(kPolyMaxTaps);
+  const long long row_blocks = (static_cast<long long>(ns_iq) + kPolyRows - 1) / kPolyRows;
+  const long long items = gx * row_blocks;
+  if (poly ? (items + 7) / 8 * 8 > 0x7fffffffLL : (gx > 0x7fffffffLL || gy > 65535)) return RCABEAM_ERROR_ARGUMENT;
   DemodArgs a;
   a.rf = rf;
   a.taps = taps;
@@ -137,6 +266,14 @@ extern "C" rcabeam_status rcabeam_rf_to_iq_device(const void* rf, int rf_type, s
   a.t0 = t_start;
   a.fs = fs_rf;
   a.fd = f_demod;
+  if (poly) {
+    const int per_phase = static_cast<int>((ntaps + decimation - 1) / decimation);
+    if (rf_type == 0)
+      launch_poly<float>(a, per_phase, row_blocks, items);
+    else
+      launch_poly<short>(a, per_phase, row_blocks, items);
+    return launch_status();
+  }
   const dim3 grid(static_cast<unsigned>(gx), static_cast<unsigned>(gy));
   if (rf_type == 0)
     k_rf_to_iq<float><<<grid, kDemodThreads>>>(a);

@@ -17,14 +17,16 @@ FS, FD = 60e6, 15e6
 
 
 @pytest.mark.parametrize("rf_type", ["float32", "int16"])
-@pytest.mark.parametrize("ntaps,dec", [(31, 3), (63, 3), (1, 1), (15, 4)])
+@pytest.mark.parametrize("ntaps,dec", [(31, 3), (63, 3), (1, 1), (15, 4), (63, 1)])
 def test_rf_to_iq_matches_oracle(rf_type, ntaps, dec):
+    """Polyphase kernel (taps per phase <= 32) and the direct kernel ((63, 1)); 273 columns (two column
+    blocks, the second partial) and 301 RF samples, so most output blocks touch the record's ends."""
     import torch
 
     from rcabeam_amd.demod import lowpass_taps, rf_to_iq
 
     rng = np.random.default_rng(ntaps + dec)
-    shape = (301, 5, 3, 7)
+    shape = (301, 13, 3, 7)
     rf = rng.normal(size=shape) * (1000 if rf_type == "int16" else 1)
     rf = rf.astype(np.int16 if rf_type == "int16" else np.float32)
     taps = lowpass_taps(ntaps, FS / (2 * dec) * 0.9, FS) if ntaps > 1 else np.ones(1, np.float32)
@@ -35,7 +37,7 @@ def test_rf_to_iq_matches_oracle(rf_type, ntaps, dec):
     assert got.shape == expected.shape and got.dtype == np.complex64
     assert rel_l2(got, expected) < 1e-5
     packed = rf_to_iq(torch.from_numpy(rf).cuda(), FS, FD, dec, taps=taps, t_start=t0, packed=True)
-    assert tuple(packed.shape) == (5, 3, expected.shape[0], 7)
+    assert tuple(packed.shape) == (13, 3, expected.shape[0], 7)
     assert rel_l2(packed.cpu().numpy(), expected.transpose(1, 2, 0, 3)) < 1e-5
 
 
