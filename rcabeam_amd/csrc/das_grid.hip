@@ -8,9 +8,11 @@
 // frames (tiles in XCD-aware 4 x 4 x 4 blocks).  Wave w owns voxel row
 // x = w; the 8 lanes l = 8 y + f of voxel (w, y) own the frame pairs
 // (2 f + 16 j, 2 f + 16 j + 1), j < FPP, so one ds_read_b128 of a wave reads
-// 128 contiguous bytes of one sample row per voxel (no LDS bank conflicts
-// whatever the rows).  Tile tables, analytic windows, live-channel skipping
-// and stage planning are those of grid_tile.hpp; windows of consecutive
+// 128 contiguous bytes of one sample row per voxel.  With TAIL the last block
+// holds at most 8 frames and lane f owns the single frame 16 (FPP - 1) + f
+// (ds_read_b64), so 200 frames cost 200 frame updates per lane group, not
+// 208.  Tile tables, analytic windows, live-channel skipping and stage
+// planning are those of grid_tile.hpp; windows of consecutive
 // angles are packed into stages copied from the packed (channels, angles,
 // samples, frames) layout into LDS rows of nt samples with 1 KiB
 // global_load_lds_dwordx4 instructions, double-buffered across angles and
@@ -18,7 +20,10 @@
 //
 // Per tap and frame: v = a + w (b - a), acc += v e^{i phi} (8 FMA), with the
 // tap (row, weight, phasor) of the lane's own voxel combined from its
-// receive and transmit terms exactly as combine() does.
+// receive and transmit terms exactly as combine() does.  The tap is formed
+// only as far as the branch needs it: an inactive term carries kInactive
+// (-2^30, active ones stay within +/- 2^28), so any inactive term already
+// makes the row negative, and the phasor is formed only for valid taps.
 //
 // Single-angle launches (na == 1, MULTI): a channel has one window, so a
 // stage packs the windows of up to 4 consecutive live channels of one
@@ -73,8 +78,10 @@ struct MultiStage {
   bool valid;
 };
 
-template <int FPP, bool APERTURES, bool MULTI>
+template <int FPP, bool APERTURES, bool MULTI, bool TAIL>
 __global__ __launch_bounds__(kVThreads) __attribute__((amdgpu_waves_per_eu(4))) void das_opw_grid_valu(GridArgs p) {
+  constexpr int kPairs = TAIL ? FPP - 1 : FPP;      // full pair blocks
+  constexpr int kAcc = 2 * kPairs + (TAIL ? 1 : 0);  // frames per lane
   extern __shared__ __align__(16) unsigned char lds[];
   __shared__ __align__(16) float2 stage_a[kVStageWords / 2];
   __shared__ __align__(16) float2 stage_b[kVStageWords / 2];
@@ -141,20 +148,34 @@ __global__ __launch_bounds__(kVThreads) __attribute__((amdgpu_waves_per_eu(4))) 
     }
   };
 
-  float2 acc[2 * FPP];
+  float2 acc[kAcc];
 #pragma unroll
-  for (int j = 0; j < 2 * FPP; ++j) acc[j] = make_float2(0.f, 0.f);
+  for (int j = 0; j < kAcc; ++j) acc[j] = make_float2(0.f, 0.f);
+  // Frame of accumulator k of this lane.
+  auto frame_of = [&](int k) __attribute__((always_inline)) {
+    return (TAIL && k == 2 * kPairs) ? 16 * kPairs + fl : 2 * fl + 16 * (k >> 1) + (k & 1);
+  };
 
   const uint32_t row_bytes = static_cast<uint32_t>(nt) * 8u;
   // One tap over all frames from staged row r0 (0 <= r0 < 2^24 here, so a
   // 24-bit multiply gives its byte offset); row r0 + 1 only where the weight
   // is non-zero.
   auto accumulate = [&](const Tap& tp, uint32_t r0, const float2* base) __attribute__((always_inline)) {
-    const unsigned char* row = reinterpret_cast<const unsigned char*>(base) + __umul24(r0, row_bytes) + 16 * fl;
+    const unsigned char* row0 = reinterpret_cast<const unsigned char*>(base) + __umul24(r0, row_bytes);
+    const unsigned next = tp.w != 0.f ? row_bytes : 0u;
+    const unsigned char* row = row0 + 16 * fl;
     const float4* s0 = reinterpret_cast<const float4*>(row);
-    const float4* s1 = reinterpret_cast<const float4*>(row + (tp.w != 0.f ? row_bytes : 0u));
+    const float4* s1 = reinterpret_cast<const float4*>(row + next);
+    if constexpr (TAIL) {
+      const float2* t0 = reinterpret_cast<const float2*>(row0 + 128 * kPairs + 8 * fl);
+      const float2* t1 = reinterpret_cast<const float2*>(row0 + 128 * kPairs + 8 * fl + next);
+      const float2 va = *t0, vb = *t1;  // frame 16 kPairs + fl
+      const float ax = fmaf(tp.w, vb.x - va.x, va.x), ay = fmaf(tp.w, vb.y - va.y, va.y);
+      acc[2 * kPairs].x = fmaf(tp.c, ax, fmaf(-tp.s, ay, acc[2 * kPairs].x));
+      acc[2 * kPairs].y = fmaf(tp.c, ay, fmaf(tp.s, ax, acc[2 * kPairs].y));
+    }
 #pragma unroll
-    for (int j = 0; j < FPP; ++j) {
+    for (int j = 0; j < kPairs; ++j) {
       const float4 va = s0[8 * j], vb = s1[8 * j];  // frames 2 fl + 16 j, + 1
       const float ax = fmaf(tp.w, vb.x - va.x, va.x), ay = fmaf(tp.w, vb.y - va.y, va.y);
       const float bx = fmaf(tp.w, vb.z - va.z, va.z), by = fmaf(tp.w, vb.w - va.w, va.w);
@@ -184,11 +205,21 @@ __global__ __launch_bounds__(kVThreads) __attribute__((amdgpu_waves_per_eu(4))) 
         row_off = off;
         off += bw.y - bw.x + 1;
       }
-      const Tap tp = tap_of(ch.rx, tile.transmit(ch.cfg, a, tpos), p.ns);
+      const Term tx = tile.transmit(ch.cfg, a, tpos);
+      int i = ch.rx.i + tx.i;
+      float f = ch.rx.f + tx.f;
+      const bool carry = f >= 1.0f;
+      i += carry ? 1 : 0;
+      f = carry ? f - 1.0f : f;
       // A lane-dependent branch around the body (not a `continue`) keeps the
       // angle loop itself wave-uniform.
-      if (tp.i0 >= 0 && tp.i0 >= t_lo && tp.i0 <= t_hi)
-        accumulate(tp, static_cast<uint32_t>(tp.i0 - first + row_off), base);
+      if (i >= 0 && (i < p.ns - 1 || (i == p.ns - 1 && f == 0.0f)) && i >= t_lo && i <= t_hi) {
+        Tap tp;
+        tp.w = f;
+        tp.c = ch.rx.c * tx.c - ch.rx.s * tx.s;
+        tp.s = ch.rx.c * tx.s + ch.rx.s * tx.c;
+        accumulate(tp, static_cast<uint32_t>(i - first + row_off), base);
+      }
     }
   };
 
@@ -196,13 +227,11 @@ __global__ __launch_bounds__(kVThreads) __attribute__((amdgpu_waves_per_eu(4))) 
   const long long v = tile.voxel(wave, vy);
   auto store_plane = [&](int cfg) __attribute__((always_inline)) {
 #pragma unroll
-    for (int j = 0; j < FPP; ++j)
-#pragma unroll
-      for (int e = 0; e < 2; ++e) {
-        const int t = 2 * fl + 16 * j + e;
-        if (t < nt && v >= 0) reinterpret_cast<float2*>(p.signal)[(2 * v + cfg) * nt + t] = acc[2 * j + e];
-        acc[2 * j + e] = make_float2(0.f, 0.f);
-      }
+    for (int k = 0; k < kAcc; ++k) {
+      const int t = frame_of(k);
+      if (t < nt && v >= 0) reinterpret_cast<float2*>(p.signal)[(2 * v + cfg) * nt + t] = acc[k];
+      acc[k] = make_float2(0.f, 0.f);
+    }
   };
   bool rc_stored = false;
 
@@ -376,26 +405,23 @@ __global__ __launch_bounds__(kVThreads) __attribute__((amdgpu_waves_per_eu(4))) 
   }
   float power = 0.f;
 #pragma unroll
-  for (int j = 0; j < FPP; ++j) {
-    const int t = 2 * fl + 16 * j;
-#pragma unroll
-    for (int e = 0; e < 2; ++e) {
-      if (t + e < nt) {
-        float2 s = acc[2 * j + e];
-        if (p.accumulate && v >= 0) {  // complex64 signal holding the sums of other channels
-          const float2 prior = reinterpret_cast<const float2*>(p.signal)[v * nt + t + e];
-          s = make_float2(s.x + prior.x, s.y + prior.y);
-        }
-        power = fmaf(s.x, s.x, fmaf(s.y, s.y, power));
-        if (p.signal && v >= 0) {
-          const long long index = v * nt + t + e;
-          if (p.out_storage == RCABEAM_STORAGE_FP16)
-            store_sample(reinterpret_cast<__half2*>(p.signal) + index, s);
-          else if (p.out_storage == RCABEAM_STORAGE_BF16)
-            store_sample(reinterpret_cast<bf16x2*>(p.signal) + index, s);
-          else
-            store_sample(reinterpret_cast<float2*>(p.signal) + index, s);
-        }
+  for (int k = 0; k < kAcc; ++k) {
+    const int t = frame_of(k);
+    if (t < nt) {
+      float2 s = acc[k];
+      if (p.accumulate && v >= 0) {  // complex64 signal holding the sums of other channels
+        const float2 prior = reinterpret_cast<const float2*>(p.signal)[v * nt + t];
+        s = make_float2(s.x + prior.x, s.y + prior.y);
+      }
+      power = fmaf(s.x, s.x, fmaf(s.y, s.y, power));
+      if (p.signal && v >= 0) {
+        const long long index = v * nt + t;
+        if (p.out_storage == RCABEAM_STORAGE_FP16)
+          store_sample(reinterpret_cast<__half2*>(p.signal) + index, s);
+        else if (p.out_storage == RCABEAM_STORAGE_BF16)
+          store_sample(reinterpret_cast<bf16x2*>(p.signal) + index, s);
+        else
+          store_sample(reinterpret_cast<float2*>(p.signal) + index, s);
       }
     }
   }
@@ -412,7 +438,7 @@ __global__ __launch_bounds__(kVThreads) __attribute__((amdgpu_waves_per_eu(4))) 
   }
 }
 
-template <int FPP, bool APERTURES>
+template <int FPP, bool APERTURES, bool TAIL>
 hipError_t launch_pairs(GridArgs a) {
   a.max_rows = std::min(kVMaxRows, stage_rows(a.nt, FPP));
   if (a.max_rows < kVMinRows) return hipErrorNotSupported;
@@ -425,7 +451,8 @@ hipError_t launch_pairs(GridArgs a) {
   if (a.nt % 2 != 0 || reinterpret_cast<uintptr_t>(a.rc) % 16 || reinterpret_cast<uintptr_t>(a.cr) % 16)
     return hipErrorNotSupported;
   a.vec = 1;
-  auto kernel = multi ? das_opw_grid_valu<FPP, APERTURES, true> : das_opw_grid_valu<FPP, APERTURES, false>;
+  auto kernel =
+      multi ? das_opw_grid_valu<FPP, APERTURES, true, TAIL> : das_opw_grid_valu<FPP, APERTURES, false, TAIL>;
   kernel<<<dim3(static_cast<unsigned>(grid)), kVThreads, bytes>>>(a);
   return hipGetLastError();
 }
@@ -433,11 +460,13 @@ hipError_t launch_pairs(GridArgs a) {
 template <bool APERTURES>
 hipError_t launch_frames(const GridArgs& a) {
   const int pairs = (a.nt + 15) / 16;
-  if (pairs <= 1) return launch_pairs<1, APERTURES>(a);
-  if (pairs <= 2) return launch_pairs<2, APERTURES>(a);
-  if (pairs <= 4) return launch_pairs<4, APERTURES>(a);
-  if (pairs <= 7) return launch_pairs<7, APERTURES>(a);
-  return launch_pairs<kMaxPairs, APERTURES>(a);
+  if (pairs <= 1) return launch_pairs<1, APERTURES, false>(a);
+  if (pairs <= 2) return launch_pairs<2, APERTURES, false>(a);
+  if (pairs <= 4) return launch_pairs<4, APERTURES, false>(a);
+  if (pairs <= 7) return launch_pairs<7, APERTURES, false>(a);
+  // At most 8 frames in the last block: single frames there (200 frames: 12 pair blocks + 8).
+  if (a.nt <= 16 * (kMaxPairs - 1) + 8) return launch_pairs<kMaxPairs, APERTURES, true>(a);
+  return launch_pairs<kMaxPairs, APERTURES, false>(a);
 }
 
 }  // namespace
