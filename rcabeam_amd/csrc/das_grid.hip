@@ -174,15 +174,29 @@ __global__ __launch_bounds__(kVThreads) __attribute__((amdgpu_waves_per_eu(4))) 
       acc[2 * kPairs].x = fmaf(tp.c, ax, fmaf(-tp.s, ay, acc[2 * kPairs].x));
       acc[2 * kPairs].y = fmaf(tp.c, ay, fmaf(tp.s, ax, acc[2 * kPairs].y));
     }
+    // Software-pipelined: the rows of pair block j + 1 are read while block j
+    // is computed (a wave keeps two blocks of LDS reads in flight; the
+    // scheduling barriers stop the compiler from sinking the reads back).
+    float4 va = s0[0], vb = s1[0];  // frames 2 fl, 2 fl + 1
 #pragma unroll
     for (int j = 0; j < kPairs; ++j) {
-      const float4 va = s0[8 * j], vb = s1[8 * j];  // frames 2 fl + 16 j, + 1
+      float4 na, nb;
+      if (j + 1 < kPairs) {
+        na = s0[8 * (j + 1)];
+        nb = s1[8 * (j + 1)];
+      }
+      __builtin_amdgcn_sched_barrier(0);
       const float ax = fmaf(tp.w, vb.x - va.x, va.x), ay = fmaf(tp.w, vb.y - va.y, va.y);
       const float bx = fmaf(tp.w, vb.z - va.z, va.z), by = fmaf(tp.w, vb.w - va.w, va.w);
       acc[2 * j].x = fmaf(tp.c, ax, fmaf(-tp.s, ay, acc[2 * j].x));
       acc[2 * j].y = fmaf(tp.c, ay, fmaf(tp.s, ax, acc[2 * j].y));
       acc[2 * j + 1].x = fmaf(tp.c, bx, fmaf(-tp.s, by, acc[2 * j + 1].x));
       acc[2 * j + 1].y = fmaf(tp.c, by, fmaf(tp.s, bx, acc[2 * j + 1].y));
+      __builtin_amdgcn_sched_barrier(0);
+      if (j + 1 < kPairs) {
+        va = na;
+        vb = nb;
+      }
     }
   };
 
