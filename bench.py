@@ -47,7 +47,7 @@ FLOPS_PER_TAP = 14  # 2 sub + 2 fma (interpolation) + 4 fma (rotate-accumulate),
 PROFILES = ROOT / "profiles"
 # The delay-and-sum kernel each precision's rcabeam_opw_grid_device launch runs on this workload.
 DAS_KERNEL = {
-    "float32": "das_opw_grid_valu<13, false> (complex64 channels, float32 VALU)",
+    "float32": "das_opw_grid_valu<13, false, false, true> (complex64 channels, float32 VALU)",
     "float16": "das_opw_mfma<__half2> (float16 channels, MFMA)",
     "bfloat16": "das_opw_mfma<bf16x2> (bfloat16 channels, MFMA)",
     "float16-lowrank": "das_opw_mfma<__half2> (low-rank coded float16 channels, MFMA)",
