@@ -103,10 +103,12 @@ def test_voxel_tail_and_large_volume(ny):
     assert rel_l2(iq.reshape(-1, 3), expected) < TOL
 
 
-@pytest.mark.parametrize("frames", [2, 8, 64, 66, 72, 200, 300])
+@pytest.mark.parametrize("frames", [2, 8, 64, 66, 72, 150, 193, 200, 206, 300])
 @pytest.mark.parametrize("channels", [1, 17])
 def test_frame_and_channel_tails(frames, channels):
-    """Reference tests/test_ensemble_targets.py:94-120 (cooperative-kernel tails)."""
+    """Reference tests/test_ensemble_targets.py:94-120 (cooperative-kernel tails).  Grid kernel frame
+    blocks: 150, 193 and 200 frames end in a block of single frames (none, one and all eight of them
+    inside nt), 206 in a partial pair block; 300 frames take the voxel-list kernel."""
     el = np.array([0.001], np.float32) if channels == 1 else np.linspace(-0.006, 0.001, channels).astype(np.float32)
     grid = (np.zeros(1, np.float32), np.array([0.007], np.float32), np.array([-0.001, 0.0, 0.001], np.float32))
     acq = scene.random_acquisition(channels=channels, angles=3, frames=frames, grid=grid, seed=19)
