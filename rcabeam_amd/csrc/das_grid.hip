@@ -125,26 +125,44 @@ __global__ __launch_bounds__(kVThreads) __attribute__((amdgpu_waves_per_eu(4))) 
                                          (__attribute__((address_space(3))) void*)(dst + i * 128), 16, 0, 0);
     }
   };
+  // Rows [0, rows) of src to dst, all issued by this wave.
+  auto copy_rows_wave = [&](const float2* src, float2* dst, int rows) __attribute__((always_inline)) {
+    const int samples = rows * nt;
+    const int chunks = (samples + 127) >> 7;
+    for (int i = 0; i < chunks; ++i) {
+      const int so = i * 128 + lane * 2;
+      if (so < samples)
+        __builtin_amdgcn_global_load_lds(reinterpret_cast<const void*>(src + so),
+                                         (__attribute__((address_space(3))) void*)(dst + i * 128), 16, 0, 0);
+    }
+  };
   // Copy a stage into buf: windows of angles [a0, a1) back to back (or one
   // chunk of a long window), contiguous in HBM and in LDS (rows of nt
-  // samples), 1 KiB per wave instruction spread over the waves.
+  // samples), 1 KiB per instruction.  Each window is copied whole by one
+  // wave (angle a0 + w, a0 + w + 8, ... by wave w), so a wave sets up only
+  // its own windows; a window's LDS offset is the exclusive prefix of the
+  // row counts over lanes a0 .. a - 1 (lane a holds angle a's window; na <=
+  // 16 keeps the scan inside one DPP row).  A chunk of a long window is
+  // spread over all waves.
   auto issue = [&](const Channel& ch, const Stage& s, float2* buf) __attribute__((always_inline)) {
     const float2* plane0 = plane_of(ch);
     const int a0 = uniform(s.a0), a1 = uniform(s.a1), sr0 = uniform(s.r0), srows = uniform(s.rows);
-    int off = 0;
-    for (int a = a0; a < a1; ++a) {
-      int lo, rows;
-      if (sr0 >= 0) {
-        lo = sr0;
-        rows = srows;
-      } else {
-        const int2 b = ch.w.at(a);
-        if (b.x > b.y) continue;
-        lo = b.x;
-        rows = b.y - b.x + 1;
-      }
-      copy_rows(plane0 + a * plane_size + static_cast<size_t>(lo) * nt, buf + off * nt, rows);
-      off += rows;
+    if (sr0 >= 0) {
+      copy_rows(plane0 + a0 * plane_size + static_cast<size_t>(sr0) * nt, buf, srows);
+      return;
+    }
+    const int rows_l = (lane >= a0 && lane < a1 && ch.w.lo <= ch.w.hi) ? ch.w.hi - ch.w.lo + 1 : 0;
+    int incl = rows_l;
+    incl += __builtin_amdgcn_update_dpp(0, incl, 0x111, 0xf, 0xf, true);  // row_shr:1
+    incl += __builtin_amdgcn_update_dpp(0, incl, 0x112, 0xf, 0xf, true);  // row_shr:2
+    incl += __builtin_amdgcn_update_dpp(0, incl, 0x114, 0xf, 0xf, true);  // row_shr:4
+    incl += __builtin_amdgcn_update_dpp(0, incl, 0x118, 0xf, 0xf, true);  // row_shr:8
+    const int excl = incl - rows_l;
+    for (int a = a0 + wave; a < a1; a += kVWaves) {
+      const int2 b = ch.w.at(a);
+      if (b.x > b.y) continue;
+      const int off = __builtin_amdgcn_readlane(excl, a);
+      copy_rows_wave(plane0 + a * plane_size + static_cast<size_t>(b.x) * nt, buf + off * nt, b.y - b.x + 1);
     }
   };
 
