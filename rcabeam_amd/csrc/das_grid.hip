@@ -18,6 +18,12 @@
 // global_load_lds_dwordx4 instructions, double-buffered across angles and
 // channels, one barrier per stage.
 //
+// Stages are planned once per workgroup, not once per wave: wave k % 8 plans
+// stage k + 2 (next channel, receive terms, windows, packing, LDS offsets)
+// into a ring of three LDS descriptors while stage k is consumed and stage
+// k + 1 is copied; the other waves read the descriptors.  Repeating the plan
+// in all 8 waves cost ~18 % of the kernel's VALU instructions.
+//
 // Per tap and frame: v = a + w (b - a), acc += v e^{i phi} (8 FMA), with the
 // tap (row, weight, phasor) of the lane's own voxel combined from its
 // receive and transmit terms exactly as combine() does.  The tap is formed
@@ -58,6 +64,25 @@ constexpr int kVStageWords = 9216;
 
 constexpr int kStageChannels = 4;  // channel windows per stage of a single-angle launch
 
+// A planned stage, shared by the waves of a workgroup through LDS: one wave
+// plans each stage (two ahead of the one being consumed) and every wave reads
+// the plan instead of repeating it.
+struct StageDesc {
+  int q;        // channel (-1: no further stage)
+  int a0, a1;   // angles [a0, a1)
+  int r0;       // >= 0: one chunk of a long window, rows [r0, r0 + rows) of angle a0
+  int rows;
+  int last;     // chunk stage: the last chunk of its window
+  int pad0, pad1;
+  int lo[16];   // tile window of each angle of channel q (lo > hi: empty)
+  int hi[16];
+  int off[16];  // LDS row of each window's first row (whole-window stages)
+  int base[16]; // LDS row = sample row - base (whole-window stages)
+  Term rx[8];   // receive terms of channel q at tile positions 0 .. 7
+};
+constexpr int kDescBytes = static_cast<int>(sizeof(StageDesc));
+constexpr int kDescs = 3;  // consumed, being copied, being planned
+
 // Dynamic LDS: tile tables at 0, then psum: float[64], then (single-angle
 // launches) two stage slots: receive terms Term[kStageChannels][8] and row
 // shifts int[kStageChannels] each.
@@ -65,7 +90,10 @@ constexpr int kSlotTermBytes = kStageChannels * kGridPatch * 16;
 constexpr int kSlotBytes = kSlotTermBytes + kStageChannels * 4;
 __host__ __device__ inline int psum_offset(int nc) { return tile_table_bytes(nc); }
 __host__ __device__ inline int slot_offset(int nc) { return psum_offset(nc) + kVTileSlots * 4; }
-__host__ __device__ inline int dynamic_bytes(int nc, bool multi) { return slot_offset(nc) + (multi ? 2 * kSlotBytes : 0); }
+__host__ __device__ inline int desc_offset(int nc) { return (slot_offset(nc) + 15) & ~15; }
+__host__ __device__ inline int dynamic_bytes(int nc, bool multi) {
+  return multi ? slot_offset(nc) + 2 * kSlotBytes : desc_offset(nc) + kDescs * kDescBytes;
+}
 
 // Rows of nt complex64 samples a stage buffer holds, with room for the frames
 // the last pair block of the last row reads past nt.
@@ -106,10 +134,6 @@ __global__ __launch_bounds__(kVThreads) __attribute__((amdgpu_waves_per_eu(4))) 
   tile.build_tables();  // ends with a barrier
 
   const size_t plane_size = static_cast<size_t>(p.ns) * nt;
-  auto plane_of = [&](const Channel& ch) __attribute__((always_inline)) {
-    const float2* data = static_cast<const float2*>(ch.cfg == 0 ? p.rc : p.cr);
-    return data + static_cast<size_t>(ch.q - ch.cfg * p.nc) * p.na * plane_size;
-  };
   // Receive positions of this lane's voxel: x for RC, y for CR.
   auto setup = [&](int q) __attribute__((always_inline)) { return tile.setup(q, wave, vy, 0); };
 
@@ -136,36 +160,6 @@ __global__ __launch_bounds__(kVThreads) __attribute__((amdgpu_waves_per_eu(4))) 
                                          (__attribute__((address_space(3))) void*)(dst + i * 128), 16, 0, 0);
     }
   };
-  // Copy a stage into buf: windows of angles [a0, a1) back to back (or one
-  // chunk of a long window), contiguous in HBM and in LDS (rows of nt
-  // samples), 1 KiB per instruction.  Each window is copied whole by one
-  // wave (angle a0 + w, a0 + w + 8, ... by wave w), so a wave sets up only
-  // its own windows; a window's LDS offset is the exclusive prefix of the
-  // row counts over lanes a0 .. a - 1 (lane a holds angle a's window; na <=
-  // 16 keeps the scan inside one DPP row).  A chunk of a long window is
-  // spread over all waves.
-  auto issue = [&](const Channel& ch, const Stage& s, float2* buf) __attribute__((always_inline)) {
-    const float2* plane0 = plane_of(ch);
-    const int a0 = uniform(s.a0), a1 = uniform(s.a1), sr0 = uniform(s.r0), srows = uniform(s.rows);
-    if (sr0 >= 0) {
-      copy_rows(plane0 + a0 * plane_size + static_cast<size_t>(sr0) * nt, buf, srows);
-      return;
-    }
-    const int rows_l = (lane >= a0 && lane < a1 && ch.w.lo <= ch.w.hi) ? ch.w.hi - ch.w.lo + 1 : 0;
-    int incl = rows_l;
-    incl += __builtin_amdgcn_update_dpp(0, incl, 0x111, 0xf, 0xf, true);  // row_shr:1
-    incl += __builtin_amdgcn_update_dpp(0, incl, 0x112, 0xf, 0xf, true);  // row_shr:2
-    incl += __builtin_amdgcn_update_dpp(0, incl, 0x114, 0xf, 0xf, true);  // row_shr:4
-    incl += __builtin_amdgcn_update_dpp(0, incl, 0x118, 0xf, 0xf, true);  // row_shr:8
-    const int excl = incl - rows_l;
-    for (int a = a0 + wave; a < a1; a += kVWaves) {
-      const int2 b = ch.w.at(a);
-      if (b.x > b.y) continue;
-      const int off = __builtin_amdgcn_readlane(excl, a);
-      copy_rows_wave(plane0 + a * plane_size + static_cast<size_t>(b.x) * nt, buf + off * nt, b.y - b.x + 1);
-    }
-  };
-
   float2 acc[kAcc];
 #pragma unroll
   for (int j = 0; j < kAcc; ++j) acc[j] = make_float2(0.f, 0.f);
@@ -214,43 +208,6 @@ __global__ __launch_bounds__(kVThreads) __attribute__((amdgpu_waves_per_eu(4))) 
       if (j + 1 < kPairs) {
         va = na;
         vb = nb;
-      }
-    }
-  };
-
-  auto consume = [&](const Channel& ch, const Stage& cur, const float2* base) __attribute__((always_inline)) {
-    // Stage fields are wave-uniform; saying so keeps the angle loop scalar.
-    const int a0 = uniform(cur.a0), a1 = uniform(cur.a1), cr0 = uniform(cur.r0);
-    const int t_lo = cr0 >= 0 ? cr0 : -kBig;
-    const int t_hi = cr0 >= 0 && !cur.last ? cr0 + max_rows - 2 : kBig;
-    const int tpos = ch.cfg == 0 ? vy : wave;  // transmit position of this lane's voxel
-    int off = 0;
-    for (int a = a0; a < a1; ++a) {
-      int first, row_off;
-      if (cr0 >= 0) {
-        first = cr0;
-        row_off = 0;
-      } else {
-        const int2 bw = ch.w.at(a);
-        if (bw.x > bw.y) continue;
-        first = bw.x;
-        row_off = off;
-        off += bw.y - bw.x + 1;
-      }
-      const Term tx = tile.transmit(ch.cfg, a, tpos);
-      int i = ch.rx.i + tx.i;
-      float f = ch.rx.f + tx.f;
-      const bool carry = f >= 1.0f;
-      i += carry ? 1 : 0;
-      f = carry ? f - 1.0f : f;
-      // A lane-dependent branch around the body (not a `continue`) keeps the
-      // angle loop itself wave-uniform.
-      if (i >= 0 && (i < p.ns - 1 || (i == p.ns - 1 && f == 0.0f)) && i >= t_lo && i <= t_hi) {
-        Tap tp;
-        tp.w = f;
-        tp.c = ch.rx.c * tx.c - ch.rx.s * tx.s;
-        tp.s = ch.rx.c * tx.s + ch.rx.s * tx.c;
-        accumulate(tp, static_cast<uint32_t>(i - first + row_off), base);
       }
     }
   };
@@ -375,49 +332,151 @@ __global__ __launch_bounds__(kVThreads) __attribute__((amdgpu_waves_per_eu(4))) 
       step(stage_b, stage_a, 1, 0);
     }
   } else {
-    // ---- stages of all live channels, one barrier per stage.  Two stages per
-    // loop trip, so each consume reads a buffer known at compile time.
-    Stage cur{0, 0, -1, 0, true, false};
-    Channel ch;
-    ch.q = -1;
-    for (int q = tile.next_live(-1); q < nq; q = tile.next_live(q)) {
-      ch = setup(q);
-      cur = plan_stage(ch.w, 0, p.na, max_rows);
-      if (cur.valid) break;
+    // ---- stages of all live channels, one barrier per stage, planned by one
+    // wave each (wave k % 8 plans stage k + 2) into a ring of descriptors.
+    StageDesc* desc = reinterpret_cast<StageDesc*>(lds + desc_offset(p.nc));
+    // Plan the stage after prev (first: the tile's first stage) into out.  Whole wave.
+    auto plan_after = [&](const StageDesc* prev, StageDesc* out, bool first) __attribute__((always_inline)) {
+      int q = -1;
+      Windows w{kBig, -kBig};
+      Stage st{0, 0, -1, 0, true, false};
+      if (!first) {
+        q = uniform(prev->q);
+        if (q < 0) {
+          if (lane == 0) out->q = -1;
+          return;
+        }
+        w.lo = prev->lo[lane & 15];
+        w.hi = prev->hi[lane & 15];
+        if (lane >= 16) w = Windows{kBig, -kBig};
+        const Stage cur{uniform(prev->a0), uniform(prev->a1), uniform(prev->r0), uniform(prev->rows),
+                        uniform(prev->last) != 0, true};
+        st = next_stage(cur, w, p.na, max_rows);
+      }
+      bool fresh = false;
+      if (!st.valid) {
+        for (int qq = tile.next_live(q); qq < nq; qq = tile.next_live(qq)) {
+          w = tile.tile_windows(qq);
+          st = plan_stage(w, 0, p.na, max_rows);
+          if (st.valid) {
+            q = qq;
+            fresh = true;
+            break;
+          }
+        }
+        if (!st.valid) {
+          if (lane == 0) out->q = -1;
+          return;
+        }
+      }
+      const int rows_l = (lane >= st.a0 && lane < st.a1 && w.lo <= w.hi) ? w.hi - w.lo + 1 : 0;
+      int incl = rows_l;
+      incl += __builtin_amdgcn_update_dpp(0, incl, 0x111, 0xf, 0xf, true);  // row_shr:1
+      incl += __builtin_amdgcn_update_dpp(0, incl, 0x112, 0xf, 0xf, true);  // row_shr:2
+      incl += __builtin_amdgcn_update_dpp(0, incl, 0x114, 0xf, 0xf, true);  // row_shr:4
+      incl += __builtin_amdgcn_update_dpp(0, incl, 0x118, 0xf, 0xf, true);  // row_shr:8
+      const int excl = incl - rows_l;
+      if (lane < 16) {
+        out->lo[lane] = w.lo;
+        out->hi[lane] = w.hi;
+        out->off[lane] = excl;
+        out->base[lane] = w.lo <= w.hi ? w.lo - excl : kBig;
+      }
+      if (fresh) {
+        const int slot = uniform(8 * (q - tile.rx_base));
+        if (lane >= slot && lane < slot + 8) out->rx[lane - slot] = tile.rx_term;
+      } else if (lane < 8) {
+        out->rx[lane] = prev->rx[lane];
+      }
+      if (lane == 0) {
+        out->q = q;
+        out->a0 = st.a0;
+        out->a1 = st.a1;
+        out->r0 = st.r0;
+        out->rows = st.rows;
+        out->last = st.last ? 1 : 0;
+      }
+    };
+    // Copies of stage d into buf: whole windows one wave each (angle a0 + w,
+    // a0 + w + 8, ...), a chunk of a long window spread over the waves.
+    auto issue_from = [&](const StageDesc* d, float2* buf) __attribute__((always_inline)) {
+      const int q = uniform(d->q);
+      if (q < 0) return;
+      const int cfg = q >= p.nc ? 1 : 0;
+      const float2* plane0 = static_cast<const float2*>(cfg == 0 ? p.rc : p.cr) +
+                             static_cast<size_t>(q - cfg * p.nc) * p.na * plane_size;
+      const int a0 = uniform(d->a0), a1 = uniform(d->a1), r0 = uniform(d->r0);
+      if (r0 >= 0) {
+        copy_rows(plane0 + a0 * plane_size + static_cast<size_t>(r0) * nt, buf, uniform(d->rows));
+        return;
+      }
+      for (int a = a0 + wave; a < a1; a += kVWaves) {
+        const int lo = uniform(d->lo[a]), hi = uniform(d->hi[a]);
+        if (lo > hi) continue;
+        copy_rows_wave(plane0 + a * plane_size + static_cast<size_t>(lo) * nt, buf + uniform(d->off[a]) * nt,
+                       hi - lo + 1);
+      }
+    };
+    // This wave's voxels, stage d from buf.
+    auto consume_from = [&](const StageDesc* d, const float2* base) __attribute__((always_inline)) {
+      const int q = uniform(d->q);
+      const int cfg = q >= p.nc ? 1 : 0;
+      const int a0 = uniform(d->a0), a1 = uniform(d->a1), cr0 = uniform(d->r0);
+      const int t_lo = cr0 >= 0 ? cr0 : -kBig;
+      const int t_hi = cr0 >= 0 && uniform(d->last) == 0 ? cr0 + max_rows - 2 : kBig;
+      const int tpos = cfg == 0 ? vy : wave;  // transmit position of this lane's voxel
+      const Term rx = d->rx[cfg == 0 ? wave : vy];
+      const int base_l = d->base[lane & 15];
+      for (int a = a0; a < a1; ++a) {
+        const int rb = cr0 >= 0 ? cr0 : __builtin_amdgcn_readlane(base_l, a);
+        if (rb == kBig) continue;
+        const Term tx = tile.transmit(cfg, a, tpos);
+        int i = rx.i + tx.i;
+        float f = rx.f + tx.f;
+        const bool carry = f >= 1.0f;
+        i += carry ? 1 : 0;
+        f = carry ? f - 1.0f : f;
+        // A lane-dependent branch around the body (not a `continue`) keeps the
+        // angle loop itself wave-uniform.
+        if (i >= 0 && (i < p.ns - 1 || (i == p.ns - 1 && f == 0.0f)) && i >= t_lo &&
+            i <= t_hi) {
+          Tap tp;
+          tp.w = f;
+          tp.c = rx.c * tx.c - rx.s * tx.s;
+          tp.s = rx.c * tx.s + rx.s * tx.c;
+          accumulate(tp, static_cast<uint32_t>(i - rb), base);
+        }
+      }
+    };
+
+    if (wave == 0) {
+      plan_after(nullptr, &desc[0], true);
+      plan_after(&desc[0], &desc[1], false);
     }
-    if (cur.valid) issue(ch, cur, stage_a);
+    __syncthreads();
+    issue_from(&desc[0], stage_a);
     __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
     __syncthreads();
-    // One stage: copy the next stage into `next` while the current one is read
-    // from `now`.
+    int k = 0;  // stage being consumed
     auto step = [&](const float2* now, float2* next) __attribute__((always_inline)) {
-      Stage nxt = next_stage(cur, ch.w, p.na, max_rows);
-      Channel nch = ch;
-      bool crossing = false;
-      if (!nxt.valid) {
-        for (int q = tile.next_live(ch.q); q < nq; q = tile.next_live(q)) {
-          nch = setup(q);
-          nxt = plan_stage(nch.w, 0, p.na, max_rows);
-          if (nxt.valid) break;
-        }
-        crossing = nxt.valid;
-      }
-      if (nxt.valid) issue(crossing ? nch : ch, nxt, next);
+      StageDesc* cur = &desc[k % kDescs];
+      StageDesc* nxt = &desc[(k + 1) % kDescs];
+      issue_from(nxt, next);
+      if (wave == (k & (kVWaves - 1))) plan_after(nxt, &desc[(k + 2) % kDescs], false);
       if constexpr (APERTURES) {
-        if (!rc_stored && ch.cfg == 1) {  // first CR stage: the RC sums are complete
+        if (!rc_stored && uniform(cur->q) >= p.nc) {  // first CR stage: the RC sums are complete
           store_plane(0);
           rc_stored = true;
         }
       }
-      consume(ch, cur, now);
+      consume_from(cur, now);
       __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
       __syncthreads();
-      if (crossing) ch = nch;
-      cur = nxt;
+      ++k;
     };
-    while (cur.valid) {
+    while (uniform(desc[k % kDescs].q) >= 0) {
       step(stage_a, stage_b);
-      if (!cur.valid) break;
+      if (uniform(desc[k % kDescs].q) < 0) break;
       step(stage_b, stage_a);
     }
   }

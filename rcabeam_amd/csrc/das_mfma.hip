@@ -45,7 +45,9 @@
 // samples, frames) layout and in LDS (row stride nt), so a stage is copied
 // with 1 KiB global_load_lds_dwordx4 instructions regardless of row
 // boundaries, double-buffered across angles and channels, one barrier per
-// stage.  B operands of frames past nt read the next row or zero padding and
+// stage.  Each window is copied by one wave in a loop of whole chunks (no
+// lane mask) and one masked tail chunk, and the next angle's transmit term
+// is read from LDS while the current angle is consumed.  B operands of frames past nt read the next row or zero padding and
 // only feed accumulator columns that are never stored.
 #include "grid_tile.hpp"
 #include "rcabeam_mi355x.h"
@@ -161,38 +163,55 @@ __global__ __launch_bounds__(kMThreads) __attribute__((amdgpu_waves_per_eu(2))) 
   // chunk of a long window).  A window is contiguous in HBM and in LDS (row
   // stride nt), so it is copied in 1 KiB global_load_lds_dwordx4
   // instructions regardless of row boundaries, spread over the waves.
+  // Copy a stage into buf: windows of angles [a0, a1) back to back (or one
+  // chunk of a long window), contiguous in HBM and in LDS (row stride nt),
+  // 1 KiB global_load_lds_dwordx4 instructions.  Each window is copied whole
+  // by one wave (angle a0 + g, a0 + g + 4, ... by wave g); its LDS offset is
+  // the exclusive prefix of the row counts over lanes a0 .. a - 1 (lane a
+  // holds angle a's window; na <= 16 keeps the scan inside one DPP row).  A
+  // chunk of a long window is spread over all waves.
+  // Copy `words` 32-bit words (a multiple of 4) from HBM to LDS, 1 KiB per
+  // instruction: chunks first, first + step, ... of this wave; whole chunks
+  // need no lane mask, only the last (partial) chunk does.
+  auto copy_words = [&](const W* src, W* dst, int words, int first, int step) {
+    if (p.vec) {
+      const int full = words >> 8;  // whole 1 KiB chunks
+      const W* s = src + lane * 4;
+      for (int i = first; i < full; i += step)
+        __builtin_amdgcn_global_load_lds(
+            reinterpret_cast<const void*>(s + i * 256),
+            reinterpret_cast<__attribute__((address_space(3))) void*>(reinterpret_cast<uintptr_t>(dst + i * 256)),
+            16, 0, 0);
+      if ((words & 255) && (full - first) % step == 0 && lane * 4 < (words & 255))
+        __builtin_amdgcn_global_load_lds(
+            reinterpret_cast<const void*>(s + full * 256),
+            reinterpret_cast<__attribute__((address_space(3))) void*>(reinterpret_cast<uintptr_t>(dst + full * 256)),
+            16, 0, 0);
+    } else {
+      for (int i = first * kWave + lane; i < words; i += step * kWave) dst[i] = src[i];
+    }
+  };
   auto issue = [&](const Channel& ch, const Stage& s, W* buf) {
     const W* plane0 = plane_of(ch);
     const int a0 = uniform(s.a0), a1 = uniform(s.a1), sr0 = uniform(s.r0), srows = uniform(s.rows);
-    int off = 0;
-    for (int a = a0; a < a1; ++a) {
-      int lo, rows;
-      if (sr0 >= 0) {
-        lo = sr0;
-        rows = srows;
-      } else {
-        const int2 b = ch.w.at(a);
-        if (b.x > b.y) continue;
-        lo = b.x;
-        rows = b.y - b.x + 1;
-      }
-      const W* src = plane0 + (a * plane_size + static_cast<size_t>(lo) * nt) * WPS;
-      W* dst = buf + off * nt * WPS;
-      off += rows;
-      const int words = rows * nt * WPS;
-      if (p.vec) {
-        const int chunks = (words + 255) >> 8;  // 256 words = 1 KiB
-        for (int i = g; i < chunks; i += kMWaves) {
-          const int wo = i * 256 + lane * 4;
-          if (wo < words)
-            __builtin_amdgcn_global_load_lds(
-                reinterpret_cast<const void*>(src + wo),
-                reinterpret_cast<__attribute__((address_space(3))) void*>(reinterpret_cast<uintptr_t>(dst + i * 256)),
-                16, 0, 0);
-        }
-      } else {
-        for (int i = tid; i < words; i += kMThreads) dst[i] = src[i];
-      }
+    if (sr0 >= 0) {
+      copy_words(plane0 + (a0 * plane_size + static_cast<size_t>(sr0) * nt) * WPS, buf, srows * nt * WPS, g,
+                 kMWaves);
+      return;
+    }
+    const int rows_l = (lane >= a0 && lane < a1 && ch.w.lo <= ch.w.hi) ? ch.w.hi - ch.w.lo + 1 : 0;
+    int incl = rows_l;
+    incl += __builtin_amdgcn_update_dpp(0, incl, 0x111, 0xf, 0xf, true);  // row_shr:1
+    incl += __builtin_amdgcn_update_dpp(0, incl, 0x112, 0xf, 0xf, true);  // row_shr:2
+    incl += __builtin_amdgcn_update_dpp(0, incl, 0x114, 0xf, 0xf, true);  // row_shr:4
+    incl += __builtin_amdgcn_update_dpp(0, incl, 0x118, 0xf, 0xf, true);  // row_shr:8
+    const int excl = incl - rows_l;
+    for (int a = a0 + g; a < a1; a += kMWaves) {
+      const int2 b = ch.w.at(a);
+      if (b.x > b.y) continue;
+      const int off = __builtin_amdgcn_readlane(excl, a);
+      copy_words(plane0 + (a * plane_size + static_cast<size_t>(b.x) * nt) * WPS, buf + off * nt * WPS,
+                 (b.y - b.x + 1) * nt * WPS, 0, 1);
     }
   };
 
@@ -210,7 +229,11 @@ __global__ __launch_bounds__(kMThreads) __attribute__((amdgpu_waves_per_eu(2))) 
     const int t_hi = cr0 >= 0 && !cur.last ? cr0 + max_rows - 2 : kBig;
     const int tpos = ch.cfg == 0 ? ay : ax;  // transmit position of the A-operand voxel
     int off = 0;
+    Term tx_next = tile.transmit(ch.cfg, a0, tpos);
     for (int a = a0; a < a1; ++a) {
+      // The next angle's transmit term is read while this angle is consumed.
+      const Term tx = tx_next;
+      if (a + 1 < a1) tx_next = tile.transmit(ch.cfg, a + 1, tpos);
       int first, row_off, lim_lo, lim_hi;
       if (cr0 >= 0) {  // one chunk of a long window
         first = cr0;
@@ -230,7 +253,7 @@ __global__ __launch_bounds__(kMThreads) __attribute__((amdgpu_waves_per_eu(2))) 
       const int lo = max(gw.x, lim_lo);
       const int hi = min(gw.y, lim_hi);
       if (lo > hi) continue;
-      const Tap tp = tap_of(ch.rx, tile.transmit(ch.cfg, a, tpos), p.ns);
+      const Tap tp = tap_of(ch.rx, tx, p.ns);
       const bool take = tp.i0 >= 0 && tp.i0 >= t_lo && tp.i0 <= t_hi;
       const float u = 1.0f - tp.w;
       const float p0r = u * tp.c, p0i = u * tp.s, p1r = tp.w * tp.c, p1i = tp.w * tp.s;

@@ -357,6 +357,29 @@ struct GridTile {
     return ch;
   }
 
+  // Tile windows of channel q (lane a: angle a; empty beyond na), its receive
+  // terms filled as by setup() (lanes slot .. slot + 7, slot = 8 (q - rx_base)).
+  // Called by the whole wave with q uniform.
+  __device__ Windows tile_windows(int q) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int cfg = q >= p.nc ? 1 : 0;
+    if (q < rx_base || q >= rx_base + 8) fill_receive(q);
+    const int slot = uniform(8 * (q - rx_base));
+    const Key lo0{__builtin_amdgcn_readlane(rx_lo.i, slot),
+                  static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(rx_lo.f), slot))};
+    const Key lo1{__builtin_amdgcn_readlane(rx_lo.i, slot + 4),
+                  static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(rx_lo.f), slot + 4))};
+    const Key hi0{__builtin_amdgcn_readlane(rx_hi.i, slot),
+                  static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(rx_hi.f), slot))};
+    const Key hi1{__builtin_amdgcn_readlane(rx_hi.i, slot + 4),
+                  static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(rx_hi.f), slot + 4))};
+    const int a = lane < p.na ? lane : 0;
+    const Key* tw = txw + (cfg * kGridMaxAngles + a) * 6;
+    int2 w = window(key_min(lo0, lo1), key_max(hi0, hi1), tw[0], tw[1], p.ns);
+    if (lane >= p.na) w = make_int2(kBig, -kBig);
+    return Windows{w.x, w.y};
+  }
+
   // Transmit term of angle a at tile position pos (along y for RC, x for CR).
   __device__ Term transmit(int cfg, int a, int pos) const { return txt[(cfg * kGridMaxAngles + a) * 8 + pos]; }
 };
