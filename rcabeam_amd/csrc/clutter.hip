@@ -12,7 +12,10 @@
 // each on v_mfma_f32_32x32x16_{bf16,f16}.  complex64 input is scaled by
 // 1/max(|Re|,|Im|) and split into bfloat16 hi + lo (x = hi + lo to ~2^-18
 // relative), accumulating all four products hi*hi + hi*lo + lo*hi + lo*lo;
-// FP16/BF16 input is exact in one product.  Each workgroup sums a bounded run of rows in float32 (the
+// FP16/BF16 input is exact in one product.  A 16-row step is staged into LDS
+// column-major by threads that each convert 8 rows of one frame and write
+// them as 16-byte words; two stage buffers alternate (one barrier per step)
+// when two workgroups still fit the CU's LDS.  Each workgroup sums a bounded run of rows in float32 (the
 // rounding of short runs stays ~1e-8 relative after the cross-workgroup sum)
 // and writes a float32 partial; partials are reduced in float64.
 #include <rocblas/rocblas.h>
@@ -62,15 +65,15 @@ __device__ __forceinline__ float bf16_bits_to_float(unsigned short b) { return _
 // MODE kExactF16:  T = __half2, one fp16 product.
 // MODE kExactBf16: T = bf16x2, one bf16 product.
 // Row r of the Gram is row r * row_step of x (row_step 1: every row).
-template <int MODE, typename T>
+template <int MODE, bool DB, typename T>
 __global__ __launch_bounds__(kGramThreads) void k_gram(const T* __restrict__ x, long long rows, long long row_step,
                                                       int frames, const float* __restrict__ scale_ptr, GramPlan plan,
                                                       float* __restrict__ partials) {
   constexpr bool kSplit = MODE == kSplitBf16;
-  extern __shared__ __align__(16) unsigned short opnd[];  // [2 (hi, lo)][cols][kGramKPad]
+  // [buffer (DB: 2)][hi, lo][cols][kGramKPad] bf16 / f16 operands of one 16-row step, column-major.
+  extern __shared__ __align__(16) unsigned short opnd[];
   const int cols = plan.nb * 32;
-  unsigned short* hi = opnd;
-  unsigned short* lo = opnd + cols * kGramKPad;
+  const int buf_elems = 2 * cols * kGramKPad;
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = threadIdx.x / kWave;
   // 1 / max(|Re|, |Im|) in float64: the max may be a float32 subnormal whose inverse overflows float32.
@@ -81,14 +84,58 @@ __global__ __launch_bounds__(kGramThreads) void k_gram(const T* __restrict__ x, 
   const int steps = static_cast<int>((row_end - row0 + kGramK - 1) / kGramK);
 
   // Zero the padding columns once (they stay zero).
-  for (int idx = threadIdx.x; idx < (cols - 2 * frames) * kGramKPad; idx += blockDim.x) {
-    hi[2 * frames * kGramKPad + idx] = 0;
-    lo[2 * frames * kGramKPad + idx] = 0;
-  }
+  for (int b = 0; b < (DB ? 2 : 1); ++b)
+    for (int idx = threadIdx.x; idx < (cols - 2 * frames) * kGramKPad; idx += blockDim.x) {
+      opnd[b * buf_elems + 2 * frames * kGramKPad + idx] = 0;
+      opnd[b * buf_elems + cols * kGramKPad + 2 * frames * kGramKPad + idx] = 0;
+    }
 
-  // Each thread stages up to kPer complex values of a 16-row step.
-  constexpr int kPer = 4;  // values staged per thread and pass
-  const int per_step = kGramK * frames;
+  // Stage step s into buffer base: thread item (h, t) converts rows 8 h .. 8 h + 7 of frame t and
+  // writes them as one 16-byte word per operand column (Re and Im of hi, and of lo when split).
+  auto stage = [&](int s, unsigned short* base) __attribute__((always_inline)) {
+    unsigned short* hi = base;
+    unsigned short* lo = base + cols * kGramKPad;
+    const long long rbase = row0 + static_cast<long long>(s) * kGramK;
+    for (int item = threadIdx.x; item < 2 * frames; item += kGramThreads) {
+      const int h = item / frames, t = item - h * frames;
+      float2 v[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const long long r = rbase + 8 * h + j;
+        v[j] = r < row_end ? load_sample(x + r * row_step * frames + t) : make_float2(0.f, 0.f);
+      }
+      uint32_t hre[4] = {0u, 0u, 0u, 0u}, him[4] = {0u, 0u, 0u, 0u}, lre[4] = {0u, 0u, 0u, 0u}, lim[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        unsigned short hr, hi_i, lr = 0, li = 0;
+        if constexpr (MODE == kExactF16) {
+          hr = __half_as_ushort(__float2half_rn(v[j].x));
+          hi_i = __half_as_ushort(__float2half_rn(v[j].y));
+        } else {
+          const float vx = static_cast<float>(static_cast<double>(v[j].x) * inv_scale);
+          const float vy = static_cast<float>(static_cast<double>(v[j].y) * inv_scale);
+          hr = bf16_rn_bits(vx);
+          hi_i = bf16_rn_bits(vy);
+          if constexpr (kSplit) {
+            lr = bf16_rn_bits(vx - bf16_bits_to_float(hr));
+            li = bf16_rn_bits(vy - bf16_bits_to_float(hi_i));
+          }
+        }
+        const int sh = 16 * (j & 1);
+        hre[j >> 1] |= static_cast<uint32_t>(hr) << sh;
+        him[j >> 1] |= static_cast<uint32_t>(hi_i) << sh;
+        lre[j >> 1] |= static_cast<uint32_t>(lr) << sh;
+        lim[j >> 1] |= static_cast<uint32_t>(li) << sh;
+      }
+      *reinterpret_cast<uint4*>(hi + (2 * t) * kGramKPad + 8 * h) = make_uint4(hre[0], hre[1], hre[2], hre[3]);
+      *reinterpret_cast<uint4*>(hi + (2 * t + 1) * kGramKPad + 8 * h) = make_uint4(him[0], him[1], him[2], him[3]);
+      if constexpr (kSplit) {
+        *reinterpret_cast<uint4*>(lo + (2 * t) * kGramKPad + 8 * h) = make_uint4(lre[0], lre[1], lre[2], lre[3]);
+        *reinterpret_cast<uint4*>(lo + (2 * t + 1) * kGramKPad + 8 * h) = make_uint4(lim[0], lim[1], lim[2], lim[3]);
+      }
+    }
+  };
+
   floatx16 acc[kGramMaxBlocksPerWave];
 #pragma unroll
   for (int b = 0; b < kGramMaxBlocksPerWave; ++b)
@@ -98,42 +145,10 @@ __global__ __launch_bounds__(kGramThreads) void k_gram(const T* __restrict__ x, 
   const int my_start = plan.wave_start[wave];
   const int my_count = plan.wave_count[wave];
 
-  for (int s = 0; s < steps; ++s) {
-    const long long rbase = row0 + static_cast<long long>(s) * kGramK;
-    __syncthreads();
-    for (int base = 0; base < per_step; base += kPer * kGramThreads) {
-#pragma unroll
-      for (int u = 0; u < kPer; ++u) {
-        const int idx = base + u * kGramThreads + threadIdx.x;
-        if (idx < per_step) {
-          const int k = idx / frames, t = idx - k * frames;
-          const long long r = rbase + k;
-          float2 v = make_float2(0.f, 0.f);
-          if (r < row_end) v = load_sample(x + r * row_step * frames + t);
-          unsigned short hr, hi_i, lr = 0, li = 0;
-          if constexpr (MODE == kExactF16) {
-            hr = __half_as_ushort(__float2half_rn(v.x));
-            hi_i = __half_as_ushort(__float2half_rn(v.y));
-          } else {
-            v.x = static_cast<float>(static_cast<double>(v.x) * inv_scale);
-            v.y = static_cast<float>(static_cast<double>(v.y) * inv_scale);
-            hr = bf16_rn_bits(v.x);
-            hi_i = bf16_rn_bits(v.y);
-            if constexpr (kSplit) {
-              lr = bf16_rn_bits(v.x - bf16_bits_to_float(hr));
-              li = bf16_rn_bits(v.y - bf16_bits_to_float(hi_i));
-            }
-          }
-          hi[(2 * t) * kGramKPad + k] = hr;
-          hi[(2 * t + 1) * kGramKPad + k] = hi_i;
-          if constexpr (kSplit) {
-            lo[(2 * t) * kGramKPad + k] = lr;
-            lo[(2 * t + 1) * kGramKPad + k] = li;
-          }
-        }
-      }
-    }
-    __syncthreads();
+  // This wave's blocks of one staged step.
+  auto compute = [&](const unsigned short* base) __attribute__((always_inline)) {
+    const unsigned short* hi = base;
+    const unsigned short* lo = base + cols * kGramKPad;
     const int koff = 8 * (lane / 32);
     int prev = -1;
     bf16x8 a_hi, a_lo;
@@ -162,6 +177,24 @@ __global__ __launch_bounds__(kGramThreads) void k_gram(const T* __restrict__ x, 
           }
         }
       }
+    }
+  };
+
+  if constexpr (DB) {
+    // Double-buffered: step s + 1 is staged while step s is multiplied, one barrier per step.
+    if (steps > 0) stage(0, opnd);
+    __syncthreads();
+    for (int s = 0; s < steps; ++s) {
+      if (s + 1 < steps) stage(s + 1, opnd + ((s + 1) & 1) * buf_elems);
+      compute(opnd + (s & 1) * buf_elems);
+      __syncthreads();
+    }
+  } else {
+    for (int s = 0; s < steps; ++s) {
+      __syncthreads();
+      stage(s, opnd);
+      __syncthreads();
+      compute(opnd);
     }
   }
   // Partials: [group][pass block][lane][16]
@@ -605,16 +638,19 @@ static hipError_t gram_launch(const void* x, long long rows, long long row_step,
   const int nb = column_blocks(frames);
   const long long groups = (rows + kGramRowsPerGroup - 1) / kGramRowsPerGroup;
   const int passes = static_cast<int>((triangle_blocks(nb) + kGramBlocksPerPass - 1) / kGramBlocksPerPass);
-  const size_t lds = lds_bytes(nb);
+  // Double-buffered staging when two workgroups per CU still fit in 160 KiB of LDS.
+  const bool db = 4 * lds_bytes(nb) <= 160 * 1024;
+  const size_t lds = (db ? 2 : 1) * lds_bytes(nb);
   if (lds > 160 * 1024) return hipErrorInvalidValue;
-  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_gram<MODE, T>),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+  auto kernel = db ? &k_gram<MODE, true, T> : &k_gram<MODE, false, T>;
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     static_cast<int>(lds));
   if (e != hipSuccess) return e;
   for (int pass = 0; pass < passes; ++pass) {
     GramPlan plan;
     make_plan(nb, pass, plan);
-    k_gram<MODE, T><<<dim3(static_cast<unsigned>(groups)), kGramThreads, lds>>>(
-        static_cast<const T*>(x), rows, row_step, frames, scale, plan, workspace);
+    kernel<<<dim3(static_cast<unsigned>(groups)), kGramThreads, lds>>>(static_cast<const T*>(x), rows, row_step, frames,
+                                                                        scale, plan, workspace);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     const long long n = static_cast<long long>(plan.pass_blocks) * 1024;
     k_gram_reduce<<<dim3(static_cast<unsigned>((n + 255) / 256)), 256>>>(workspace, static_cast<int>(groups), plan, c,
