@@ -133,9 +133,10 @@ class Pipeline:
         self.pd = dev.empty((nv, 2), torch.float32)
         self.signal = dev.empty((nv, self.nt), torch.complex64 if self.code == 0 else torch.int32)
         if self.lowrank:
-            from rcabeam_amd.lowrank import ROW_STEP
+            from rcabeam_amd.lowrank import ROW_STEP, SUBSPACE_ITERATIONS
 
             self.row_step = ROW_STEP
+            self.sub_iterations = SUBSPACE_ITERATIONS
             rows = (self.ns * self.nc * self.na + ROW_STEP - 1) // ROW_STEP
             self.c_rows = rows
             self.c_work_bytes = int(self.lib.rcabeam_gram_workspace_bytes(rows, self.nt))
@@ -143,7 +144,6 @@ class Pipeline:
             self.c_stats = dev.empty((3,), torch.int32)
             self.c_real = dev.empty((int(self.lib.rcabeam_gram_real_elements(self.nt)),), torch.float64)
             self.c_gram = dev.empty((self.nt, self.nt), torch.complex128)
-            self.c_values = dev.empty((self.nt,), torch.float64)
             self.basis = dev.empty((self.k, self.nt), torch.complex128)
             self.coded = dev.empty((nv, self.ntc), torch.complex64)
         self.stats = dev.empty((3,), torch.int32)
@@ -189,8 +189,7 @@ class Pipeline:
         channel_code = _lib.STORAGE[self.channel_storage]
         if self.lowrank:
             # Channel basis (rcabeam_amd.lowrank.channel_basis): split-bfloat16 Gram of every row_step-th row
-            # of both apertures' Casorati matrices, float64 EVD, the k strongest modes (descending; rows of the
-            # row-major view of the column-major eigenvectors).
+            # of both apertures' Casorati matrices, float64 subspace iteration for its k strongest modes.
             for i, raw in enumerate((self.rc, self.cr)):
                 self._call("rcabeam_iq_absmax_device", dev.ptr(raw), 0, raw.numel(), dev.ptr(self.c_stats), i)
             for i, raw in enumerate((self.rc, self.cr)):
@@ -199,9 +198,8 @@ class Pipeline:
                            self.c_work_bytes, i)
             self._call("rcabeam_gram_hermitian_device", dev.ptr(self.c_real), self.nt, 1.0 / (2 * self.c_rows),
                        dev.ptr(self.c_gram))
-            self._call("rcabeam_eigh_device", dev.ptr(self.c_gram), dev.ptr(self.c_values), dev.ptr(self.e_work),
-                       dev.ptr(self.info), self.nt)
-            self.basis.copy_(self.c_gram[self.nt - self.k:].flip(0))
+            self._call("rcabeam_dominant_subspace_device", dev.ptr(self.c_gram), self.nt, self.k,
+                       self.sub_iterations, dev.ptr(self.basis))
             mark("channel_basis")
             for raw, packed in zip((self.rc, self.cr), self.packed):
                 self._call("rcabeam_pack_ensemble_lowrank_device", dev.ptr(raw), dev.ptr(packed), channel_code,

@@ -211,6 +211,19 @@ rcabeam_status rcabeam_pack_ensemble_lowrank_device(const void* source, void* ta
 rcabeam_status rcabeam_lowrank_expand_device(const void* coded, const void* basis, size_t k, void* out,
                                              int out_storage, float* pd, size_t nv, size_t nt);
 
+/* Orthonormal basis (complex128, k rows of `frames` = the columns of a
+ * frames x k matrix) of the dominant k-dimensional invariant subspace of the
+ * Hermitian frames x frames complex128 matrix `gram` (column-major, both
+ * triangles), by `iterations` rounds of subspace iteration from k fixed
+ * pseudo-random vectors (modified Gram-Schmidt after each product; a zero
+ * matrix gives a zero basis).  One workgroup, no allocation.  The span converges at the rate (lambda_{k+1} / lambda_k) per
+ * round; the low-rank channel coding uses it for its basis (any orthonormal
+ * basis codes exactly; one that catches the strong modes keeps the 16-bit
+ * residual small).  No reference counterpart.  1 <= k <= 12, k < frames,
+ * k * frames <= 4096. */
+rcabeam_status rcabeam_dominant_subspace_device(const void* gram, size_t frames, size_t k, size_t iterations,
+                                                void* basis);
+
 /* ---- temporal clutter filter (replaces the CuPy/cuBLAS/cuSOLVER path of
  * reference src/rcabeam/gpu_clutter.py:112-273) ----------------------------
  * iq is a row-major Casorati matrix (rows, frames) of `storage` samples. */

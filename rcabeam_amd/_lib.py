@@ -63,6 +63,7 @@ _SIGNATURES: dict[str, list] = {
     + [c_void_p] * 8 + [c_void_p, c_size_t] + [c_size_t] * 4 + [c_float] * 4,
     "rcabeam_pack_ensemble_lowrank_device": [c_void_p, c_void_p, c_int, c_void_p, c_size_t, c_void_p] + [c_size_t] * 4,
     "rcabeam_lowrank_expand_device": [c_void_p, c_void_p, c_size_t, c_void_p, c_int, c_void_p, c_size_t, c_size_t],
+    "rcabeam_dominant_subspace_device": [c_void_p, c_size_t, c_size_t, c_size_t, c_void_p],
     "rcabeam_count_active_taps_device": [c_void_p] * 5 + [c_size_t] * 4 + [c_float] * 4 + [c_void_p],
     "rcabeam_iq_absmax_device": [c_void_p, c_int, c_size_t, c_void_p, c_int],
     "rcabeam_gram_workspace_bytes": [c_size_t, c_size_t],

@@ -186,10 +186,143 @@ __global__ __launch_bounds__(256) void k_expand(const float2* __restrict__ coded
   }
 }
 
+// Dominant k-dimensional invariant subspace of a Hermitian nt x nt complex128
+// matrix G (column-major) by subspace iteration: from k fixed pseudo-random
+// vectors, `iterations` times W = G V followed by modified Gram-Schmidt (twice
+// on the last pass), in one workgroup with V and W in LDS.  A zero G gives a
+// zero basis.  The result is an
+// orthonormal basis whose span approaches that of the k leading eigenvectors
+// at the rate (lambda_{k+1} / lambda_k)^iterations; the low-rank coding needs
+// only a basis that catches the strong clutter modes (any orthonormal basis
+// codes the channels exactly).
+constexpr int kSubThreads = 1024;
+constexpr int kSubWaves = kSubThreads / kWave;
+constexpr int kSubMaxK = 12;
+constexpr int kSubMaxElems = 4096;  // k * nt per LDS array (64 KiB of complex128)
+
+__device__ __forceinline__ double2 cmulc_d(double2 a, double2 b) {  // conj(a) * b
+  return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+
+// Sum over the workgroup of n <= kSubMaxK complex values per thread; every
+// thread receives the sums.  Ends with a barrier.
+__device__ void block_sum(double2 (&v)[kSubMaxK], int n, double2* red) {
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  for (int j = 0; j < n; ++j) {
+    double x = v[j].x, y = v[j].y;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      x += __shfl_xor(x, off, kWave);
+      y += __shfl_xor(y, off, kWave);
+    }
+    if (lane == 0) red[wave * kSubMaxK + j] = make_double2(x, y);
+  }
+  __syncthreads();
+  for (int j = 0; j < n; ++j) {
+    double x = 0.0, y = 0.0;
+    for (int w = 0; w < kSubWaves; ++w) {  // fixed order: the same sums on every thread
+      x += red[w * kSubMaxK + j].x;
+      y += red[w * kSubMaxK + j].y;
+    }
+    v[j] = make_double2(x, y);
+  }
+  __syncthreads();
+}
+
+// Orthonormalize the k columns of w (rows of [k][nt]) into v by modified
+// Gram-Schmidt; w is overwritten.  A zero column stays zero.
+__device__ void mgs(double2* w, double2* v, int nt, int k, double2* red) {
+  for (int a = 0; a < k; ++a) {
+    double2 part[kSubMaxK];
+    part[0] = make_double2(0.0, 0.0);
+    for (int i = threadIdx.x; i < nt; i += kSubThreads) {
+      const double2 x = w[a * nt + i];
+      part[0].x += x.x * x.x + x.y * x.y;
+    }
+    block_sum(part, 1, red);
+    const double inv = part[0].x > 0.0 ? 1.0 / sqrt(part[0].x) : 0.0;
+    for (int i = threadIdx.x; i < nt; i += kSubThreads) {
+      const double2 x = w[a * nt + i];
+      v[a * nt + i] = make_double2(x.x * inv, x.y * inv);
+    }
+    __syncthreads();
+    const int rest = k - a - 1;
+    if (rest == 0) break;
+    for (int b = 0; b < rest; ++b) part[b] = make_double2(0.0, 0.0);
+    for (int i = threadIdx.x; i < nt; i += kSubThreads) {
+      const double2 va = v[a * nt + i];
+      for (int b = 0; b < rest; ++b) {
+        const double2 d = cmulc_d(va, w[(a + 1 + b) * nt + i]);
+        part[b].x += d.x;
+        part[b].y += d.y;
+      }
+    }
+    block_sum(part, rest, red);
+    for (int i = threadIdx.x; i < nt; i += kSubThreads) {
+      const double2 va = v[a * nt + i];
+      for (int b = 0; b < rest; ++b) {
+        double2& x = w[(a + 1 + b) * nt + i];
+        x.x -= part[b].x * va.x - part[b].y * va.y;
+        x.y -= part[b].x * va.y + part[b].y * va.x;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(kSubThreads) void k_subspace(const double2* __restrict__ g, int nt, int k, int iterations,
+                                                          double2* __restrict__ basis) {
+  extern __shared__ double2 sub_lds[];
+  double2* v = sub_lds;           // [k][nt]: the basis, row c = column c of V
+  double2* w = v + k * nt;        // [k][nt]
+  double2* red = w + k * nt;      // [kSubWaves][kSubMaxK]
+  // Start from fixed pseudo-random vectors (entries in [-1, 1) from an integer hash), so that no column of G
+  // that happens to be zero or dependent can leave a start vector without a dominant component.
+  for (int idx = threadIdx.x; idx < k * nt; idx += kSubThreads) {
+    unsigned h = static_cast<unsigned>(idx) * 0x9E3779B9u + 0x7F4A7C15u;
+    h ^= h >> 16; h *= 0x85EBCA6Bu; h ^= h >> 13; h *= 0xC2B2AE35u; h ^= h >> 16;
+    v[idx] = make_double2(static_cast<double>(h & 0xFFFFu) / 32768.0 - 1.0, static_cast<double>(h >> 16) / 32768.0 - 1.0);
+  }
+  __syncthreads();
+  for (int it = 0; it < iterations; ++it) {
+    for (int idx = threadIdx.x; idx < k * nt; idx += kSubThreads) {
+      const int c = idx / nt, i = idx - c * nt;
+      double re = 0.0, im = 0.0;
+      for (int j = 0; j < nt; ++j) {
+        const double2 gij = g[i + static_cast<long long>(j) * nt];
+        const double2 vj = v[c * nt + j];
+        re = fma(gij.x, vj.x, fma(-gij.y, vj.y, re));
+        im = fma(gij.x, vj.y, fma(gij.y, vj.x, im));
+      }
+      w[idx] = make_double2(re, im);
+    }
+    __syncthreads();
+    mgs(w, v, nt, k, red);
+  }
+  // Second pass on the result: orthonormal to working precision.
+  for (int idx = threadIdx.x; idx < k * nt; idx += kSubThreads) w[idx] = v[idx];
+  __syncthreads();
+  mgs(w, v, nt, k, red);
+  for (int idx = threadIdx.x; idx < k * nt; idx += kSubThreads) basis[idx] = v[idx];
+}
+
 }  // namespace
 }  // namespace rcb
 
 using namespace rcb;
+
+extern "C" rcabeam_status rcabeam_dominant_subspace_device(const void* gram, size_t frames, size_t k,
+                                                           size_t iterations, void* basis) {
+  if (!gram || !basis || !frames || !k || k > static_cast<size_t>(kSubMaxK) || k >= frames ||
+      k * frames > static_cast<size_t>(kSubMaxElems) || !iterations || iterations > 1000)
+    return RCABEAM_ERROR_ARGUMENT;
+  const size_t lds = (2 * k * frames + kSubWaves * kSubMaxK) * sizeof(double2);
+  const hipError_t e = allow_dynamic_lds(reinterpret_cast<const void*>(&k_subspace), static_cast<int>(lds));
+  if (e != hipSuccess) return RCABEAM_ERROR_CUDA;
+  k_subspace<<<1, kSubThreads, lds>>>(static_cast<const double2*>(gram), static_cast<int>(frames), static_cast<int>(k),
+                                       static_cast<int>(iterations), static_cast<double2*>(basis));
+  return launch_status();
+}
 
 extern "C" rcabeam_status rcabeam_pack_ensemble_lowrank_device(const void* source, void* target, int storage,
                                                                const void* basis, size_t k, int* range_error,

@@ -21,16 +21,19 @@ from rcabeam_amd.sim import RCAGeometry
 
 MAX_BASIS = 12  # rcabeam_pack_ensemble_lowrank_device: k <= 12
 ROW_STEP = 4  # channel rows sampled for the basis: every 4th (sample, channel, angle) row
+SUBSPACE_ITERATIONS = 12  # rounds of rcabeam_dominant_subspace_device for the basis
 
 
-def channel_basis(rc, cr, k: int, row_step: int = ROW_STEP):
+def channel_basis(rc, cr, k: int, row_step: int = ROW_STEP, iterations: int = SUBSPACE_ITERATIONS):
     """The k strongest temporal modes of the RC and CR channels together: complex128 (k, nt), row j = column j
     of V (orthonormal).
 
     The Gram is formed from every `row_step`-th row of both channel Casorati matrices (rows = (sample,
-    channel, angle), complex64) with split bfloat16 (hi + lo) products on MFMA, then decomposed in float64
-    on the GPU.  Any orthonormal V codes the channels exactly up to the 16-bit rounding of the residual;
-    the sample only has to catch the strong tissue modes, which every depth band carries."""
+    channel, angle), complex64) with split bfloat16 (hi + lo) products on MFMA; its dominant k-dimensional
+    subspace comes from `iterations` rounds of float64 subspace iteration on the GPU (one workgroup,
+    rcabeam_dominant_subspace_device) rather than a full eigendecomposition.  Any orthonormal V codes the
+    channels exactly up to the 16-bit rounding of the residual; V only has to catch the strong tissue
+    modes, which every depth band carries and which stand well above the rest of the spectrum."""
     t = dev.torch()
     if not 1 <= k <= MAX_BASIS:
         raise ValueError(f"k must be in [1, {MAX_BASIS}]")
@@ -39,6 +42,10 @@ def channel_basis(rc, cr, k: int, row_step: int = ROW_STEP):
     nt = rc.shape[-1]
     if k >= nt:
         raise ValueError("k must be smaller than the frame count")
+    if k * nt > 4096:
+        raise ValueError("k x frames must be at most 4096 (rcabeam_dominant_subspace_device)")
+    if iterations < 1:
+        raise ValueError("iterations must be positive")
     lib = _lib.load()
     stats = dev.empty((3,), t.int32)
     for i, a in enumerate((rc, cr)):
@@ -55,14 +62,9 @@ def channel_basis(rc, cr, k: int, row_step: int = ROW_STEP):
                   dev.ptr(real), dev.ptr(work), nbytes, int(i > 0))
     gram = dev.empty((nt, nt), t.complex128)
     _lib.call("rcabeam_gram_hermitian_device", dev.ptr(real), nt, 1.0 / (2 * rows), dev.ptr(gram))
-    values = dev.empty((nt,), t.float64)
-    e_work = dev.empty((nt,), t.float64)
-    info = dev.empty((1,), t.int32)
-    _lib.call("rcabeam_eigh_device", dev.ptr(gram), dev.ptr(values), dev.ptr(e_work), dev.ptr(info), nt)
-    if int(info.item()) != 0:
-        raise ValueError("Channel Gram eigendecomposition failed (non-finite channels?)")
-    # Eigenvectors (ascending) in column-major order: row j of the row-major view is vector j.
-    return gram[nt - k:].flip(0).contiguous()
+    basis = dev.empty((k, nt), t.complex128)
+    _lib.call("rcabeam_dominant_subspace_device", dev.ptr(gram), nt, k, iterations, dev.ptr(basis))
+    return basis
 
 
 def pack_channels_lowrank(raw, storage: str, basis, packed=None, range_error=None):

@@ -58,6 +58,11 @@ def test_status_strings_and_argument_errors_without_gpu_work():
     # Null pointers are rejected before any device call.
     assert lib.rcabeam_pack_ensemble_device(None, None, 4, 2, 2, 2) == _lib.ERROR_ARGUMENT
     assert lib.rcabeam_opw_device(None, None, 1, 1, 1) == _lib.ERROR_ARGUMENT
+    # Subspace iteration: k in [1, 12], k < frames, k x frames <= 4096, 1 <= iterations <= 1000.
+    fake = ctypes.c_void_p(4096)  # never dereferenced: the shape checks come first
+    assert lib.rcabeam_dominant_subspace_device(None, 200, 4, 12, fake) == _lib.ERROR_ARGUMENT
+    for frames, k, iterations in ((200, 0, 12), (200, 13, 12), (4, 4, 12), (512, 9, 12), (200, 4, 0), (200, 4, 1001)):
+        assert lib.rcabeam_dominant_subspace_device(fake, frames, k, iterations, fake) == _lib.ERROR_ARGUMENT
     with pytest.raises(ValueError, match="invalid argument"):
         _lib.check(_lib.ERROR_ARGUMENT, "probe")
     with pytest.raises(RuntimeError):
