@@ -342,10 +342,41 @@ __global__ __launch_bounds__(kVThreads) __attribute__((amdgpu_waves_per_eu(4))) 
     // wave each (wave k % 8 plans stage k + 2) into a ring of descriptors.
     StageDesc* desc = reinterpret_cast<StageDesc*>(lds + desc_offset(p.nc));
     // Plan the stage after prev (first: the tile's first stage) into out.  Whole wave.
+    // The stage of windows w that starts at the first live angle at or after
+    // a, as plan_stage() packs it, with the angles in lanes: rows per lane,
+    // their inclusive prefix sum (DPP), and ballots for the first live angle
+    // and the first one that no longer fits.  excl: LDS row offset of each
+    // lane's window within the stage.
+    auto pack = [&](const Windows& win, int a, Stage& st, int& excl) __attribute__((always_inline)) {
+      const int r = (lane >= a && lane < p.na && win.lo <= win.hi) ? win.hi - win.lo + 1 : 0;
+      const unsigned long long live = __ballot(r > 0);
+      if (live == 0) {
+        st.valid = false;
+        return;
+      }
+      const int af = __builtin_ctzll(live);
+      const int rf = __builtin_amdgcn_readlane(r, af);
+      if (rf > max_rows) {  // first chunk of a window longer than a stage
+        st = Stage{af, af + 1, __builtin_amdgcn_readlane(win.lo, af), max_rows, false, true};
+        excl = 0;
+        return;
+      }
+      int incl = r;  // lanes 0 .. 15 hold the angles (na <= 16): one DPP row
+      incl += __builtin_amdgcn_update_dpp(0, incl, 0x111, 0xf, 0xf, true);  // row_shr:1
+      incl += __builtin_amdgcn_update_dpp(0, incl, 0x112, 0xf, 0xf, true);  // row_shr:2
+      incl += __builtin_amdgcn_update_dpp(0, incl, 0x114, 0xf, 0xf, true);  // row_shr:4
+      incl += __builtin_amdgcn_update_dpp(0, incl, 0x118, 0xf, 0xf, true);  // row_shr:8
+      const unsigned long long over = __ballot(incl > max_rows);
+      const int a1 = over ? __builtin_ctzll(over) : p.na;
+      st = Stage{af, a1, -1, __builtin_amdgcn_readlane(incl, a1 - 1), true, true};
+      excl = incl - r;
+    };
+    // Plan the stage after prev (first: the tile's first stage) into out.  Whole wave.
     auto plan_after = [&](const StageDesc* prev, StageDesc* out, bool first) __attribute__((always_inline)) {
       int q = -1;
       Windows w{kBig, -kBig};
       Stage st{0, 0, -1, 0, true, false};
+      int excl = 0;
       if (!first) {
         q = uniform(prev->q);
         if (q < 0) {
@@ -357,13 +388,16 @@ __global__ __launch_bounds__(kVThreads) __attribute__((amdgpu_waves_per_eu(4))) 
         if (lane >= 16) w = Windows{kBig, -kBig};
         const Stage cur{uniform(prev->a0), uniform(prev->a1), uniform(prev->r0), uniform(prev->rows),
                         uniform(prev->last) != 0, true};
-        st = next_stage(cur, w, p.na, max_rows);
+        if (cur.r0 >= 0 && !cur.last)
+          st = next_stage(cur, w, p.na, max_rows);  // next chunk of a long window
+        else
+          pack(w, cur.a1, st, excl);
       }
       bool fresh = false;
       if (!st.valid) {
         for (int qq = tile.next_live(q); qq < nq; qq = tile.next_live(qq)) {
           w = tile.tile_windows(qq);
-          st = plan_stage(w, 0, p.na, max_rows);
+          pack(w, 0, st, excl);
           if (st.valid) {
             q = qq;
             fresh = true;
@@ -375,18 +409,11 @@ __global__ __launch_bounds__(kVThreads) __attribute__((amdgpu_waves_per_eu(4))) 
           return;
         }
       }
-      const int rows_l = (lane >= st.a0 && lane < st.a1 && w.lo <= w.hi) ? w.hi - w.lo + 1 : 0;
-      int incl = rows_l;
-      incl += __builtin_amdgcn_update_dpp(0, incl, 0x111, 0xf, 0xf, true);  // row_shr:1
-      incl += __builtin_amdgcn_update_dpp(0, incl, 0x112, 0xf, 0xf, true);  // row_shr:2
-      incl += __builtin_amdgcn_update_dpp(0, incl, 0x114, 0xf, 0xf, true);  // row_shr:4
-      incl += __builtin_amdgcn_update_dpp(0, incl, 0x118, 0xf, 0xf, true);  // row_shr:8
-      const int excl = incl - rows_l;
       if (lane < 16) {
         out->lo[lane] = w.lo;
         out->hi[lane] = w.hi;
         out->off[lane] = excl;
-        out->base[lane] = w.lo <= w.hi ? w.lo - excl : kBig;
+        out->base[lane] = (st.r0 < 0 && lane >= st.a0 && lane < st.a1 && w.lo <= w.hi) ? w.lo - excl : kBig;
       }
       if (fresh) {
         const int slot = uniform(8 * (q - tile.rx_base));
@@ -443,13 +470,19 @@ __global__ __launch_bounds__(kVThreads) __attribute__((amdgpu_waves_per_eu(4))) 
         i += carry ? 1 : 0;
         f = carry ? f - 1.0f : f;
         // A lane-dependent branch around the body (not a `continue`) keeps the
-        // angle loop itself wave-uniform.
-        if (i >= 0 && (i < p.ns - 1 || (i == p.ns - 1 && f == 0.0f)) && i >= t_lo &&
-            i <= t_hi) {
+        // angle loop itself wave-uniform.  One predicate, without short-circuit
+        // branches: 0 <= i < ns - 1 (or i = ns - 1 with f = 0; inactive terms make
+        // i negative) and t_lo <= i <= t_hi, both as unsigned range tests.
+        const bool valid = (static_cast<unsigned>(i) < static_cast<unsigned>(p.ns - 1)) |
+                           ((i == p.ns - 1) & (f == 0.0f));
+        const bool in_chunk = static_cast<unsigned>(i) - static_cast<unsigned>(t_lo) <=
+                              static_cast<unsigned>(t_hi) - static_cast<unsigned>(t_lo);
+        if (valid & in_chunk) {
           Tap tp;
           tp.w = f;
-          tp.c = rx.c * tx.c - rx.s * tx.s;
-          tp.s = rx.c * tx.s + rx.s * tx.c;
+          // Contractions written out, so that code layout cannot change the rounding.
+          tp.c = fmaf(rx.c, tx.c, -(rx.s * tx.s));
+          tp.s = fmaf(rx.s, tx.c, rx.c * tx.s);
           accumulate(tp, static_cast<uint32_t>(i - rb), base);
         }
       }
