@@ -430,9 +430,11 @@ __global__ __launch_bounds__(kVThreads) __attribute__((amdgpu_waves_per_eu(4))) 
         out->last = st.last ? 1 : 0;
       }
     };
-    // Copies of stage d into buf: whole windows one wave each (angle a0 + w,
-    // a0 + w + 8, ...), a chunk of a long window spread over the waves.
-    auto issue_from = [&](const StageDesc* d, float2* buf) __attribute__((always_inline)) {
+    // Copies of stage d into buf: whole windows one wave each (window j of the
+    // stage by wave (first + j) % 8, so a stage's planning wave, first - 1,
+    // copies only its eighth window), a chunk of a long window spread over
+    // the waves.
+    auto issue_from = [&](const StageDesc* d, float2* buf, int first) __attribute__((always_inline)) {
       const int q = uniform(d->q);
       if (q < 0) return;
       const int cfg = q >= p.nc ? 1 : 0;
@@ -443,7 +445,7 @@ __global__ __launch_bounds__(kVThreads) __attribute__((amdgpu_waves_per_eu(4))) 
         copy_rows(plane0 + a0 * plane_size + static_cast<size_t>(r0) * nt, buf, uniform(d->rows));
         return;
       }
-      for (int a = a0 + wave; a < a1; a += kVWaves) {
+      for (int a = a0 + ((wave - first) & (kVWaves - 1)); a < a1; a += kVWaves) {
         const int lo = uniform(d->lo[a]), hi = uniform(d->hi[a]);
         if (lo > hi) continue;
         copy_rows_wave(plane0 + a * plane_size + static_cast<size_t>(lo) * nt, buf + uniform(d->off[a]) * nt,
@@ -493,15 +495,14 @@ __global__ __launch_bounds__(kVThreads) __attribute__((amdgpu_waves_per_eu(4))) 
       plan_after(&desc[0], &desc[1], false);
     }
     __syncthreads();
-    issue_from(&desc[0], stage_a);
+    issue_from(&desc[0], stage_a, 0);
     __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
     __syncthreads();
     int k = 0;  // stage being consumed
     auto step = [&](const float2* now, float2* next) __attribute__((always_inline)) {
       StageDesc* cur = &desc[k % kDescs];
       StageDesc* nxt = &desc[(k + 1) % kDescs];
-      issue_from(nxt, next);
-      if (wave == (k & (kVWaves - 1))) plan_after(nxt, &desc[(k + 2) % kDescs], false);
+      issue_from(nxt, next, (k + 1) & (kVWaves - 1));
       if constexpr (APERTURES) {
         if (!rc_stored && uniform(cur->q) >= p.nc) {  // first CR stage: the RC sums are complete
           store_plane(0);
@@ -509,6 +510,9 @@ __global__ __launch_bounds__(kVThreads) __attribute__((amdgpu_waves_per_eu(4))) 
         }
       }
       consume_from(cur, now);
+      // The planning wave plans after its share of the stage: its consumption
+      // then starts with the others'.
+      if (wave == (k & (kVWaves - 1))) plan_after(nxt, &desc[(k + 2) % kDescs], false);
       __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
       __syncthreads();
       ++k;
