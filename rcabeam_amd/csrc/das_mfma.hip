@@ -323,7 +323,7 @@ __global__ __launch_bounds__(kMThreads) __attribute__((amdgpu_waves_per_eu(2))) 
   ch.q = -1;
   for (int q = next_live(-1); q < nq; q = next_live(q)) {
     ch = setup(q);
-    cur = plan_stage(ch.w, 0, p.na, max_rows);
+    cur = plan_stage_lanes(ch.w, 0, p.na, max_rows);
     if (cur.valid) break;
   }
   int b = 0;
@@ -331,13 +331,13 @@ __global__ __launch_bounds__(kMThreads) __attribute__((amdgpu_waves_per_eu(2))) 
   __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
   __syncthreads();
   while (cur.valid) {
-    Stage nxt = next_stage(cur, ch.w, p.na, max_rows);
+    Stage nxt = next_stage_lanes(cur, ch.w, p.na, max_rows);
     Channel nch = ch;
     bool crossing = false;
     if (!nxt.valid) {
       for (int q = next_live(ch.q); q < nq; q = next_live(q)) {
         nch = setup(q);
-        nxt = plan_stage(nch.w, 0, p.na, max_rows);
+        nxt = plan_stage_lanes(nch.w, 0, p.na, max_rows);
         if (nxt.valid) break;
       }
       crossing = nxt.valid;

@@ -290,7 +290,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
     const W* plane0 = plane_of(q);
     Stage cur = pending;
     if (!cur.valid) {
-      cur = plan_stage(w, 0, p.na, max_rows);
+      cur = plan_stage_lanes(w, 0, p.na, max_rows);
       if (cur.valid) issue_stage<W, STRIDE, kThreads>(plane0, plane_size, p.nt, cur, w, win + b * buffer, vec);
     }
     __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
@@ -298,13 +298,13 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
     pending.valid = false;
     const Tap* tq = taps + (q & 1) * p.na * kTile;
     while (cur.valid) {
-      Stage nxt = next_stage(cur, w, p.na, max_rows);
+      Stage nxt = next_stage_lanes(cur, w, p.na, max_rows);
       bool crossing = false;
       if (nxt.valid) {
         issue_stage<W, STRIDE, kThreads>(plane0, plane_size, p.nt, nxt, w, win + (b ^ 1) * buffer, vec);
       } else if (q + 1 < nq) {
         const Windows w1 = load_windows(bounds + ((q + 1) & 1) * kMaxAngles, p.na);
-        nxt = plan_stage(w1, 0, p.na, max_rows);
+        nxt = plan_stage_lanes(w1, 0, p.na, max_rows);
         if (nxt.valid) issue_stage<W, STRIDE, kThreads>(plane_of(q + 1), plane_size, p.nt, nxt, w1, win + (b ^ 1) * buffer, vec);
         crossing = true;
       }

@@ -122,6 +122,45 @@ __device__ __forceinline__ Stage plan_stage(const Windows& w, int a, int na, int
   return Stage{0, 0, -1, 0, true, false};
 }
 
+// plan_stage() with the angles in lanes (na <= 16, so lanes 0 .. 15 form one
+// DPP row): rows per lane, their inclusive prefix sum, and ballots for the
+// first live angle and the first one that no longer fits, instead of a loop
+// of readlanes.  Same stage; *excl (optional) receives each lane's row offset
+// within it.  Called by the whole wave.
+__device__ __forceinline__ Stage plan_stage_lanes(const Windows& w, int a, int na, int max_rows,
+                                                  int* excl = nullptr) {
+  const int lane = static_cast<int>(threadIdx.x % kWave);
+  const int r = (lane >= a && lane < na && w.lo <= w.hi) ? w.hi - w.lo + 1 : 0;
+  const unsigned long long live = __ballot(r > 0);
+  if (live == 0) return Stage{0, 0, -1, 0, true, false};
+  const int af = __builtin_ctzll(live);
+  const int rf = __builtin_amdgcn_readlane(r, af);
+  if (rf > max_rows) {  // first chunk of a window longer than a stage
+    if (excl) *excl = 0;
+    return Stage{af, af + 1, __builtin_amdgcn_readlane(w.lo, af), max_rows, false, true};
+  }
+  int incl = r;
+  incl += __builtin_amdgcn_update_dpp(0, incl, 0x111, 0xf, 0xf, true);  // row_shr:1
+  incl += __builtin_amdgcn_update_dpp(0, incl, 0x112, 0xf, 0xf, true);  // row_shr:2
+  incl += __builtin_amdgcn_update_dpp(0, incl, 0x114, 0xf, 0xf, true);  // row_shr:4
+  incl += __builtin_amdgcn_update_dpp(0, incl, 0x118, 0xf, 0xf, true);  // row_shr:8
+  const unsigned long long over = __ballot(incl > max_rows);
+  const int a1 = over ? __builtin_ctzll(over) : na;
+  if (excl) *excl = incl - r;
+  return Stage{af, a1, -1, __builtin_amdgcn_readlane(incl, a1 - 1), true, true};
+}
+
+__device__ __forceinline__ Stage next_stage_lanes(const Stage& s, const Windows& w, int na, int max_rows,
+                                                  int* excl = nullptr) {
+  if (s.r0 >= 0 && !s.last) {
+    const int hi = w.at(s.a0).y;
+    const int r0 = s.r0 + max_rows - 1;
+    const bool last = r0 + max_rows - 1 >= hi;
+    return Stage{s.a0, s.a1, r0, last ? hi + 1 - r0 : max_rows, last, true};
+  }
+  return plan_stage_lanes(w, s.a1, na, max_rows, excl);
+}
+
 __device__ __forceinline__ Stage next_stage(const Stage& s, const Windows& w, int na, int max_rows) {
   if (s.r0 >= 0 && !s.last) {
     const int hi = w.at(s.a0).y;
