@@ -100,8 +100,12 @@ __host__ __device__ inline int dynamic_bytes(int nc, bool multi) {
 }
 
 // Rows of nt complex64 samples a stage buffer holds, with room for the frames
-// the last pair block of the last row reads past nt.
-__host__ __device__ inline int stage_rows(int nt, int fpp) { return (kVStageWords / 2 - 16 * fpp) / nt; }
+// the last row's reads cover past nt: pair blocks reach frame 16 fpp, or
+// 16 (fpp - 1) + 8 with a tail block (none past nt = 200).
+__host__ __device__ inline int stage_rows(int nt, int fpp, bool tail) {
+  const int span = tail ? 16 * (fpp - 1) + 8 : 16 * fpp;
+  return (kVStageWords / 2 - (span > nt ? span - nt : 0)) / nt;
+}
 
 // A stage of a single-angle launch: n channel windows (or one chunk of a
 // long window, taps filtered to rows [t_lo, t_hi]) of aperture cfg.
@@ -574,7 +578,7 @@ __global__ __launch_bounds__(kVThreads) __attribute__((amdgpu_waves_per_eu(4))) 
 
 template <int FPP, bool APERTURES, bool TAIL>
 hipError_t launch_pairs(GridArgs a) {
-  a.max_rows = std::min(kVMaxRows, stage_rows(a.nt, FPP));
+  a.max_rows = std::min(kVMaxRows, stage_rows(a.nt, FPP, TAIL));
   if (a.max_rows < kVMinRows) return hipErrorNotSupported;
   const bool multi = a.na == 1;
   const int bytes = dynamic_bytes(a.nc, multi);
