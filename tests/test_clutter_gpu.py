@@ -23,9 +23,11 @@ def _spectrum(values, rows=13, seed=47):
     return (left * values) @ right.conj().T, left, right
 
 
-@pytest.mark.parametrize("shape", [(7,), (3, 7), (19, 6), (2, 4, 6), (2, 1), (16385, 4), (3000, 200)])
+@pytest.mark.parametrize("shape", [(7,), (3, 7), (19, 6), (2, 4, 6), (2, 1), (16385, 4), (3000, 200), (2000, 224),
+                                   (700, 300)])
 def test_basis_and_filter_match_oracle(shape):
-    """complex128 precision: exact float64 MFMA Gram + float64 EVD."""
+    """complex128 precision: exact float64 MFMA Gram + float64 EVD (32-row steps up to 224 frames, 16-row
+    steps beyond)."""
     gc = _gpu()
     rng = np.random.default_rng(8)
     host = (rng.normal(size=shape) + 1j * rng.normal(size=shape)).astype(np.complex64)
