@@ -44,22 +44,28 @@ __device__ __forceinline__ bool in_range<bf16x2>(float v) {
   return fabsf(v) <= 3.3895313892515355e38f;
 }
 
-// Eight lanes per target (channel, angle, sample) row: lane f holds frames
-// f + 8 j, so the k coordinate reductions stay inside 8 lanes; 32 rows per
-// 256-thread workgroup, the basis staged in LDS.  Coordinates and residual in
-// float64 (the row is read once; HBM-bound).
-constexpr int kPackRowsPerBlock = 32;
-constexpr int kPerLaneLR = kMaxFramesLR / 8;  // 32 frames per lane
+// Sixteen lanes per target (channel, angle, sample) row: lane f holds frames
+// f + 16 j (j < FPL), so one load instruction reads 128 contiguous bytes of a
+// row and the K coordinate reductions stay inside 16 lanes; 16 rows per
+// 256-thread workgroup, the basis staged in LDS.  K and FPL are compile-time
+// bounds of k and of the frames per lane (registers for those alone: 191
+// VGPRs when they were sized for k <= 12 and 256 frames).  Coordinates and
+// residual in float64 (the row is read once; HBM-bound).
+constexpr int kPackLanes = 16;
+constexpr int kPackRowsPerBlock = 256 / kPackLanes;
+// Rows of the expansion per 256-thread workgroup (8 lanes per row).
+constexpr int kExpandRowsPerBlock = 32;
+constexpr int kPerLaneLR = kMaxFramesLR / 8;  // expansion: up to 32 frames per lane
 
-template <typename T>
+template <typename T, int K, int FPL>
 __global__ __launch_bounds__(256) void k_pack_lowrank(const float2* __restrict__ src, uint32_t* __restrict__ dst,
                                                       const double2* __restrict__ basis, int k, int* range_error,
                                                       int ns, int nc, int na, int nt, long long rows) {
   extern __shared__ double2 vs[];  // [k][nt]
   for (int i = threadIdx.x; i < k * nt; i += blockDim.x) vs[i] = basis[i];
   __syncthreads();
-  const long long row = static_cast<long long>(blockIdx.x) * kPackRowsPerBlock + threadIdx.x / 8;
-  const int f = threadIdx.x & 7;
+  const long long row = static_cast<long long>(blockIdx.x) * kPackRowsPerBlock + threadIdx.x / kPackLanes;
+  const int f = threadIdx.x & (kPackLanes - 1);
   bool bad = false;
   if (row < rows) {
     const long long s = row % ns;
@@ -69,20 +75,20 @@ __global__ __launch_bounds__(256) void k_pack_lowrank(const float2* __restrict__
     const float2* in = src + ((s * nc + ch) * na + a) * nt;
     const int ntc = nt + 2 * k;
     uint32_t* out = dst + row * ntc;
-    float2 x[kPerLaneLR];
+    float2 x[FPL];
 #pragma unroll
-    for (int j = 0; j < kPerLaneLR; ++j) {
-      const int t = f + 8 * j;
+    for (int j = 0; j < FPL; ++j) {
+      const int t = f + kPackLanes * j;
       x[j] = t < nt ? in[t] : make_float2(0.f, 0.f);
     }
-    double wr[kMaxBasis], wi[kMaxBasis];  // w_q = sum_t c_t V[t, q], in every lane of the row
+    double wr[K], wi[K];  // w_q = sum_t c_t V[t, q], in every lane of the row
 #pragma unroll
-    for (int q = 0; q < kMaxBasis; ++q) {
+    for (int q = 0; q < K; ++q) {
       wr[q] = wi[q] = 0.0;
       if (q < k) {
 #pragma unroll
-        for (int j = 0; j < kPerLaneLR; ++j) {
-          const int t = f + 8 * j;
+        for (int j = 0; j < FPL; ++j) {
+          const int t = f + kPackLanes * j;
           if (t < nt) {
             const double2 b = vs[q * nt + t];
             wr[q] += static_cast<double>(x[j].x) * b.x - static_cast<double>(x[j].y) * b.y;
@@ -90,19 +96,19 @@ __global__ __launch_bounds__(256) void k_pack_lowrank(const float2* __restrict__
           }
         }
 #pragma unroll
-        for (int off = 1; off < 8; off <<= 1) {
+        for (int off = 1; off < kPackLanes; off <<= 1) {
           wr[q] += __shfl_xor(wr[q], off, kWave);
           wi[q] += __shfl_xor(wi[q], off, kWave);
         }
       }
     }
 #pragma unroll
-    for (int j = 0; j < kPerLaneLR; ++j) {
-      const int t = f + 8 * j;
+    for (int j = 0; j < FPL; ++j) {
+      const int t = f + kPackLanes * j;
       if (t < nt) {  // e_t = c_t - sum_q w_q conj(V[t, q])
         double er = x[j].x, ei = x[j].y;
 #pragma unroll
-        for (int q = 0; q < kMaxBasis; ++q) {
+        for (int q = 0; q < K; ++q) {
           if (q < k) {
             const double2 b = vs[q * nt + t];
             er -= wr[q] * b.x + wi[q] * b.y;
@@ -114,10 +120,10 @@ __global__ __launch_bounds__(256) void k_pack_lowrank(const float2* __restrict__
         out[t] = pack2<T>(re, im);
       }
     }
-    // Frames nt + 2q (hi) and nt + 2q + 1 (lo) hold w_q; lane f writes q = f, f + 8.
+    // Frames nt + 2q (hi) and nt + 2q + 1 (lo) hold w_q; lane f writes q = f.
 #pragma unroll
-    for (int q = 0; q < kMaxBasis; ++q) {
-      if (q < k && (q & 7) == f) {
+    for (int q = 0; q < K; ++q) {
+      if (q < k && q == f) {
         const float re = static_cast<float>(wr[q]), im = static_cast<float>(wi[q]);
         bad |= !in_range<T>(re) || !in_range<T>(im);
         const uint32_t hi = pack2<T>(re, im);
@@ -128,6 +134,27 @@ __global__ __launch_bounds__(256) void k_pack_lowrank(const float2* __restrict__
     }
   }
   if (range_error && __any(bad) && (threadIdx.x & (kWave - 1)) == 0) *range_error = 1;
+}
+
+template <typename T, int K>
+void launch_pack_lowrank(unsigned grid, size_t lds, const float2* src, uint32_t* dst, const double2* v, int k,
+                         int* range_error, int ns, int nc, int na, int nt, long long rows) {
+  if (nt <= 13 * kPackLanes)
+    k_pack_lowrank<T, K, 13><<<grid, 256, lds>>>(src, dst, v, k, range_error, ns, nc, na, nt, rows);
+  else
+    k_pack_lowrank<T, K, kMaxFramesLR / kPackLanes><<<grid, 256, lds>>>(src, dst, v, k, range_error, ns, nc, na,
+                                                                         nt, rows);
+}
+
+template <typename T>
+void launch_pack_lowrank_k(unsigned grid, size_t lds, const float2* src, uint32_t* dst, const double2* v, int k,
+                           int* range_error, int ns, int nc, int na, int nt, long long rows) {
+  if (k <= 4)
+    launch_pack_lowrank<T, 4>(grid, lds, src, dst, v, k, range_error, ns, nc, na, nt, rows);
+  else if (k <= 8)
+    launch_pack_lowrank<T, 8>(grid, lds, src, dst, v, k, range_error, ns, nc, na, nt, rows);
+  else
+    launch_pack_lowrank<T, kMaxBasis>(grid, lds, src, dst, v, k, range_error, ns, nc, na, nt, rows);
 }
 
 // Eight lanes per voxel row of the coded delay-and-sum output (nt + 2k
@@ -142,7 +169,7 @@ __global__ __launch_bounds__(256) void k_expand(const float2* __restrict__ coded
   extern __shared__ double2 vs[];  // [k][nt]
   for (int i = threadIdx.x; i < k * nt; i += blockDim.x) vs[i] = basis[i];
   __syncthreads();
-  const long long row = static_cast<long long>(blockIdx.x) * kPackRowsPerBlock + threadIdx.x / 8;
+  const long long row = static_cast<long long>(blockIdx.x) * kExpandRowsPerBlock + threadIdx.x / 8;
   const int f = threadIdx.x & 7;
   if (row >= rows) return;  // after the only barrier
   const int ntc = nt + 2 * k;
@@ -331,17 +358,19 @@ extern "C" rcabeam_status rcabeam_pack_ensemble_lowrank_device(const void* sourc
   if (k > kMaxBasis || nt > kMaxFramesLR || ns * nc * na > (1ull << 40)) return RCABEAM_ERROR_ARGUMENT;
   if (storage != RCABEAM_STORAGE_FP16 && storage != RCABEAM_STORAGE_BF16) return RCABEAM_ERROR_ARGUMENT;
   const long long rows = static_cast<long long>(ns * nc * na);
-  const unsigned grid = static_cast<unsigned>((rows + kPackRowsPerBlock - 1) / kPackRowsPerBlock);
+  const long long blocks = (rows + kPackRowsPerBlock - 1) / kPackRowsPerBlock;
+  if (blocks > 0x7fffffffLL) return RCABEAM_ERROR_ARGUMENT;
+  const unsigned grid = static_cast<unsigned>(blocks);
   const size_t lds = std::max<size_t>(k * nt * sizeof(double2), sizeof(double2));  // <= 48 KiB
   const double2* v = static_cast<const double2*>(basis);
+  const float2* src = static_cast<const float2*>(source);
+  uint32_t* dst = static_cast<uint32_t*>(target);
   if (storage == RCABEAM_STORAGE_FP16)
-    k_pack_lowrank<__half2><<<grid, 256, lds>>>(static_cast<const float2*>(source), static_cast<uint32_t*>(target),
-                                                v, static_cast<int>(k), range_error, static_cast<int>(ns),
-                                                static_cast<int>(nc), static_cast<int>(na), static_cast<int>(nt), rows);
+    launch_pack_lowrank_k<__half2>(grid, lds, src, dst, v, static_cast<int>(k), range_error, static_cast<int>(ns),
+                                   static_cast<int>(nc), static_cast<int>(na), static_cast<int>(nt), rows);
   else
-    k_pack_lowrank<bf16x2><<<grid, 256, lds>>>(static_cast<const float2*>(source), static_cast<uint32_t*>(target),
-                                               v, static_cast<int>(k), range_error, static_cast<int>(ns),
-                                               static_cast<int>(nc), static_cast<int>(na), static_cast<int>(nt), rows);
+    launch_pack_lowrank_k<bf16x2>(grid, lds, src, dst, v, static_cast<int>(k), range_error, static_cast<int>(ns),
+                                  static_cast<int>(nc), static_cast<int>(na), static_cast<int>(nt), rows);
   return launch_status();
 }
 
@@ -351,7 +380,7 @@ extern "C" rcabeam_status rcabeam_lowrank_expand_device(const void* coded, const
     return RCABEAM_ERROR_ARGUMENT;
   if (out_storage < RCABEAM_STORAGE_C64 || out_storage > RCABEAM_STORAGE_BF16) return RCABEAM_ERROR_ARGUMENT;
   const long long rows = static_cast<long long>(nv);
-  const unsigned grid = static_cast<unsigned>((rows + kPackRowsPerBlock - 1) / kPackRowsPerBlock);
+  const unsigned grid = static_cast<unsigned>((rows + kExpandRowsPerBlock - 1) / kExpandRowsPerBlock);
   const size_t lds = std::max<size_t>(k * nt * sizeof(double2), sizeof(double2));
   const float2* c = static_cast<const float2*>(coded);
   const double2* v = static_cast<const double2*>(basis);
