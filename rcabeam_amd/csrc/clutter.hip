@@ -521,42 +521,43 @@ __global__ __launch_bounds__(256) void k_project(const T* __restrict__ x, long l
 }
 
 // The same projection for short rows (frames <= 256) and few vectors (r <= 8),
-// the clutter filter's usual shape: 8 lanes per row (lane f holds frames
-// f + 8 j), 8 rows per wave, so the three coordinate reductions per vector
-// stay inside 8 lanes; the vectors are staged in LDS once per workgroup.  The
-// residual x_t - sum_k c_k conj(V[t, k]) is formed frame by frame in float64
-// from the float32 row kept in registers.
-constexpr int kProjectRowsPerBlock = 32;  // 256 threads, 8 lanes per row
+// the clutter filter's usual shape: L lanes per row (lane f holds frames
+// f + L j, j < kPer; 8 lanes up to 128 frames, 16 beyond, so a load reads 64
+// or 128 contiguous bytes of a row), 64 / L rows per wave, so the coordinate
+// reductions stay inside the row's lanes; the vectors are staged in LDS once
+// per workgroup.  The residual x_t - sum_k c_k conj(V[t, k]) is formed frame
+// by frame in float64 from the float32 row kept in registers.  R bounds r at
+// compile time (registers for R coordinates only).
 constexpr int kProjectMaxR = 8;
 
-template <typename T, bool FILTER, int kPer>
+template <typename T, bool FILTER, int L, int kPer, int R>
 __global__ __launch_bounds__(256) void k_project_rows8(const T* __restrict__ x, long long rows, int frames,
                                                        const double2* __restrict__ v, int r, int subtract,
                                                        double* __restrict__ power, float2* __restrict__ filtered) {
   extern __shared__ double2 vs[];  // [r][frames]
   for (int i = threadIdx.x; i < r * frames; i += blockDim.x) vs[i] = v[i];
   __syncthreads();
-  const long long row = static_cast<long long>(blockIdx.x) * kProjectRowsPerBlock + threadIdx.x / 8;
-  const int f = threadIdx.x & 7;
+  const long long row = static_cast<long long>(blockIdx.x) * (256 / L) + threadIdx.x / L;
+  const int f = threadIdx.x & (L - 1);
   if (row >= rows) return;  // after the only barrier
   const T* xr = x + row * frames;
   float2 xv[kPer];
 #pragma unroll
   for (int j = 0; j < kPer; ++j) {
-    const int t = f + 8 * j;
+    const int t = f + L * j;
     xv[j] = t < frames ? load_sample(xr + t) : make_float2(0.f, 0.f);
   }
-  // Coordinates c_k = sum_t x_t V[t, k] of this row (all 8 lanes hold them after the reduction).
-  double cre[kProjectMaxR], cim[kProjectMaxR];
+  // Coordinates c_k = sum_t x_t V[t, k] of this row (all L lanes hold them after the reduction).
+  double cre[R], cim[R];
   double kept = 0.0;
 #pragma unroll
-  for (int k = 0; k < kProjectMaxR; ++k) {
+  for (int k = 0; k < R; ++k) {
     cre[k] = cim[k] = 0.0;
     if (k < r) {
       const double2* vk = vs + k * frames;
 #pragma unroll
       for (int j = 0; j < kPer; ++j) {
-        const int t = f + 8 * j;
+        const int t = f + L * j;
         if (t < frames) {
           const double2 b = vk[t];
           const double xr_ = xv[j].x, xi_ = xv[j].y;
@@ -565,7 +566,7 @@ __global__ __launch_bounds__(256) void k_project_rows8(const T* __restrict__ x, 
         }
       }
 #pragma unroll
-      for (int off = 1; off < 8; off <<= 1) {
+      for (int off = 1; off < L; off <<= 1) {
         cre[k] += __shfl_xor(cre[k], off, kWave);
         cim[k] += __shfl_xor(cim[k], off, kWave);
       }
@@ -580,12 +581,12 @@ __global__ __launch_bounds__(256) void k_project_rows8(const T* __restrict__ x, 
   double e = 0.0;
 #pragma unroll
   for (int j = 0; j < kPer; ++j) {
-    const int t = f + 8 * j;
+    const int t = f + L * j;
     if (t < frames) {
       double re = subtract ? static_cast<double>(xv[j].x) : 0.0;
       double im = subtract ? static_cast<double>(xv[j].y) : 0.0;
 #pragma unroll
-      for (int k = 0; k < kProjectMaxR; ++k) {
+      for (int k = 0; k < R; ++k) {
         if (k < r) {
           const double2 b = vs[k * frames + t];  // c_k conj(V[t, k]) (row-vector convention x V V^H)
           re += sign * (cre[k] * b.x + cim[k] * b.y);
@@ -600,7 +601,7 @@ __global__ __launch_bounds__(256) void k_project_rows8(const T* __restrict__ x, 
   }
   if constexpr (!FILTER) {
 #pragma unroll
-    for (int off = 1; off < 8; off <<= 1) e += __shfl_xor(e, off, kWave);
+    for (int off = 1; off < L; off <<= 1) e += __shfl_xor(e, off, kWave);
     if (f == 0) power[row] = e / frames;
   }
 }
@@ -837,16 +838,24 @@ static rcabeam_status project(const void* iq, int storage, size_t rows, size_t f
   float2* out = static_cast<float2*>(filtered);
   const int ri = static_cast<int>(r);
   if (frames <= 256 && r <= static_cast<size_t>(kProjectMaxR)) {
-    // Short rows, few vectors: 8 lanes per row, vectors in LDS.
-    const unsigned blocks = static_cast<unsigned>((rows + kProjectRowsPerBlock - 1) / kProjectRowsPerBlock);
+    // Short rows, few vectors: 8 or 16 lanes per row, vectors in LDS.
+    const int lanes = frames <= 128 ? 8 : 16;
+    const unsigned blocks = static_cast<unsigned>((rows + 256 / lanes - 1) / (256 / lanes));
     const size_t lds = std::max<size_t>(r * frames * sizeof(double2), sizeof(double2));
-#define RCB_PROJECT8(TYPE, PER)                                                                             \
-  k_project_rows8<TYPE, FILTER, PER><<<blocks, 256, lds>>>(static_cast<const TYPE*>(iq), n, t, v, ri, subtract, \
-                                                            power, out)
-#define RCB_PROJECT8_SHAPES(TYPE)              \
-  if (frames <= 64) RCB_PROJECT8(TYPE, 8);      \
-  else if (frames <= 128) RCB_PROJECT8(TYPE, 16); \
-  else RCB_PROJECT8(TYPE, 32)
+#define RCB_PROJECT8(TYPE, L, PER)                                                                           \
+  do {                                                                                                        \
+    if (ri <= 4)                                                                                              \
+      k_project_rows8<TYPE, FILTER, L, PER, 4><<<blocks, 256, lds>>>(static_cast<const TYPE*>(iq), n, t, v, ri, \
+                                                                     subtract, power, out);                  \
+    else                                                                                                      \
+      k_project_rows8<TYPE, FILTER, L, PER, kProjectMaxR><<<blocks, 256, lds>>>(static_cast<const TYPE*>(iq), n, \
+                                                                                t, v, ri, subtract, power, out); \
+  } while (0)
+#define RCB_PROJECT8_SHAPES(TYPE)                    \
+  if (frames <= 64) RCB_PROJECT8(TYPE, 8, 8);         \
+  else if (frames <= 128) RCB_PROJECT8(TYPE, 8, 16);  \
+  else if (frames <= 208) RCB_PROJECT8(TYPE, 16, 13); \
+  else RCB_PROJECT8(TYPE, 16, 16)
     switch (storage) {
       case RCABEAM_STORAGE_C64:
         RCB_PROJECT8_SHAPES(float2);
