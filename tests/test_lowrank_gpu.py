@@ -27,14 +27,16 @@ def _device_basis(v):
     return dev.to_device(np.ascontiguousarray(v.T), torch.complex128)  # (k, nt): row j = column j
 
 
-def test_pack_matches_numpy_coding():
+@pytest.mark.parametrize("nt,k", [(40, 3), (100, 6), (208, 12), (230, 4)])
+def test_pack_matches_numpy_coding(nt, k):
+    """Every compile-time bound of the pack kernel: k <= 4, 8, 12 and up to 13 or 16 frames per lane."""
     import torch
 
     from rcabeam_amd import _device as dev
     from rcabeam_amd.lowrank import pack_channels_lowrank
 
-    rng = np.random.default_rng(3)
-    ns, nc, na, nt, k = 9, 5, 3, 40, 3
+    rng = np.random.default_rng(3 + nt + k)
+    ns, nc, na = 9, 5, 3
     v = _basis(rng, nt, k)
     # rows dominated by the basis modes (clutter) plus a small residual
     coef = (rng.normal(size=(ns, nc, na, k)) + 1j * rng.normal(size=(ns, nc, na, k))) * 30
