@@ -159,17 +159,6 @@ __global__ __launch_bounds__(kMThreads) __attribute__((amdgpu_waves_per_eu(2))) 
   };
   const size_t plane_size = static_cast<size_t>(p.ns) * nt;
 
-  // Copy a stage into buf: windows of angles [a0, a1) back to back (or one
-  // chunk of a long window).  A window is contiguous in HBM and in LDS (row
-  // stride nt), so it is copied in 1 KiB global_load_lds_dwordx4
-  // instructions regardless of row boundaries, spread over the waves.
-  // Copy a stage into buf: windows of angles [a0, a1) back to back (or one
-  // chunk of a long window), contiguous in HBM and in LDS (row stride nt),
-  // 1 KiB global_load_lds_dwordx4 instructions.  Each window is copied whole
-  // by one wave (angle a0 + g, a0 + g + 4, ... by wave g); its LDS offset is
-  // the exclusive prefix of the row counts over lanes a0 .. a - 1 (lane a
-  // holds angle a's window; na <= 16 keeps the scan inside one DPP row).  A
-  // chunk of a long window is spread over all waves.
   // Copy `words` 32-bit words (a multiple of 4) from HBM to LDS, 1 KiB per
   // instruction: chunks first, first + step, ... of this wave; whole chunks
   // need no lane mask, only the last (partial) chunk does.
@@ -191,6 +180,13 @@ __global__ __launch_bounds__(kMThreads) __attribute__((amdgpu_waves_per_eu(2))) 
       for (int i = first * kWave + lane; i < words; i += step * kWave) dst[i] = src[i];
     }
   };
+  // Copy a stage into buf: windows of angles [a0, a1) back to back (or one
+  // chunk of a long window), contiguous in HBM and in LDS (row stride nt),
+  // 1 KiB global_load_lds_dwordx4 instructions.  Each window is copied whole
+  // by one wave (angle a0 + g, a0 + g + 4, ... by wave g); its LDS offset is
+  // the exclusive prefix of the row counts over lanes a0 .. a - 1 (lane a
+  // holds angle a's window; na <= 16 keeps the scan inside one DPP row).  A
+  // chunk of a long window is spread over all waves.
   auto issue = [&](const Channel& ch, const Stage& s, W* buf) {
     const W* plane0 = plane_of(ch);
     const int a0 = uniform(s.a0), a1 = uniform(s.a1), sr0 = uniform(s.r0), srows = uniform(s.rows);
