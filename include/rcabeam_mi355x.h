@@ -220,7 +220,7 @@ rcabeam_status rcabeam_lowrank_expand_device(const void* coded, const void* basi
  * round; the low-rank channel coding uses it for its basis (any orthonormal
  * basis codes exactly; one that catches the strong modes keeps the 16-bit
  * residual small).  No reference counterpart.  1 <= k <= 12, k < frames,
- * k * frames <= 4096. */
+ * k * frames <= 4096, 1 <= iterations <= 1000, 16-byte aligned buffers. */
 rcabeam_status rcabeam_dominant_subspace_device(const void* gram, size_t frames, size_t k, size_t iterations,
                                                 void* basis);
 

@@ -341,7 +341,8 @@ using namespace rcb;
 extern "C" rcabeam_status rcabeam_dominant_subspace_device(const void* gram, size_t frames, size_t k,
                                                            size_t iterations, void* basis) {
   if (!gram || !basis || !frames || !k || k > static_cast<size_t>(kSubMaxK) || k >= frames ||
-      k * frames > static_cast<size_t>(kSubMaxElems) || !iterations || iterations > 1000)
+      k * frames > static_cast<size_t>(kSubMaxElems) || !iterations || iterations > 1000 ||
+      reinterpret_cast<uintptr_t>(gram) % 16 || reinterpret_cast<uintptr_t>(basis) % 16)
     return RCABEAM_ERROR_ARGUMENT;
   const size_t lds = (2 * k * frames + kSubWaves * kSubMaxK) * sizeof(double2);
   const hipError_t e = allow_dynamic_lds(reinterpret_cast<const void*>(&k_subspace), static_cast<int>(lds));

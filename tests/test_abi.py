@@ -63,6 +63,7 @@ def test_status_strings_and_argument_errors_without_gpu_work():
     assert lib.rcabeam_dominant_subspace_device(None, 200, 4, 12, fake) == _lib.ERROR_ARGUMENT
     for frames, k, iterations in ((200, 0, 12), (200, 13, 12), (4, 4, 12), (512, 9, 12), (200, 4, 0), (200, 4, 1001)):
         assert lib.rcabeam_dominant_subspace_device(fake, frames, k, iterations, fake) == _lib.ERROR_ARGUMENT
+    assert lib.rcabeam_dominant_subspace_device(ctypes.c_void_p(4104), 200, 4, 12, fake) == _lib.ERROR_ARGUMENT
     with pytest.raises(ValueError, match="invalid argument"):
         _lib.check(_lib.ERROR_ARGUMENT, "probe")
     with pytest.raises(RuntimeError):
