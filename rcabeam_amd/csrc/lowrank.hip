@@ -53,9 +53,9 @@ __device__ __forceinline__ bool in_range<bf16x2>(float v) {
 // residual in float64 (the row is read once; HBM-bound).
 constexpr int kPackLanes = 16;
 constexpr int kPackRowsPerBlock = 256 / kPackLanes;
-// Rows of the expansion per 256-thread workgroup (8 lanes per row).
-constexpr int kExpandRowsPerBlock = 32;
-constexpr int kPerLaneLR = kMaxFramesLR / 8;  // expansion: up to 32 frames per lane
+// The expansion: 16 lanes per row, 16 rows per 256-thread workgroup.
+constexpr int kExpandLanes = 16;
+constexpr int kExpandRowsPerBlock = 256 / kExpandLanes;
 
 template <typename T, int K, int FPL>
 __global__ __launch_bounds__(256) void k_pack_lowrank(const float2* __restrict__ src, uint32_t* __restrict__ dst,
@@ -157,48 +157,51 @@ void launch_pack_lowrank_k(unsigned grid, size_t lds, const float2* src, uint32_
     launch_pack_lowrank<T, kMaxBasis>(grid, lds, src, dst, v, k, range_error, ns, nc, na, nt, rows);
 }
 
-// Eight lanes per voxel row of the coded delay-and-sum output (nt + 2k
-// complex64 columns), lane f holding frames f + 8 j; 32 rows per workgroup,
-// the basis staged in LDS: x_t = d_t + sum_j (d[nt + 2j] + d[nt + 2j + 1])
-// conj(V[t, j]), stored in `O`; power (mean over the nt frames of |x_t|^2)
-// into pd[2v].
+// Sixteen lanes per voxel row of the coded delay-and-sum output (nt + 2k
+// complex64 columns), lane f holding frames f + 16 j, so one load reads 128
+// contiguous bytes of a row; 16 rows per workgroup, the basis staged in LDS
+// as float32 (converted once per workgroup, the values each frame used to
+// convert): x_t = d_t + sum_j (d[nt + 2j] + d[nt + 2j + 1]) conj(V[t, j]),
+// stored in `O`; power (mean over the nt frames of |x_t|^2) into pd[2v].
 template <typename O>
 __global__ __launch_bounds__(256) void k_expand(const float2* __restrict__ coded, const double2* __restrict__ basis,
                                                 int k, O* __restrict__ out, float* __restrict__ pd, int nt,
                                                 long long rows) {
-  extern __shared__ double2 vs[];  // [k][nt]
-  for (int i = threadIdx.x; i < k * nt; i += blockDim.x) vs[i] = basis[i];
+  extern __shared__ float2 vf[];  // [k][nt]
+  for (int i = threadIdx.x; i < k * nt; i += blockDim.x) {
+    const double2 b = basis[i];
+    vf[i] = make_float2(static_cast<float>(b.x), static_cast<float>(b.y));
+  }
   __syncthreads();
-  const long long row = static_cast<long long>(blockIdx.x) * kExpandRowsPerBlock + threadIdx.x / 8;
-  const int f = threadIdx.x & 7;
+  const long long row = static_cast<long long>(blockIdx.x) * kExpandRowsPerBlock + threadIdx.x / kExpandLanes;
+  const int f = threadIdx.x & (kExpandLanes - 1);
   if (row >= rows) return;  // after the only barrier
   const int ntc = nt + 2 * k;
   const float2* in = coded + row * ntc;
-  // The delay-and-sum of w_j (hi + lo parts), read by lane j % 8 and shared in the row's 8 lanes.
+  // The delay-and-sum of w_j (hi + lo parts), read by lane j and shared in the row's 16 lanes.
   float dwr[kMaxBasis], dwi[kMaxBasis];
 #pragma unroll
   for (int q = 0; q < kMaxBasis; ++q) {
     float2 d = make_float2(0.f, 0.f);
-    if (q < k && (q & 7) == f) {
+    if (q < k && q == f) {
       const float2 h = in[nt + 2 * q], l = in[nt + 2 * q + 1];
       d = make_float2(h.x + l.x, h.y + l.y);
     }
-    dwr[q] = __shfl(d.x, (threadIdx.x & (kWave - 8)) | (q & 7), kWave);
-    dwi[q] = __shfl(d.y, (threadIdx.x & (kWave - 8)) | (q & 7), kWave);
+    dwr[q] = __shfl(d.x, (threadIdx.x & (kWave - kExpandLanes)) | q, kWave);
+    dwi[q] = __shfl(d.y, (threadIdx.x & (kWave - kExpandLanes)) | q, kWave);
   }
   float power = 0.f;
 #pragma unroll 4
-  for (int j = 0; j < kPerLaneLR; ++j) {
-    const int t = f + 8 * j;
+  for (int j = 0; j < kMaxFramesLR / kExpandLanes; ++j) {
+    const int t = f + kExpandLanes * j;
     if (t < nt) {
       float2 x = in[t];
 #pragma unroll
       for (int q = 0; q < kMaxBasis; ++q) {
         if (q < k) {
-          const double2 b = vs[q * nt + t];
-          const float br = static_cast<float>(b.x), bi = static_cast<float>(b.y);
-          x.x = fmaf(dwr[q], br, fmaf(dwi[q], bi, x.x));  // + w conj(b)
-          x.y = fmaf(dwi[q], br, fmaf(-dwr[q], bi, x.y));
+          const float2 b = vf[q * nt + t];
+          x.x = fmaf(dwr[q], b.x, fmaf(dwi[q], b.y, x.x));  // + w conj(b)
+          x.y = fmaf(dwi[q], b.x, fmaf(-dwr[q], b.y, x.y));
         }
       }
       power = fmaf(x.x, x.x, fmaf(x.y, x.y, power));
@@ -206,7 +209,7 @@ __global__ __launch_bounds__(256) void k_expand(const float2* __restrict__ coded
     }
   }
 #pragma unroll
-  for (int off = 1; off < 8; off <<= 1) power += __shfl_xor(power, off, kWave);
+  for (int off = 1; off < kExpandLanes; off <<= 1) power += __shfl_xor(power, off, kWave);
   if (pd && f == 0) {
     pd[2 * row] = power / static_cast<float>(nt);
     pd[2 * row + 1] = 0.f;
@@ -382,7 +385,7 @@ extern "C" rcabeam_status rcabeam_lowrank_expand_device(const void* coded, const
   if (out_storage < RCABEAM_STORAGE_C64 || out_storage > RCABEAM_STORAGE_BF16) return RCABEAM_ERROR_ARGUMENT;
   const long long rows = static_cast<long long>(nv);
   const unsigned grid = static_cast<unsigned>((rows + kExpandRowsPerBlock - 1) / kExpandRowsPerBlock);
-  const size_t lds = std::max<size_t>(k * nt * sizeof(double2), sizeof(double2));
+  const size_t lds = std::max<size_t>(k * nt * sizeof(float2), sizeof(float2));
   const float2* c = static_cast<const float2*>(coded);
   const double2* v = static_cast<const double2*>(basis);
   const int ki = static_cast<int>(k), ti = static_cast<int>(nt);
