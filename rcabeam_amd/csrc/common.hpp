@@ -136,17 +136,17 @@ __device__ __forceinline__ float2 load_sample<bf16x2>(const bf16x2* p) {
   return make_float2(__uint_as_float(b << 16), __uint_as_float(b & 0xffff0000u));
 }
 
-__device__ __forceinline__ uint32_t bf16_bits_rne(float f) {
-  uint32_t u = __float_as_uint(f);
-  if ((u & 0x7f800000u) == 0x7f800000u) return (u >> 16) | ((u & 0xffffu) ? 0x40u : 0u);  // inf / nan
-  return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+// Two float32 values rounded to nearest even into one word of bfloat16 (lo
+// in the low half): one gfx950 v_cvt_pk_bf16_f32.
+typedef __bf16 bf16v2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t bf16_pack2_rne(float lo, float hi) {
+  const bf16v2 v = {static_cast<__bf16>(lo), static_cast<__bf16>(hi)};
+  return __builtin_bit_cast(uint32_t, v);
 }
 
 __device__ __forceinline__ void store_sample(float2* p, float2 v) { *p = v; }
 __device__ __forceinline__ void store_sample(__half2* p, float2 v) { *p = __floats2half2_rn(v.x, v.y); }
-__device__ __forceinline__ void store_sample(bf16x2* p, float2 v) {
-  p->bits = bf16_bits_rne(v.x) | (bf16_bits_rne(v.y) << 16);
-}
+__device__ __forceinline__ void store_sample(bf16x2* p, float2 v) { p->bits = bf16_pack2_rne(v.x, v.y); }
 
 // Wave-wide sum (64 lanes) by butterfly shuffles; every lane gets the total.
 __device__ __forceinline__ float wave_sum(float x) {

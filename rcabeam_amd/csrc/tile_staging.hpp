@@ -65,7 +65,7 @@ __device__ __forceinline__ uint32_t pack2<__half2>(float lo, float hi) {
 }
 template <>
 __device__ __forceinline__ uint32_t pack2<bf16x2>(float lo, float hi) {
-  return bf16_bits_rne(lo) | (bf16_bits_rne(hi) << 16);
+  return bf16_pack2_rne(lo, hi);
 }
 
 __device__ __forceinline__ int wave_min(int x) {
