@@ -54,13 +54,6 @@ struct GramPlan {
 
 enum GramMode { kSplitBf16 = 0, kExactF16 = 1, kExactBf16 = 2 };
 
-__device__ __forceinline__ unsigned short bf16_rn_bits(float f) {
-  uint32_t u = __float_as_uint(f);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return static_cast<unsigned short>(u >> 16);
-}
-__device__ __forceinline__ float bf16_bits_to_float(unsigned short b) { return __uint_as_float(static_cast<uint32_t>(b) << 16); }
-
 // MODE kSplitBf16: T = float2 (complex64), hi/lo bfloat16.
 // MODE kExactF16:  T = __half2, one fp16 product.
 // MODE kExactBf16: T = bf16x2, one bf16 product.
@@ -105,27 +98,29 @@ __global__ __launch_bounds__(kGramThreads) void k_gram(const T* __restrict__ x, 
         v[j] = r < row_end ? load_sample(x + r * row_step * frames + t) : make_float2(0.f, 0.f);
       }
       uint32_t hre[4] = {0u, 0u, 0u, 0u}, him[4] = {0u, 0u, 0u, 0u}, lre[4] = {0u, 0u, 0u, 0u}, lim[4] = {0u, 0u, 0u, 0u};
+      if constexpr (MODE == kExactF16) {
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        unsigned short hr, hi_i, lr = 0, li = 0;
-        if constexpr (MODE == kExactF16) {
-          hr = __half_as_ushort(__float2half_rn(v[j].x));
-          hi_i = __half_as_ushort(__float2half_rn(v[j].y));
-        } else {
-          const float vx = static_cast<float>(static_cast<double>(v[j].x) * inv_scale);
-          const float vy = static_cast<float>(static_cast<double>(v[j].y) * inv_scale);
-          hr = bf16_rn_bits(vx);
-          hi_i = bf16_rn_bits(vy);
+        for (int j = 0; j < 8; ++j) {
+          const int sh = 16 * (j & 1);
+          hre[j >> 1] |= static_cast<uint32_t>(__half_as_ushort(__float2half_rn(v[j].x))) << sh;
+          him[j >> 1] |= static_cast<uint32_t>(__half_as_ushort(__float2half_rn(v[j].y))) << sh;
+        }
+      } else {
+        // Rows 2 m and 2 m + 1 of a column share a word: one v_cvt_pk_bf16_f32 per word (round to
+        // nearest even, as the integer form), and the low parts from the words' high halves.
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+          const float x0 = static_cast<float>(static_cast<double>(v[2 * m].x) * inv_scale);
+          const float x1 = static_cast<float>(static_cast<double>(v[2 * m + 1].x) * inv_scale);
+          const float y0 = static_cast<float>(static_cast<double>(v[2 * m].y) * inv_scale);
+          const float y1 = static_cast<float>(static_cast<double>(v[2 * m + 1].y) * inv_scale);
+          hre[m] = bf16_pack2_rne(x0, x1);
+          him[m] = bf16_pack2_rne(y0, y1);
           if constexpr (kSplit) {
-            lr = bf16_rn_bits(vx - bf16_bits_to_float(hr));
-            li = bf16_rn_bits(vy - bf16_bits_to_float(hi_i));
+            lre[m] = bf16_pack2_rne(x0 - __uint_as_float(hre[m] << 16), x1 - __uint_as_float(hre[m] & 0xffff0000u));
+            lim[m] = bf16_pack2_rne(y0 - __uint_as_float(him[m] << 16), y1 - __uint_as_float(him[m] & 0xffff0000u));
           }
         }
-        const int sh = 16 * (j & 1);
-        hre[j >> 1] |= static_cast<uint32_t>(hr) << sh;
-        him[j >> 1] |= static_cast<uint32_t>(hi_i) << sh;
-        lre[j >> 1] |= static_cast<uint32_t>(lr) << sh;
-        lim[j >> 1] |= static_cast<uint32_t>(li) << sh;
       }
       *reinterpret_cast<uint4*>(hi + (2 * t) * kGramKPad + 8 * h) = make_uint4(hre[0], hre[1], hre[2], hre[3]);
       *reinterpret_cast<uint4*>(hi + (2 * t + 1) * kGramKPad + 8 * h) = make_uint4(him[0], him[1], him[2], him[3]);
