@@ -313,13 +313,17 @@ __global__ __launch_bounds__(kG64Threads) void k_gram64(const float2* __restrict
   const int base = count / kG64Waves, extra = count % kG64Waves;
   const int my_count = base + (wave < extra ? 1 : 0);
   const int my_start = first + wave * base + (wave < extra ? wave : extra);
-  int bi[kG64MaxBlocks], bj[kG64MaxBlocks];
+  // Stage offsets of the two 16-column slices each block multiplies.  Blocks
+  // past my_count read slice 0 and are computed but never stored, so every
+  // wave runs one straight-line sequence per 4 rows: all operand reads, then
+  // the MFMAs back to back (no LDS wait between them).
+  int aoff[kG64MaxBlocks], boff[kG64MaxBlocks];
   {
     int2 ij = triangle_block(my_start, shape.nb);
 #pragma unroll
     for (int b = 0; b < kG64MaxBlocks; ++b) {
-      bi[b] = ij.x;
-      bj[b] = ij.y;
+      aoff[b] = b < my_count ? 16 * ij.x : 0;
+      boff[b] = b < my_count ? 16 * ij.y : 0;
       if (++ij.y == shape.nb) {
         ++ij.x;
         ij.y = ij.x;
@@ -361,20 +365,31 @@ __global__ __launch_bounds__(kG64Threads) void k_gram64(const float2* __restrict
     }
     __syncthreads();
     if (s + 1 < steps) fetch(s + 1);
+    // This lane's first stage element, opaque to the compiler, so that the
+    // 14 read addresses of each 4-row block are formed where they are used
+    // instead of all being hoisted out of the step loop (and spilled).
+    int lane_elem = (lane / 16) * ld + (lane & 15);
+    asm volatile("" : "+v"(lane_elem));
+    // Operand reads in flight per group of blocks: all 7 where the prefetch
+    // leaves room in the 128 registers, 4 then 3 otherwise.
+    constexpr int kGroup = kPrefetch > 8 ? 4 : kG64MaxBlocks;
 #pragma unroll
     for (int kk = 0; kk < ROWS; kk += 4) {
-      const double* rowp = stage + (kk + lane / 16) * ld + (lane & 15);
-      int prev = -1;
-      double a = 0.0;
+      const double* rowp = stage + kk * ld + lane_elem;
 #pragma unroll
-      for (int b = 0; b < kG64MaxBlocks; ++b) {
-        if (b < my_count) {
-          if (bi[b] != prev) {
-            a = rowp[16 * bi[b]];
-            prev = bi[b];
+      for (int b0 = 0; b0 < kG64MaxBlocks; b0 += kGroup) {
+        double a[kGroup], bv[kGroup];
+#pragma unroll
+        for (int b = 0; b < kGroup; ++b)
+          if (b0 + b < kG64MaxBlocks) {
+            a[b] = rowp[aoff[b0 + b]];
+            bv[b] = rowp[boff[b0 + b]];
           }
-          acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, rowp[16 * bj[b]], acc[b], 0, 0, 0);
-        }
+#pragma unroll
+        for (int b = 0; b < kGroup; ++b)
+          if (b0 + b < kG64MaxBlocks)
+            acc[b0 + b] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[b], bv[b], acc[b0 + b], 0, 0, 0);
+        __builtin_amdgcn_sched_barrier(0);  // keep the next reads after these MFMAs
       }
     }
   }
