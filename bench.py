@@ -246,11 +246,14 @@ class Pipeline:
             self._call("rcabeam_lowrank_expand_device", dev.ptr(self.coded), dev.ptr(self.basis), self.k,
                        dev.ptr(self.signal), 0, dev.ptr(self.pd), nv, self.nt)
             mark("expand")
-        self._call("rcabeam_iq_absmax_device", dev.ptr(self.signal), self.code, nv * self.nt, dev.ptr(self.stats), 0)
         if self.exact_gram:
+            # Float64 products and sums of complex64 samples cannot overflow, so the exact Gram needs no
+            # |max| scaling pass (its eigenvectors, all the filter uses, do not depend on the scale).
             self._call("rcabeam_temporal_gram_f64_device", dev.ptr(self.signal), nv, self.nt, dev.ptr(self.real),
                        dev.ptr(self.work), self.work_bytes, 0)
         else:
+            self._call("rcabeam_iq_absmax_device", dev.ptr(self.signal), self.code, nv * self.nt,
+                       dev.ptr(self.stats), 0)
             self._call("rcabeam_temporal_gram_device", dev.ptr(self.signal), self.code, nv, self.nt,
                        dev.ptr(self.stats), dev.ptr(self.real), dev.ptr(self.work), self.work_bytes, 0)
         mark("gram")
@@ -803,8 +806,8 @@ def stage_roofs(pipe, stage_ms: dict, das_flops: float) -> dict:
                 peak, form = BF16_PEAK_TFLOPS, "one exact 16-bit MFMA product"
             entry.update(bound="mfma", flops=flops, achieved_TFLOPs=flops / (ms * 1e9), peak_TFLOPs=peak,
                          frac=flops / (ms * 1e9) / peak, bytes=nv * nt * sample,
-                         note=f"4 (T + 1) flop per voxel-frame (upper triangle of the complex Gram); {form}; "
-                         "includes the |max| pass")
+                         note=f"4 (T + 1) flop per voxel-frame (upper triangle of the complex Gram); {form}"
+                         + ("" if pipe.exact_gram else "; includes the |max| pass"))
         elif name == "evd":
             entry.update(bound="latency", note="200 x 200 complex128 Hermitian EVD (rocSOLVER zheevd) and "
                          "the T x T Hermitian assembly")
