@@ -286,10 +286,13 @@ rcabeam_status rcabeam_temporal_gram_f64_noalloc_device(const void* iq, size_t r
  * frames x frames) from the real Gram above (reference gpu_clutter.py:160-161). */
 rcabeam_status rcabeam_gram_hermitian_device(const double* real_gram, size_t frames, double scale2, void* gram);
 
-/* Hermitian eigendecomposition in float64 on the GPU (rocSOLVER zheevd):
- * eigenvalues ascending in values[frames], eigenvectors overwrite gram
- * (columns).  workspace_e: frames doubles; info: one device int (0 = ok).
- * The solver handle is created once per device on first use (the only
+/* Hermitian eigendecomposition in float64 on the GPU: eigenvalues ascending
+ * in values[frames], eigenvectors overwrite gram (columns).  Only the upper
+ * triangle and the real part of the diagonal are read.  frames <= 256: a
+ * one-workgroup Householder reduction to a real tridiagonal matrix (the
+ * reflectors accumulated in place) and rocSOLVER zstedc; larger: rocSOLVER
+ * zheevd.  workspace_e: frames doubles; info: one device int (0 = ok).  The
+ * solver handle is created once per device on first use (the only
  * allocation in this ABI). */
 rcabeam_status rcabeam_eigh_device(void* gram, double* values, double* workspace_e, int* info, size_t frames);
 

@@ -721,6 +721,235 @@ static hipError_t gram64_launch(const float2* iq, size_t rows, size_t frames, co
   return hipGetLastError();
 }
 
+// ---- Hermitian EVD, T <= kEvdMaxN: one-workgroup tridiagonalization ---------
+// rocSOLVER's zheevd at T = 200 spends 3.7 of its 5.6 ms in zhetrd, ~800
+// launches of a few microseconds each.  k_hetrd_q does the whole reduction in
+// one workgroup (the matrix stays in L2): Householder reflectors
+// H_i = I - tau_i u_i u_i^H (u_i[i + 1] = 1, u_i[i + 2 ..] below the
+// subdiagonal) take the lower triangle to a real tridiagonal T = Q^H A Q with
+// Q = H_0 H_1 ... H_{n-2} (the steps of LAPACK's zhetd2 'L' with zlarfg's
+// choice of beta and tau); then Q is accumulated in place from the last
+// reflector to the first (as zung2l/zungtr do), so rocSOLVER's zstedc
+// ('original' vectors) returns the eigenvectors of A.  The input's upper
+// triangle is the one referenced (the lower is rebuilt from it first), as
+// with zheevd(fill_upper).
+constexpr int kEvdMaxN = 256;
+constexpr int kEvdThreads = 1024;
+constexpr int kEvdWaves = kEvdThreads / kWave;
+
+__device__ __forceinline__ double2 zmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 zmulc(double2 a, double2 b) {  // conj(a) * b
+  return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ double2 zadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 zsub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+
+// The lower triangle from the upper one and a real diagonal (rocSOLVER's
+// zheevd reads both, and k_hetrd_q does the same in its prologue).
+__global__ void k_hermitian_from_upper(double2* __restrict__ a, int n) {
+  const long long idx = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (idx >= static_cast<long long>(n) * n) return;
+  const int r = static_cast<int>(idx % n), c = static_cast<int>(idx / n);
+  if (r > c) {
+    const double2 u = a[c + static_cast<size_t>(r) * n];
+    a[idx] = make_double2(u.x, -u.y);
+  } else if (r == c) {
+    a[idx].y = 0.0;
+  }
+}
+
+__global__ __launch_bounds__(kEvdThreads) void k_hetrd_q(double2* __restrict__ a, int n, double* __restrict__ d,
+                                                        double* __restrict__ e) {
+  __shared__ double2 v[kEvdMaxN], w[kEvdMaxN], taus[kEvdMaxN];
+  __shared__ double2 part[kEvdWaves][kEvdMaxN];
+  __shared__ double red[2][kEvdWaves][2];
+  const int tid = static_cast<int>(threadIdx.x), lane = tid & (kWave - 1), wv = tid / kWave;
+  int rb = 0;
+  // Workgroup sum of (x, y).  Two alternating buffers: a buffer is written
+  // again only after the next sum's barrier, which every reader has passed.
+  auto block_sum = [&](double x, double y) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+      x += __shfl_xor(x, off, kWave);
+      y += __shfl_xor(y, off, kWave);
+    }
+    if (lane == 0) {
+      red[rb][wv][0] = x;
+      red[rb][wv][1] = y;
+    }
+    __syncthreads();
+    double sx = 0.0, sy = 0.0;
+#pragma unroll
+    for (int k = 0; k < kEvdWaves; ++k) {
+      sx += red[rb][k][0];
+      sy += red[rb][k][1];
+    }
+    rb ^= 1;
+    return make_double2(sx, sy);
+  };
+  const size_t nn = static_cast<size_t>(n);
+  // Lower triangle from the upper one; real diagonal.
+  for (int idx = tid; idx < n * n; idx += kEvdThreads) {
+    const int r = idx % n, c = idx / n;
+    if (r > c) {
+      const double2 u = a[c + r * nn];
+      a[idx] = make_double2(u.x, -u.y);
+    } else if (r == c) {
+      a[idx].y = 0.0;
+    }
+  }
+  __syncthreads();
+
+  // ---- reduction to tridiagonal form
+  for (int i = 0; i + 1 < n; ++i) {
+    const int m = n - 1 - i;  // length of the column below the diagonal
+    double2* colm = a + static_cast<size_t>(i) * nn + i + 1;
+    double2 x = make_double2(0.0, 0.0);
+    double nrm = 0.0;
+    if (tid < m) {
+      x = colm[tid];
+      if (tid > 0) nrm = x.x * x.x + x.y * x.y;
+    }
+    const double2 alpha = colm[0];
+    const double xnorm2 = block_sum(nrm, 0.0).x;
+    double2 tau = make_double2(0.0, 0.0), scale = make_double2(0.0, 0.0);
+    double beta = alpha.x;
+    if (xnorm2 != 0.0 || alpha.y != 0.0) {
+      const double r = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xnorm2);
+      beta = alpha.x >= 0.0 ? -r : r;
+      tau = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
+      const double2 den = make_double2(alpha.x - beta, alpha.y);  // scale = 1 / (alpha - beta)
+      const double dd = den.x * den.x + den.y * den.y;
+      scale = make_double2(den.x / dd, -den.y / dd);
+    }
+    const bool reflect = tau.x != 0.0 || tau.y != 0.0;
+    if (tid < m) {
+      const double2 vj = tid == 0 ? make_double2(1.0, 0.0) : (reflect ? zmul(x, scale) : x);
+      v[tid] = vj;
+      if (tid > 0) colm[tid] = vj;
+    }
+    if (tid == 0) {
+      e[i] = beta;
+      d[i] = a[static_cast<size_t>(i) * nn + i].x;
+      taus[i] = tau;
+    }
+    __syncthreads();
+    if (reflect) {
+      // p = tau A22 v (A22 = rows and columns i + 1 .., both triangles kept).
+      double2 acc[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc[q] = make_double2(0.0, 0.0);
+      for (int c = wv; c < m; c += kEvdWaves) {
+        const double2 vc = v[c];
+        const double2* colc = a + static_cast<size_t>(i + 1 + c) * nn + i + 1;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int r = lane + kWave * q;
+          if (r < m) acc[q] = zadd(acc[q], zmul(colc[r], vc));
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int r = lane + kWave * q;
+        if (r < m) part[wv][r] = acc[q];
+      }
+      __syncthreads();
+      double2 pj = make_double2(0.0, 0.0), vj = make_double2(0.0, 0.0);
+      if (tid < m) {
+        double2 sum = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int k = 0; k < kEvdWaves; ++k) sum = zadd(sum, part[k][tid]);
+        pj = zmul(tau, sum);
+        vj = v[tid];
+      }
+      // w = p - (tau / 2) (p^H v) v
+      const double2 pv = zmulc(pj, vj);
+      const double2 dot = block_sum(pv.x, pv.y);
+      const double2 alpha2 = zmul(make_double2(-0.5 * tau.x, -0.5 * tau.y), dot);
+      if (tid < m) w[tid] = zadd(pj, zmul(alpha2, vj));
+      __syncthreads();
+      // A22 -= v w^H + w v^H
+      double2 vr[4], wr[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int r = lane + kWave * q;
+        vr[q] = r < m ? v[r] : make_double2(0.0, 0.0);
+        wr[q] = r < m ? w[r] : make_double2(0.0, 0.0);
+      }
+      for (int c = wv; c < m; c += kEvdWaves) {
+        const double2 vc = v[c], wc = w[c];
+        double2* colc = a + static_cast<size_t>(i + 1 + c) * nn + i + 1;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int r = lane + kWave * q;
+          if (r < m) {
+            // v_r conj(w_c) + w_r conj(v_c)
+            const double2 t = zadd(zmulc(wc, vr[q]), zmulc(vc, wr[q]));
+            colc[r] = zsub(colc[r], t);
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0) d[n - 1] = a[static_cast<size_t>(n - 1) * nn + n - 1].x;
+  __syncthreads();
+
+  // ---- Q = H_0 ... H_{n-2} in place, last reflector first: column i + 1 of
+  // Q is H_i e_{i+1}; columns i + 2 .. (already formed, zero in row i + 1)
+  // take H_i from the left.
+  for (int i = n - 2; i >= 0; --i) {
+    const int m = n - 1 - i;
+    const double2 tau = taus[i];
+    const double2* colm = a + static_cast<size_t>(i) * nn + i + 1;
+    if (tid < m) v[tid] = tid == 0 ? make_double2(1.0, 0.0) : colm[tid];
+    __syncthreads();
+    if (tau.x != 0.0 || tau.y != 0.0) {
+      double2 ur[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int r = lane + kWave * q;
+        ur[q] = r < m ? v[r] : make_double2(0.0, 0.0);
+      }
+      for (int c = i + 2 + wv; c < n; c += kEvdWaves) {
+        double2* colc = a + static_cast<size_t>(c) * nn + i + 1;
+        double2 t[4];
+        double sx = 0.0, sy = 0.0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int r = lane + kWave * q;
+          t[q] = r < m ? colc[r] : make_double2(0.0, 0.0);
+          const double2 pr = zmulc(ur[q], t[q]);
+          sx += pr.x;
+          sy += pr.y;
+        }
+#pragma unroll
+        for (int off = kWave / 2; off > 0; off >>= 1) {
+          sx += __shfl_xor(sx, off, kWave);
+          sy += __shfl_xor(sy, off, kWave);
+        }
+        const double2 ts = zmul(tau, make_double2(sx, sy));
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int r = lane + kWave * q;
+          if (r < m) colc[r] = zsub(t[q], zmul(ur[q], ts));
+        }
+      }
+    }
+    double2* colq = a + static_cast<size_t>(i + 1) * nn;
+    for (int r = tid; r < n; r += kEvdThreads) {
+      double2 val = make_double2(0.0, 0.0);
+      if (r == i + 1) val = make_double2(1.0 - tau.x, -tau.y);
+      else if (r > i + 1) val = zmul(make_double2(-tau.x, -tau.y), v[r - i - 1]);
+      colq[r] = val;
+    }
+    __syncthreads();
+  }
+  for (int r = tid; r < n; r += kEvdThreads) a[r] = make_double2(r == 0 ? 1.0 : 0.0, 0.0);
+}
+
 struct SolverState {
   std::mutex lock;
   rocblas_handle handle[64] = {};
@@ -829,9 +1058,20 @@ extern "C" rcabeam_status rcabeam_eigh_device(void* gram, double* values, double
   rocblas_handle handle = solver_handle();
   if (!handle) return RCABEAM_ERROR_CUDA;
   const rocblas_int n = static_cast<rocblas_int>(frames);
-  const rocblas_status s =
-      rocsolver_zheevd(handle, rocblas_evect_original, rocblas_fill_upper, n,
-                       static_cast<rocblas_double_complex*>(gram), n, values, workspace_e, info);
+  rocblas_status s;
+  if (frames <= static_cast<size_t>(kEvdMaxN)) {
+    k_hetrd_q<<<1, kEvdThreads>>>(static_cast<double2*>(gram), static_cast<int>(frames), values, workspace_e);
+    if (hipGetLastError() != hipSuccess) return RCABEAM_ERROR_CUDA;
+    s = rocsolver_zstedc(handle, rocblas_evect_original, n, values, workspace_e,
+                         static_cast<rocblas_double_complex*>(gram), n, info);
+  } else {
+    const long long count = static_cast<long long>(frames) * static_cast<long long>(frames);
+    if ((count + 255) / 256 > 0xffffffffll) return RCABEAM_ERROR_ARGUMENT;
+    k_hermitian_from_upper<<<static_cast<unsigned>((count + 255) / 256), 256>>>(static_cast<double2*>(gram), n);
+    if (hipGetLastError() != hipSuccess) return RCABEAM_ERROR_CUDA;
+    s = rocsolver_zheevd(handle, rocblas_evect_original, rocblas_fill_upper, n,
+                         static_cast<rocblas_double_complex*>(gram), n, values, workspace_e, info);
+  }
   if (s == rocblas_status_invalid_size || s == rocblas_status_invalid_pointer) return RCABEAM_ERROR_ARGUMENT;
   return s == rocblas_status_success ? RCABEAM_SUCCESS : RCABEAM_ERROR_CUDA;
 }
