@@ -809,8 +809,8 @@ def stage_roofs(pipe, stage_ms: dict, das_flops: float) -> dict:
                          note=f"4 (T + 1) flop per voxel-frame (upper triangle of the complex Gram); {form}"
                          + ("" if pipe.exact_gram else "; includes the |max| pass"))
         elif name == "evd":
-            entry.update(bound="latency", note="200 x 200 complex128 Hermitian EVD (rocSOLVER zheevd) and "
-                         "the T x T Hermitian assembly")
+            entry.update(bound="latency", note="200 x 200 complex128 Hermitian EVD (one-workgroup Householder "
+                         "tridiagonalization + rocSOLVER zstedc) and the T x T Hermitian assembly")
         elif name == "project":
             entry.update(hbm(nv * nt * sample + nv * 8, ms))
             entry["note"] = "IQ read once, float64 filtered power written once (3 discarded vectors)"
